@@ -1,0 +1,278 @@
+// C ABI of libctdm: argument checking, status conversion and the context object. The compute lives
+// in the k_*.hip modules; this file never launches kernels itself.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "internal.hpp"
+
+struct ctdm_ctx {
+    ctdm::Ctx c;
+};
+
+namespace ctdm {
+
+const float2* Ctx::tw(int n)
+{
+    auto it = twiddles.find(n);
+    if (it != twiddles.end()) return it->second;
+    CTDM_REQUIRE(!capturing, "twiddle table requested during graph capture");
+    std::vector<float2> h(n);
+    for (int m = 0; m < n; ++m) {
+        const double a = -2.0 * 3.14159265358979323846 * (double)m / (double)n;
+        h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    float2* d = nullptr;
+    CTDM_HIP(hipMalloc(&d, sizeof(float2) * n));
+    CTDM_HIP(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    twiddles[n] = d;
+    return d;
+}
+
+void* Ctx::buf(const std::string& name, size_t bytes)
+{
+    DevBuf& b = scratch[name];
+    if (b.bytes >= bytes && b.ptr) return b.ptr;
+    CTDM_REQUIRE(!capturing, "scratch buffer '" + name + "' grown during graph capture");
+    if (b.ptr) {
+        CTDM_HIP(hipStreamSynchronize(stream));
+        CTDM_HIP(hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    CTDM_HIP(hipMalloc(&b.ptr, bytes ? bytes : 1));
+    b.bytes = bytes;
+    return b.ptr;
+}
+
+}  // namespace ctdm
+
+namespace {
+
+template <class F>
+ctdm_status guarded(ctdm_ctx* ctx, F&& f)
+{
+    if (!ctx) return CTDM_EINVAL;
+    try {
+        if (hipSetDevice(ctx->c.device) != hipSuccess) ctdm::fail(CTDM_EDEVICE, "hipSetDevice failed");
+        f(ctx->c);
+        return CTDM_OK;
+    } catch (const ctdm::Error& e) {
+        ctx->c.last_error = e.msg;
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        ctx->c.last_error = "host allocation failed";
+        return CTDM_ENOMEM;
+    } catch (...) {
+        ctx->c.last_error = "unknown error";
+        return CTDM_EDEVICE;
+    }
+}
+
+void check_optics(const ctdm_optics* o)
+{
+    CTDM_REQUIRE(o, "optics: null pointer");
+    CTDM_REQUIRE(o->wavelength > 0 && o->n_med > 0 && o->dx > 0, "optics: non-positive parameter");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ctdm_version(void) { return "ctdm-mi355x 0.1 (gfx950)"; }
+
+ctdm_status ctdm_create(ctdm_ctx** out, const ctdm_config* cfg, ctdm_stream stream)
+{
+    if (!out) return CTDM_EINVAL;
+    *out = nullptr;
+    ctdm_ctx* ctx = new (std::nothrow) ctdm_ctx();
+    if (!ctx) return CTDM_ENOMEM;
+    ctx->c.device = cfg ? cfg->device : 0;
+    ctx->c.stream = (hipStream_t)stream;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || ctx->c.device < 0 || ctx->c.device >= n) {
+        delete ctx;
+        return CTDM_EDEVICE;
+    }
+    if (hipSetDevice(ctx->c.device) != hipSuccess) {
+        delete ctx;
+        return CTDM_EDEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->c.device) == hipSuccess)
+        ctx->c.max_lds = (int)prop.sharedMemPerBlock;
+    *out = ctx;
+    return CTDM_OK;
+}
+
+ctdm_status ctdm_destroy(ctdm_ctx* ctx)
+{
+    if (!ctx) return CTDM_EINVAL;
+    hipSetDevice(ctx->c.device);
+    hipStreamSynchronize(ctx->c.stream);
+    for (auto& kv : ctx->c.graphs) {
+        if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) hipGraphDestroy(kv.second.graph);
+    }
+    for (auto& kv : ctx->c.twiddles) hipFree(kv.second);
+    for (auto& kv : ctx->c.scratch)
+        if (kv.second.ptr) hipFree(kv.second.ptr);
+    delete ctx;
+    return CTDM_OK;
+}
+
+ctdm_status ctdm_set_stream(ctdm_ctx* ctx, ctdm_stream stream)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        CTDM_HIP(hipStreamSynchronize(c.stream));
+        c.stream = (hipStream_t)stream;
+        for (auto& kv : c.graphs) {  // graphs were captured on the previous stream's pointers only
+            (void)kv;
+        }
+    });
+}
+
+ctdm_status ctdm_sync(ctdm_ctx* ctx)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) { CTDM_HIP(hipStreamSynchronize(c.stream)); });
+}
+
+const char* ctdm_last_error(const ctdm_ctx* ctx)
+{
+    if (!ctx) return "null context";
+    return ctx->c.last_error.c_str();
+}
+
+ctdm_status ctdm_find_sideband(ctdm_ctx* ctx, const float* bg, int H, int W, int M, int row_start,
+                               ctdm_crop* out)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) { ctdm::find_sideband(c, bg, H, W, M, row_start, out); });
+}
+
+ctdm_status ctdm_retrieve(ctdm_ctx* ctx, const float* holo, const float* bg, int batch, int H, int W,
+                          const ctdm_crop* crop, float* wrapped, float* logamp, ctdm_complex* field)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        CTDM_REQUIRE(crop, "retrieve: null crop");
+        ctdm::retrieve_batched(c, holo, bg, batch, H, W, *crop, wrapped, logamp, (float2*)field);
+    });
+}
+
+ctdm_status ctdm_unwrap_dct(ctdm_ctx* ctx, const float* wrapped, float* unwrapped, int batch, int M,
+                            unsigned flags)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        if (flags & CTDM_EXEC_PLAIN) {
+            for (int b = 0; b < batch; ++b)
+                ctdm::unwrap_dct_plain_one(c, wrapped + (size_t)b * M * M,
+                                           unwrapped + (size_t)b * M * M, M, flags & 0xffu);
+        } else {
+            ctdm::unwrap_dct_batched(c, wrapped, unwrapped, batch, M, flags);
+        }
+    });
+}
+
+ctdm_status ctdm_unwrap_dct_adjoint(ctdm_ctx* ctx, const float* wrapped, const float* g_out,
+                                    float* g_in, int batch, int M, unsigned flags)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        ctdm::unwrap_dct_adjoint(c, wrapped, g_out, g_in, batch, M, flags);
+    });
+}
+
+ctdm_status ctdm_goldstein(ctdm_ctx* ctx, const float* wrapped, int batch, int H, int W,
+                           int8_t* residues, uint8_t* cuts, int32_t* k2pi, float* unwrapped)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        ctdm::goldstein(c, wrapped, batch, H, W, residues, cuts, k2pi, unwrapped);
+    });
+}
+
+ctdm_status ctdm_rytov_spectrum(ctdm_ctx* ctx, const float* unwrapped, const float* logamp, int A,
+                                int M, ctdm_complex* spec)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        ctdm::rytov_spectrum(c, unwrapped, logamp, A, M, (float2*)spec);
+    });
+}
+
+ctdm_status ctdm_ewald_scatter(ctdm_ctx* ctx, const ctdm_complex* spec, const ctdm_angle* angles,
+                               int A, int M, int V, const ctdm_optics* optics, ctdm_complex* F,
+                               float* hits)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        check_optics(optics);
+        ctdm::ewald_scatter(c, (const float2*)spec, angles, A, M, V, *optics, (float2*)F, hits);
+    });
+}
+
+ctdm_status ctdm_ewald_gather(ctdm_ctx* ctx, const ctdm_complex* F, const ctdm_angle* angles, int A,
+                              int M, int V, const ctdm_optics* optics, ctdm_complex* spec)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        check_optics(optics);
+        ctdm::ewald_gather(c, (const float2*)F, angles, A, M, V, *optics, (float2*)spec);
+    });
+}
+
+ctdm_status ctdm_normalize_measured(ctdm_ctx* ctx, ctdm_complex* F, const float* hits,
+                                    ctdm_complex* F_meas, int V, double dx)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        ctdm::normalize_measured(c, (float2*)F, hits, (float2*)F_meas, V, dx);
+    });
+}
+
+ctdm_status ctdm_fft3d(ctdm_ctx* ctx, ctdm_complex* vol, int V, int dir)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) { ctdm::fft3d(c, (float2*)vol, V, dir); });
+}
+
+ctdm_status ctdm_positivity(ctdm_ctx* ctx, ctdm_complex* f, const ctdm_complex* F_meas,
+                            const float* hits, ctdm_complex* n, int V, int iters,
+                            const ctdm_optics* optics, unsigned flags)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        check_optics(optics);
+        if (flags & CTDM_EXEC_PLAIN)
+            ctdm::positivity_plain(c, (float2*)f, (const float2*)F_meas, hits, (float2*)n, V, iters,
+                                   *optics, flags);
+        else
+            ctdm::positivity_fused(c, (float2*)f, (const float2*)F_meas, hits, (float2*)n, V, iters,
+                                   *optics, flags);
+    });
+}
+
+ctdm_status ctdm_reconstruct(ctdm_ctx* ctx, ctdm_cell* cells, int ncells, const ctdm_batch_opts* opts)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        CTDM_REQUIRE(cells || ncells == 0, "reconstruct: null cells");
+        ctdm_batch_opts o{};
+        if (opts) o = *opts;
+        ctdm::reconstruct(c, cells, ncells, o);
+    });
+}
+
+ctdm_status ctdm_autotune(ctdm_ctx* ctx, const int* sizes, int nsizes, char* report, size_t len)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        const std::string r = ctdm::autotune(c, sizes, nsizes);
+        if (report && len) {
+            std::strncpy(report, r.c_str(), len - 1);
+            report[len - 1] = 0;
+        }
+    });
+}
+
+ctdm_status ctdm_fit_pgd(ctdm_ctx* ctx, ctdm_complex* f, const ctdm_complex* spec,
+                         const ctdm_angle* angles, int A, int V, const ctdm_optics* optics, int iters,
+                         double step, ctdm_complex* n, unsigned flags)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        check_optics(optics);
+        ctdm::fit_pgd(c, (float2*)f, (const float2*)spec, angles, A, V, *optics, iters, step,
+                      (float2*)n, flags);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,264 @@
+// Device-side FFT building blocks for gfx950: register DFT butterflies and an LDS-staged Stockham
+// auto-sort transform executed cooperatively by T threads per transform.
+//
+// Convention: DIR = -1 is the forward transform exp(-2*pi*i*j*k/N) (cuFFT CUFFT_FORWARD), DIR = +1
+// the unnormalised inverse exp(+2*pi*i*j*k/N) (CUFFT_INVERSE). Scaling is applied by the callers'
+// epilogues (the reference scales in separate kernels, FFT.cu:853-916).
+//
+// Stockham stage (radix R, L = product of the radices already applied, S = N/R slots):
+//   slot j, k = j mod L:  a_r = x[j + r*S] * W_{L*R}^{DIR*r*k},  a = DFT_R(a),
+//                         y[(j - k)*R + k + r*L] = a_r
+// Applying the stages for radices R0, R1, ... in order yields the DFT in natural order.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ctdm {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b)  // a * conj(b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// Multiply by DIR*i: DIR = -1 -> -i, DIR = +1 -> +i.
+template <int DIR>
+__device__ __forceinline__ float2 mul_i(float2 a)
+{
+    return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// cos / sin of 2*pi*k/32, k = 0..31 (exact to double rounding).
+struct Tw32 {
+    static constexpr double c[32] = {
+        1.0, 0.98078528040323043, 0.92387953251128674, 0.83146961230254524, 0.70710678118654757,
+        0.55557023301960218, 0.38268343236508984, 0.19509032201612833, 0.0, -0.19509032201612819,
+        -0.38268343236508973, -0.55557023301960196, -0.70710678118654746, -0.83146961230254524,
+        -0.92387953251128674, -0.98078528040323043, -1.0, -0.98078528040323043, -0.92387953251128674,
+        -0.83146961230254546, -0.70710678118654768, -0.55557023301960218, -0.38268343236509034,
+        -0.19509032201612866, 0.0, 0.19509032201612803, 0.38268343236509000, 0.55557023301960184,
+        0.70710678118654735, 0.83146961230254524, 0.92387953251128652, 0.98078528040323043};
+};
+
+// In-register DFT of length R (power of two up to 32, or 3), direction DIR.
+template <int R, int DIR>
+struct DFT;
+
+template <int DIR>
+struct DFT<1, DIR> {
+    __device__ __forceinline__ static void run(float2*) {}
+};
+
+template <int DIR>
+struct DFT<2, DIR> {
+    __device__ __forceinline__ static void run(float2* a)
+    {
+        const float2 t = a[1];
+        a[1] = csub(a[0], t);
+        a[0] = cadd(a[0], t);
+    }
+};
+
+template <int DIR>
+struct DFT<3, DIR> {
+    __device__ __forceinline__ static void run(float2* a)
+    {
+        // W = exp(DIR*2*pi*i/3) = -1/2 + DIR*i*sqrt(3)/2
+        const float h = 0.86602540378443865f;
+        const float2 t = cadd(a[1], a[2]);
+        const float2 d = csub(a[1], a[2]);
+        const float2 m = make_float2(a[0].x - 0.5f * t.x, a[0].y - 0.5f * t.y);
+        const float2 r = mul_i<DIR>(cscale(d, h));
+        a[0] = cadd(a[0], t);
+        a[1] = cadd(m, r);
+        a[2] = csub(m, r);
+    }
+};
+
+template <int DIR>
+struct DFT<4, DIR> {
+    __device__ __forceinline__ static void run(float2* a)
+    {
+        const float2 t0 = cadd(a[0], a[2]);
+        const float2 t1 = csub(a[0], a[2]);
+        const float2 t2 = cadd(a[1], a[3]);
+        const float2 t3 = mul_i<DIR>(csub(a[1], a[3]));
+        a[0] = cadd(t0, t2);
+        a[2] = csub(t0, t2);
+        a[1] = cadd(t1, t3);
+        a[3] = csub(t1, t3);
+    }
+};
+
+// Radix-2 decimation in time over two half-length register DFTs (R = 8, 16, 32).
+template <int R, int DIR>
+struct DFT {
+    static_assert(R % 2 == 0 && R <= 32, "register DFT supports 2^k <= 32 and 3");
+    __device__ __forceinline__ static void run(float2* a)
+    {
+        float2 e[R / 2], o[R / 2];
+#pragma unroll
+        for (int k = 0; k < R / 2; ++k) {
+            e[k] = a[2 * k];
+            o[k] = a[2 * k + 1];
+        }
+        DFT<R / 2, DIR>::run(e);
+        DFT<R / 2, DIR>::run(o);
+#pragma unroll
+        for (int k = 0; k < R / 2; ++k) {
+            float2 t;
+            if (k == 0) {
+                t = o[k];
+            } else if (4 * k == R) {
+                t = mul_i<DIR>(o[k]);
+            } else {
+                const int m = k * (32 / R);
+                const float c = (float)Tw32::c[m];
+                const float s = (float)(DIR * Tw32::c[(m + 24) & 31]);  // sin(x) = cos(x - pi/2)
+                t = cmul(o[k], make_float2(c, s));
+            }
+            a[k] = cadd(e[k], t);
+            a[k + R / 2] = csub(e[k], t);
+        }
+    }
+};
+
+// LDS index with one float2 of padding every 16 elements (spreads stride-16 accesses over banks).
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+template <int N>
+struct LdsSize {
+    static constexpr int value = N + N / 16;
+};
+
+// Twiddle from a per-N table tw[m] = exp(-2*pi*i*m/N) (float2, computed in double on the host).
+template <int DIR>
+__device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m)
+{
+    const float2 w = tw[m];
+    return DIR > 0 ? make_float2(w.x, -w.y) : w;
+}
+
+// Cooperative N-point transform by T threads of one transform; threads with t >= T only take part
+// in the barriers (so a workgroup sized for a longer transform can run a shorter one).
+// gload(i) supplies input element i (called in the first stage), gstore(i, v) receives output
+// element i (called in the last stage); intermediate stages exchange through the LDS area `s`
+// (LdsSize<N>::value float2). Every thread of the workgroup must call this with the same template
+// arguments: it contains __syncthreads().
+template <int N, int T, int DIR, int L, int R, int... Rest>
+struct Stockham {
+    template <class GLoad, class GStore>
+    __device__ __forceinline__ static void run(int t, float2* s, const float2* __restrict__ tw,
+                                               GLoad& gload, GStore& gstore)
+    {
+        constexpr int S = N / R;
+        constexpr int P = (S + T - 1) / T;
+        constexpr bool FIRST = (L == 1);
+        constexpr bool LAST = (sizeof...(Rest) == 0);
+        float2 v[P][R];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const int j = t + p * T;
+            if (t < T && (S % T == 0 || j < S)) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if constexpr (FIRST)
+                        v[p][r] = gload(j + r * S);
+                    else
+                        v[p][r] = s[lds_pad(j + r * S)];
+                }
+            }
+        }
+        __syncthreads();  // every read of this stage precedes any write into `s`
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const int j = t + p * T;
+            if (t < T && (S % T == 0 || j < S)) {
+                const int k = j % L;
+                if constexpr (L > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; ++r)
+                        v[p][r] = cmul(v[p][r], twiddle<DIR>(tw, r * k * (N / (L * R))));
+                }
+                DFT<R, DIR>::run(v[p]);
+                const int base = (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if constexpr (LAST)
+                        gstore(base + r * L, v[p][r]);
+                    else
+                        s[lds_pad(base + r * L)] = v[p][r];
+                }
+            }
+        }
+        if constexpr (!LAST) {
+            __syncthreads();
+            Stockham<N, T, DIR, L * R, Rest...>::run(t, s, tw, gload, gstore);
+        }
+    }
+};
+
+// Radix plans per transform length. T = threads cooperating on one transform.
+template <int N>
+struct Plan;
+template <>
+struct Plan<64> {
+    static constexpr int T = 8;
+    template <int DIR, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    {
+        Stockham<64, T, DIR, 1, 8, 8>::run(t, s, tw, gl, gs);
+    }
+};
+template <>
+struct Plan<128> {
+    static constexpr int T = 16;
+    template <int DIR, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    {
+        Stockham<128, T, DIR, 1, 8, 16>::run(t, s, tw, gl, gs);
+    }
+};
+template <>
+struct Plan<192> {
+    static constexpr int T = 16;
+    template <int DIR, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    {
+        Stockham<192, T, DIR, 1, 3, 4, 16>::run(t, s, tw, gl, gs);
+    }
+};
+template <>
+struct Plan<256> {
+    static constexpr int T = 16;
+    template <int DIR, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    {
+        Stockham<256, T, DIR, 1, 16, 16>::run(t, s, tw, gl, gs);
+    }
+};
+template <>
+struct Plan<512> {
+    static constexpr int T = 32;
+    template <int DIR, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    {
+        Stockham<512, T, DIR, 1, 16, 32>::run(t, s, tw, gl, gs);
+    }
+};
+template <>
+struct Plan<1024> {
+    static constexpr int T = 64;
+    template <int DIR, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    {
+        Stockham<1024, T, DIR, 1, 16, 16, 4>::run(t, s, tw, gl, gs);
+    }
+};
+
+}  // namespace ctdm

@@ -1,0 +1,154 @@
+// Host-side internals shared by the libctdm translation units: the context object, status
+// plumbing, twiddle tables and the kernel-launch entry points each module exports to the API layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/ctdm.h"
+
+namespace ctdm {
+
+// Thrown inside the library, converted to a ctdm_status at the C boundary (never crosses it).
+struct Error {
+    ctdm_status status;
+    std::string msg;
+};
+
+[[noreturn]] inline void fail(ctdm_status s, const std::string& msg) { throw Error{s, msg}; }
+
+#define CTDM_HIP(call)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            ::ctdm::fail(e_ == hipErrorOutOfMemory ? CTDM_ENOMEM : CTDM_EDEVICE,                \
+                         std::string(#call) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+#define CTDM_CHECK_LAUNCH(name)                                                                 \
+    do {                                                                                        \
+        hipError_t e_ = hipGetLastError();                                                      \
+        if (e_ != hipSuccess)                                                                   \
+            ::ctdm::fail(CTDM_EDEVICE, std::string("launch ") + (name) + ": " +                 \
+                                           hipGetErrorString(e_));                              \
+    } while (0)
+
+#define CTDM_REQUIRE(cond, msg)                                                                 \
+    do {                                                                                        \
+        if (!(cond)) ::ctdm::fail(CTDM_EINVAL, msg);                                            \
+    } while (0)
+
+// Device buffer owned by the context; grows on demand (never inside graph capture).
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct Ctx;
+
+// Per-size kernel choices picked by the autotuner (defaults are the untuned picks).
+struct Tune {
+    int vol_pencils = 16;  // pencils per workgroup in the volume passes (8 or 16)
+    int grid_block = 256;  // threads per workgroup of the pencil gridding kernel
+    bool tuned = false;
+};
+
+struct GraphKey {
+    std::vector<uintptr_t> words;
+    bool operator<(const GraphKey& o) const { return words < o.words; }
+};
+
+struct GraphEntry {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    std::map<int, float2*> twiddles;   // N -> device table exp(-2*pi*i*m/N), m < N
+    std::map<std::string, DevBuf> scratch;
+    std::map<int, Tune> tune;
+    std::map<GraphKey, GraphEntry> graphs;
+    bool capturing = false;
+    int max_lds = 65536;
+
+    const float2* tw(int n);
+    // Scratch buffer `name` of at least `bytes` bytes (contents undefined).
+    void* buf(const std::string& name, size_t bytes);
+    Tune& tuning(int n) { return tune[n]; }
+};
+
+// ---------------------------------------------------------------- module launchers
+// Retrieval (k_retrieve.hip)
+void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, ctdm_crop* out);
+void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int H, int W,
+                      const ctdm_crop& crop, float* wrapped, float* logamp, float2* field);
+// Reference-structured retrieval of one hologram pair (full 2-D FFTs, shift, crop, inverse,
+// scale+conjugate, calcWrapPhase semantics: phase in (-2pi, 2pi), amplitude ratio).
+void retrieve_plain_one(Ctx& c, const float* holo, const float* bg, int H, int W,
+                        const ctdm_crop& crop, float* phase, float* amp);
+
+// Generic FFT passes (k_fft.hip)
+// Batched 1-D transforms of length n over pencils of a strided array (see k_fft.hip).
+void fft_rows(Ctx& c, float2* data, int n, long long count, int dir, float scale, bool conj_out);
+void fft_cols(Ctx& c, float2* data, int n, int width, long long stride_n, long long nplanes,
+              long long stride_plane, int dir, float scale, bool conj_out);
+void fftshift2d(Ctx& c, float2* data, int W, int H, long long batch);
+void fftshift3d(Ctx& c, float2* data, int V);
+
+// Unwrapping (k_unwrap.hip)
+void unwrap_dct_batched(Ctx& c, const float* wrapped, float* unwrapped, int batch, int M,
+                        unsigned flags);
+void unwrap_dct_plain_one(Ctx& c, const float* wrapped, float* unwrapped, int M, unsigned flags);
+void unwrap_dct_adjoint(Ctx& c, const float* wrapped, const float* g_out, float* g_in, int batch,
+                        int M, unsigned flags);
+
+// Goldstein (k_goldstein.hip)
+void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* res, uint8_t* cuts,
+               int32_t* k2pi, float* unwrapped);
+
+// Reconstruction (k_recon.hip)
+struct EwaldGeom {
+    double fm0, fm02, df, dx, k2, n_med;
+};
+EwaldGeom ewald_geom(const ctdm_optics& o, int V);
+struct AngleRow {
+    double fx0, fy0, fz0;
+};
+std::vector<AngleRow> angle_rows(const EwaldGeom& g, const ctdm_angle* ang, int A);
+
+void rytov_spectrum(Ctx& c, const float* unwrapped, const float* logamp, int A, int M,
+                    float2* spec);
+void rytov_spectrum_plain_one(Ctx& c, const float* phase, const float* amp, int M, float2* u_sp);
+void ewald_scatter(Ctx& c, const float2* spec, const ctdm_angle* ang, int A, int M, int V,
+                   const ctdm_optics& o, float2* F, float* hits);
+void ewald_gather(Ctx& c, const float2* F, const ctdm_angle* ang, int A, int M, int V,
+                  const ctdm_optics& o, float2* spec);
+void normalize_measured(Ctx& c, float2* F, const float* hits, float2* Fm, int V, double dx);
+void fft3d(Ctx& c, float2* vol, int V, int dir);
+void positivity_plain(Ctx& c, float2* f, const float2* Fm, const float* hits, float2* n, int V,
+                      int iters, const ctdm_optics& o, unsigned flags);
+void positivity_fused(Ctx& c, float2* f, const float2* Fm, const float* hits, float2* n, int V,
+                      int iters, const ctdm_optics& o, unsigned flags);
+// Batched cell volume: spectra of all angles -> fused gridding + inverse FFT + positivity loop.
+void volume_batched(Ctx& c, const float2* spec, const ctdm_angle* ang, int A, int V,
+                    const ctdm_optics& o, int iters, unsigned flags, float2* n_out);
+void fit_pgd(Ctx& c, float2* f, const float2* spec, const ctdm_angle* ang, int A, int V,
+             const ctdm_optics& o, int iters, double step, float2* n, unsigned flags);
+
+// Pipeline (pipeline.cpp)
+void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& opts);
+std::string autotune(Ctx& c, const int* sizes, int nsizes);
+
+inline int grid1d(long long n, int block) { return (int)((n + block - 1) / block); }
+
+}  // namespace ctdm

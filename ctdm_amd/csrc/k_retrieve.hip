@@ -1,0 +1,384 @@
+// Off-axis sideband retrieval ("HilbertTransform", Algorithm A) for gfx950.
+//
+// Reference sequence (HilbertTransform.cu:1546-1585, 3566-3586, 3618-3639; FFT.cu:220-236, 853-868):
+//   FFT2(H x W hologram) -> fftshift -> crop M x M around (row, col) keeping a square passband of
+//   half-width `radius` -> ifftshift -> iFFT2 -> x 1/M^2, conjugate -> phase = atan2(BG) - atan2(SP),
+//   amp = |SP| / |BG|.
+// With S the cropped spectrum re-centred on the sideband (p', q' in [-M/2, M/2)), the field is
+//   g(x, y) = sum_{|p'|,|q'| <= radius} F((c0 + p') mod W, (r0 + q') mod H) exp(+2 pi i (p' x + q' y) / M)
+// with c0 = col - W/2, r0 = row - H/2 and F the unshifted forward DFT. The batched path evaluates
+// exactly this with three kernels per batch of hologram pairs:
+//   1. k_row_real: real-input row FFT (W/2-point complex FFT + split), keeps only the passband
+//      columns kx = c0 + p'                        -> R[img][y][p'],  img < 2*batch (SP then BG)
+//   2. k_col_band: H-point column FFT of each kept column, keeps the passband rows, places them at
+//      q' mod M and runs the M-point inverse along y -> G[img][y][p']
+//   3. k_row_final: M-point inverse along x for SP and BG rows, then the fused phase / log-amplitude
+//      (scale 1/M^2 and the conjugation cancel in arg(SP conj BG) and |SP|/|BG|).
+// The fftshifts are index arithmetic; nothing is zero-padded.
+#include "fft_dev.hpp"
+#include "internal.hpp"
+
+#include <cmath>
+
+namespace ctdm {
+
+struct Band {
+    int lo;   // first kept offset (p' or q')
+    int n;    // number of kept offsets
+};
+
+static Band passband(int radius, int M)
+{
+    const int lo = std::max(-M / 2, -radius);
+    const int hi = std::min(M / 2 - 1, radius);
+    return Band{lo, hi - lo + 1};
+}
+
+// ---------------------------------------------------------------- 1. real row FFT, pruned output
+// Each row of W reals is read as W/2 complex z[n] = x[2n] + i x[2n+1]; Z = FFT_{W/2}(z);
+// X[k] = (Z[k] + conj Z[-k]) / 2 - i/2 * exp(-2 pi i k / W) * (Z[k] - conj Z[-k]).
+// Output: out[(img*H + y)*nout + q] = X[(k_lo + q) mod W], q < nout. Images img < n0 come from
+// src0, the others from src1 (sample holograms first, then backgrounds).
+template <int W, int G>
+__global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
+    const float* __restrict__ src0, int n0, const float* __restrict__ src1, int nimg, int H,
+    const float2* __restrict__ tw_half, const float2* __restrict__ tw_full, int k_lo, int nout,
+    float2* __restrict__ out)
+{
+    constexpr int NH = W / 2;
+    constexpr int T = Plan<NH>::T;
+    __shared__ float2 lds[G * LdsSize<NH>::value];
+    const int g = threadIdx.x / T, t = threadIdx.x % T;
+    const long long row = (long long)blockIdx.x * G + g;  // over nimg*H rows
+    const long long nrows = (long long)nimg * H;
+    const bool valid = row < nrows;
+    const long long img = valid ? row / H : 0;
+    const int y = valid ? (int)(row % H) : 0;
+    const float* src = (img < n0 ? src0 + img * (long long)H * W : src1 + (img - n0) * (long long)H * W) + (long long)y * W;
+    const float2* src2 = reinterpret_cast<const float2*>(src);
+    float2* s = lds + g * LdsSize<NH>::value;
+    auto gl = [&](int i) -> float2 { return valid ? src2[i] : make_float2(0.f, 0.f); };
+    auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };
+    Plan<NH>::template run<-1>(t, s, tw_half, gl, gs);
+    __syncthreads();
+    if (!valid) return;
+    float2* dst = out + row * nout;
+    for (int q = t; q < nout; q += T) {
+        int k = (k_lo + q) % W;
+        if (k < 0) k += W;
+        const int ka = k % NH;
+        const int kb = (NH - ka) % NH;
+        const float2 za = s[lds_pad(ka)], zb = s[lds_pad(kb)];
+        const float2 e = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));  // (Z[k] + conj Z[-k]) / 2
+        const float2 o = make_float2(0.5f * (za.y + zb.y), -0.5f * (za.x - zb.x)); // (Z[k] - conj Z[-k]) / (2i)
+        const float2 w = tw_full[k];                                               // exp(-2 pi i k / W)
+        dst[q] = cadd(e, cmul(w, o));
+    }
+}
+
+// ---------------------------------------------------------------- 2. column FFT over the band
+// in: R[img][y][q] (H rows of nin columns), column q <-> p' = p_lo + q.
+// For each column: F(ky) = FFT_H along y; keep rows ky = (r0 + q') mod H for q' in the band,
+// place at index q' mod M of an M-vector (zero elsewhere), inverse M-point FFT ->
+// out: G[img][y'][q], y' < M.
+template <int H, int M, int C>
+__global__ __launch_bounds__(C* Plan<H>::T) void k_col_band(
+    const float2* __restrict__ in, int nimg, int nin, const float2* __restrict__ tw_h,
+    const float2* __restrict__ tw_m, int r0, int q_lo, int q_n, float2* __restrict__ out)
+{
+    constexpr int T = Plan<H>::T;
+    static_assert(LdsSize<M>::value <= LdsSize<H>::value, "band transform must fit the column area");
+    __shared__ float2 lds[C * LdsSize<H>::value];
+    const int cidx = threadIdx.x % C, t = threadIdx.x / C;  // lanes walk columns (coalesced)
+    const int tiles = (nin + C - 1) / C;
+    const int img = blockIdx.x / tiles;
+    const int q = (blockIdx.x % tiles) * C + cidx;
+    const bool valid = q < nin && img < nimg;
+    const float2* col = in + (long long)img * H * nin + (valid ? q : 0);
+    float2* s = lds + cidx * LdsSize<H>::value;
+    auto gl = [&](int i) -> float2 { return valid ? col[(long long)i * nin] : make_float2(0.f, 0.f); };
+    auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };
+    Plan<H>::template run<-1>(t, s, tw_h, gl, gs);
+    __syncthreads();
+    // Inverse M-point transform along y of the band (ifftshift folded into the index).
+    auto gl2 = [&](int i) -> float2 {
+        const int qq = i < M / 2 ? i : i - M;  // q'
+        if (qq < q_lo || qq >= q_lo + q_n) return make_float2(0.f, 0.f);
+        int ky = (r0 + qq) % H;
+        if (ky < 0) ky += H;
+        return s[lds_pad(ky)];
+    };
+    float2* dst = out + (long long)img * M * nin + (valid ? q : 0);
+    auto gs2 = [&](int i, float2 v) {
+        if (valid) dst[(long long)i * nin] = v;
+    };
+    Plan<M>::template run<+1>(t, s, tw_m, gl2, gs2);
+}
+
+// ---------------------------------------------------------------- 3. final rows + phase / log-amp
+// in: G[img][y][q] for img < 2*batch (SP items first). Each workgroup does GR rows of one pair:
+// inverse M-point along x of SP and BG rows, then
+//   z = g_SP * conj(g_BG): wrapped = atan2(Im z, Re z), logamp = (ln|g_SP|^2 - ln|g_BG|^2) / 2,
+//   field = g_SP / g_BG (optional).
+template <int M, int GR>
+__global__ __launch_bounds__(2 * GR * Plan<M>::T) void k_row_final(
+    const float2* __restrict__ in, int batch, int nin, int p_lo, const float2* __restrict__ tw_m,
+    float* __restrict__ wrapped, float* __restrict__ logamp, float2* __restrict__ field)
+{
+    constexpr int T = Plan<M>::T;
+    __shared__ float2 lds[2 * GR * LdsSize<M>::value];
+    const int tr = threadIdx.x / T, t = threadIdx.x % T;  // tr in [0, 2*GR): SP rows then BG rows
+    const int which = tr / GR, r = tr % GR;
+    const int tiles = M / GR;
+    const int pair = blockIdx.x / tiles;
+    const int y = (blockIdx.x % tiles) * GR + r;
+    const long long img = (long long)pair + (long long)which * batch;
+    const float2* row = in + (img * M + y) * (long long)nin;
+    float2* s = lds + tr * LdsSize<M>::value;
+    auto gl = [&](int i) -> float2 {
+        const int pp = i < M / 2 ? i : i - M;  // p'
+        const int q = pp - p_lo;
+        return (q >= 0 && q < nin) ? row[q] : make_float2(0.f, 0.f);
+    };
+    auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };
+    Plan<M>::template run<+1>(t, s, tw_m, gl, gs);
+    __syncthreads();
+    // elementwise combine: GR rows x M pixels, 2*GR*T threads
+    for (int e = threadIdx.x; e < GR * M; e += 2 * GR * T) {
+        const int rr = e / M, x = e % M;
+        const float2 sp = lds[rr * LdsSize<M>::value + lds_pad(x)];
+        const float2 bgv = lds[(GR + rr) * LdsSize<M>::value + lds_pad(x)];
+        const float2 z = cmulc(sp, bgv);
+        const float a2 = sp.x * sp.x + sp.y * sp.y;
+        const float b2 = bgv.x * bgv.x + bgv.y * bgv.y;
+        float ph = atan2f(z.y, z.x);
+        float la = 0.5f * (logf(a2) - logf(b2));
+        if (!isfinite(ph)) ph = 0.f;
+        if (!isfinite(la)) la = 0.f;
+        const long long o = ((long long)pair * M + ((blockIdx.x % tiles) * GR + rr)) * M + x;
+        wrapped[o] = ph;
+        logamp[o] = la;
+        if (field) {
+            float2 f = make_float2(z.x / b2, z.y / b2);
+            if (!isfinite(f.x) || !isfinite(f.y)) f = make_float2(0.f, 0.f);
+            field[o] = f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- sideband search
+// Key = (bits of |F|^2) << 32 | (0xffffffff - raster index): the maximum key is the largest
+// magnitude, ties resolved to the first pixel in raster order of the shifted spectrum
+// (obtainRadius scans rows j then columns i with a strict '>', HilbertTransform.cu:3439-3450).
+__global__ void k_argmax_band(const float2* __restrict__ spec, int H, int W, int row_start,
+                              unsigned long long* __restrict__ best)
+{
+    const long long n = (long long)(H - row_start) * W;
+    unsigned long long local = 0;
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int j = row_start + (int)(g / W), i = (int)(g % W);  // shifted coordinates
+        const int ky = (j + H / 2) % H, kx = (i + W / 2) % W;     // unshifted
+        const float2 v = spec[(long long)ky * W + kx];
+        const float m = v.x * v.x + v.y * v.y;
+        const unsigned long long key = ((unsigned long long)__float_as_uint(m) << 32) |
+                                       (0xffffffffull - (unsigned long long)(i + (long long)W * j));
+        local = key > local ? key : local;
+    }
+    // wave reduction then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(local, off);
+        local = o > local ? o : local;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(best, local);
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+template <int W>
+static void launch_row_real(Ctx& c, const float* src0, int n0, const float* src1, int nimg, int H,
+                            int k_lo, int nout, float2* out)
+{
+    constexpr int G = 256 / Plan<W / 2>::T;
+    const long long rows = (long long)nimg * H;
+    hipLaunchKernelGGL((k_row_real<W, G>), dim3(grid1d(rows, G)), dim3(G * Plan<W / 2>::T), 0,
+                       c.stream, src0, n0, src1, nimg, H, c.tw(W / 2), c.tw(W), k_lo, nout, out);
+    CTDM_CHECK_LAUNCH("k_row_real");
+}
+
+static void row_real(Ctx& c, const float* src0, int n0, const float* src1, int nimg, int H, int W,
+                     int k_lo, int nout, float2* out)
+{
+    switch (W) {
+    case 512: launch_row_real<512>(c, src0, n0, src1, nimg, H, k_lo, nout, out); break;
+    case 1024: launch_row_real<1024>(c, src0, n0, src1, nimg, H, k_lo, nout, out); break;
+    default: fail(CTDM_EUNSUPPORTED, "hologram width " + std::to_string(W) + " has no kernel");
+    }
+}
+
+template <int H, int M>
+static void launch_col_band(Ctx& c, const float2* in, int nimg, int nin, int r0, const Band& qb,
+                            float2* out)
+{
+    constexpr int C = 256 / Plan<H>::T;
+    const int tiles = (nin + C - 1) / C;
+    hipLaunchKernelGGL((k_col_band<H, M, C>), dim3(tiles * nimg), dim3(C * Plan<H>::T), 0, c.stream,
+                       in, nimg, nin, c.tw(H), c.tw(M), r0, qb.lo, qb.n, out);
+    CTDM_CHECK_LAUNCH("k_col_band");
+}
+
+template <int M>
+static void launch_row_final(Ctx& c, const float2* in, int batch, int nin, int p_lo, float* wrapped,
+                             float* logamp, float2* field)
+{
+    constexpr int GR = 128 / Plan<M>::T;
+    hipLaunchKernelGGL((k_row_final<M, GR>), dim3(batch * (M / GR)), dim3(2 * GR * Plan<M>::T), 0,
+                       c.stream, in, batch, nin, p_lo, c.tw(M), wrapped, logamp, field);
+    CTDM_CHECK_LAUNCH("k_row_final");
+}
+
+template <int H>
+static void col_band_m(Ctx& c, int M, const float2* in, int nimg, int nin, int r0, const Band& qb,
+                       float2* out)
+{
+    switch (M) {
+    case 64: launch_col_band<H, 64>(c, in, nimg, nin, r0, qb, out); break;
+    case 128: launch_col_band<H, 128>(c, in, nimg, nin, r0, qb, out); break;
+    case 192: launch_col_band<H, 192>(c, in, nimg, nin, r0, qb, out); break;
+    case 256: launch_col_band<H, 256>(c, in, nimg, nin, r0, qb, out); break;
+    default: fail(CTDM_EUNSUPPORTED, "crop size " + std::to_string(M) + " has no kernel");
+    }
+}
+
+static void validate_crop(int H, int W, const ctdm_crop& cr)
+{
+    CTDM_REQUIRE(H == W, "retrieval: square holograms required");
+    CTDM_REQUIRE(cr.M > 0 && cr.M % 2 == 0 && cr.M <= H, "retrieval: bad crop size");
+    CTDM_REQUIRE(cr.radius >= 0, "retrieval: negative passband radius");
+    CTDM_REQUIRE(cr.row - cr.M / 2 >= 0 && cr.row + cr.M / 2 <= H && cr.col - cr.M / 2 >= 0 &&
+                     cr.col + cr.M / 2 <= W,
+                 "retrieval: crop window leaves the spectrum");
+}
+
+void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int H, int W,
+                      const ctdm_crop& crop, float* wrapped, float* logamp, float2* field)
+{
+    CTDM_REQUIRE(holo && bg && wrapped && logamp, "retrieve: null pointer");
+    CTDM_REQUIRE(batch >= 0, "retrieve: negative batch");
+    validate_crop(H, W, crop);
+    if (batch == 0) return;
+    const int M = crop.M;
+    const Band pb = passband(crop.radius, M), qb = passband(crop.radius, M);
+    const int c0 = crop.col - W / 2, r0 = crop.row - H / 2;
+    const int nimg = 2 * batch;
+    float2* R = (float2*)c.buf("ret.R", sizeof(float2) * (size_t)nimg * H * pb.n);
+    float2* G = (float2*)c.buf("ret.G", sizeof(float2) * (size_t)nimg * M * pb.n);
+    row_real(c, holo, batch, bg, nimg, H, W, c0 + pb.lo, pb.n, R);
+    switch (H) {
+    case 512: col_band_m<512>(c, M, R, nimg, pb.n, r0, qb, G); break;
+    case 1024: col_band_m<1024>(c, M, R, nimg, pb.n, r0, qb, G); break;
+    default: fail(CTDM_EUNSUPPORTED, "hologram height " + std::to_string(H) + " has no kernel");
+    }
+    switch (M) {
+    case 64: launch_row_final<64>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;
+    case 128: launch_row_final<128>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;
+    case 192: launch_row_final<192>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;
+    case 256: launch_row_final<256>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;
+    default: fail(CTDM_EUNSUPPORTED, "crop size has no kernel");
+    }
+}
+
+void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, ctdm_crop* out)
+{
+    CTDM_REQUIRE(bg && out, "find_sideband: null pointer");
+    CTDM_REQUIRE(H == W && (H == 512 || H == 1024), "find_sideband: square 512 or 1024 holograms");
+    if (row_start <= 0) row_start = 600 * H / 1024;
+    CTDM_REQUIRE(row_start < H, "find_sideband: row_start beyond the spectrum");
+    float2* spec = (float2*)c.buf("sb.spec", sizeof(float2) * (size_t)H * W);
+    unsigned long long* best = (unsigned long long*)c.buf("sb.best", sizeof(unsigned long long));
+    // Full spectrum of one real image: real row pass keeping every bin, then column pass.
+    row_real(c, bg, 1, bg, 1, H, W, 0, W, spec);
+    fft_cols(c, spec, H, W, W, 1, 0, -1, 1.f, false);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c.stream, best);
+    hipLaunchKernelGGL(k_argmax_band, dim3(256), dim3(256), 0, c.stream, spec, H, W, row_start, best);
+    CTDM_CHECK_LAUNCH("k_argmax_band");
+    unsigned long long h = 0;
+    CTDM_HIP(hipMemcpyAsync(&h, best, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    CTDM_HIP(hipStreamSynchronize(c.stream));
+    const long long idx = (long long)(0xffffffffull - (h & 0xffffffffull));
+    out->row = (int)(idx / W);
+    out->col = (int)(idx % W);
+    out->radius = std::abs(out->row - H / 2) / 3;
+    out->M = M;
+}
+
+// ---------------------------------------------------------------- plain (reference-structured) path
+__global__ void k_real_to_complex(const float* __restrict__ in, float2* __restrict__ out, long long n)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) out[g] = make_float2(in[g], 0.f);
+}
+
+// get1stOrder_new (HilbertTransform.cu:3566-3586), M-generalised.
+__global__ void k_get_first_order(float2* __restrict__ out, const float2* __restrict__ in, int radius,
+                                  int r, int c, int W, int M)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= M || j >= M) return;
+    const int ii = i + (c - M / 2), jj = j + (r - M / 2);
+    if (ii >= c - radius && ii <= c + radius && jj >= r - radius && jj <= r + radius)
+        out[i + M * j] = in[ii + (long long)W * jj];
+    else
+        out[i + M * j] = make_float2(0.f, 0.f);
+}
+
+// calcWrapPhase (HilbertTransform.cu:3618-3639) on conjugated fields.
+__global__ void k_calc_wrap_phase(float* __restrict__ phase, float* __restrict__ amp,
+                                  const float2* __restrict__ sp, const float2* __restrict__ bgf, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float p = atan2f(bgf[i].y, bgf[i].x) - atan2f(sp[i].y, sp[i].x);
+    if (!isfinite(p)) p = 0.f;
+    float a = sqrtf(sp[i].y * sp[i].y + sp[i].x * sp[i].x) / sqrtf(bgf[i].y * bgf[i].y + bgf[i].x * bgf[i].x);
+    if (!isfinite(a)) a = 0.f;
+    phase[i] = p;
+    amp[i] = a;
+}
+
+void retrieve_plain_one(Ctx& c, const float* holo, const float* bg, int H, int W,
+                        const ctdm_crop& crop, float* phase, float* amp)
+{
+    validate_crop(H, W, crop);
+    const int M = crop.M;
+    const long long n = (long long)H * W;
+    float2* SP = (float2*)c.buf("plain.SP", sizeof(float2) * n);
+    float2* BG = (float2*)c.buf("plain.BG", sizeof(float2) * n);
+    float2* SP2 = (float2*)c.buf("plain.SP2", sizeof(float2) * M * M);
+    float2* BG2 = (float2*)c.buf("plain.BG2", sizeof(float2) * M * M);
+    hipLaunchKernelGGL(k_real_to_complex, dim3(grid1d(n, 256)), dim3(256), 0, c.stream, holo, SP, n);
+    hipLaunchKernelGGL(k_real_to_complex, dim3(grid1d(n, 256)), dim3(256), 0, c.stream, bg, BG, n);
+    CTDM_CHECK_LAUNCH("k_real_to_complex");
+    for (float2* X : {SP, BG}) {  // cufftExecC2C 2-D forward (HilbertTransform.cu:1564-1565)
+        fft_rows(c, X, W, H, -1, 1.f, false);
+        fft_cols(c, X, H, W, W, 1, 0, -1, 1.f, false);
+    }
+    fftshift2d(c, SP, W, H, 1);
+    fftshift2d(c, BG, W, H, 1);
+    dim3 b2(32, 8), g2((M + 31) / 32, (M + 7) / 8);
+    hipLaunchKernelGGL(k_get_first_order, g2, b2, 0, c.stream, SP2, SP, crop.radius, crop.row, crop.col, W, M);
+    hipLaunchKernelGGL(k_get_first_order, g2, b2, 0, c.stream, BG2, BG, crop.radius, crop.row, crop.col, W, M);
+    CTDM_CHECK_LAUNCH("k_get_first_order");
+    fftshift2d(c, SP2, M, M, 1);
+    fftshift2d(c, BG2, M, M, 1);
+    const float sc = 1.f / ((float)M * (float)M);
+    for (float2* X : {SP2, BG2}) {  // cufftExecC2C inverse + scaleFFT2D (scale and conjugate)
+        fft_rows(c, X, M, M, +1, 1.f, false);
+        fft_cols(c, X, M, M, M, 1, 0, +1, sc, true);
+    }
+    hipLaunchKernelGGL(k_calc_wrap_phase, dim3(grid1d(M * M, 256)), dim3(256), 0, c.stream, phase, amp, SP2, BG2, M * M);
+    CTDM_CHECK_LAUNCH("k_calc_wrap_phase");
+}
+
+}  // namespace ctdm

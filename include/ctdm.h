@@ -1,0 +1,255 @@
+/*
+ * ctdm.h — C ABI of the MI355X-native cTDM reconstruction library (libctdm.so).
+ *
+ * Path: interferogram stack + background holograms -> wrapped / unwrapped phase + log-amplitude
+ *       -> Rytov spectra -> Ewald-cap k-space -> positivity-constrained 3-D RI volume.
+ *
+ * Conventions shared by every entry point
+ *   - All array pointers are caller-owned DEVICE memory unless a comment says "host".
+ *   - Images are row-major with idx = x + W*y; volumes idx = x + V*y + V*V*z (the reference's
+ *     layout, e.g. HilbertTransform.cu:3566-3586, Recon_kernel.cu:262).
+ *   - Complex values are interleaved float pairs (ctdm_complex), bit-compatible with cufftComplex.
+ *   - Every call is stream-ordered on the context's stream and returns a ctdm_status; nothing
+ *     calls exit() (the reference's checkCudaErrors does, HilbertTransform.cu:480-574).
+ *     ctdm_find_sideband and ctdm_autotune additionally synchronise the stream because their
+ *     results are host values.
+ *   - The context owns scratch memory, FFT twiddle tables, the autotune cache and captured graphs.
+ *     One context must not be used by two host threads at the same time.
+ */
+#ifndef CTDM_H
+#define CTDM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ctdm_status {
+    CTDM_OK = 0,
+    CTDM_EINVAL = 1,       /* bad argument (sizes, null pointers, crop outside the spectrum) */
+    CTDM_ENOMEM = 2,       /* device allocation failed */
+    CTDM_EDEVICE = 3,      /* HIP runtime / kernel launch error */
+    CTDM_EUNSUPPORTED = 4  /* size or flag combination without a kernel */
+} ctdm_status;
+
+typedef struct ctdm_ctx ctdm_ctx;
+
+typedef struct ctdm_complex { float x, y; } ctdm_complex;
+
+/* hipStream_t, passed opaquely so that FFI callers need no HIP headers; NULL = default stream. */
+typedef void *ctdm_stream;
+
+typedef struct ctdm_config {
+    int device;        /* HIP device ordinal the context binds to */
+    int reserved[7];   /* must be zero */
+} ctdm_config;
+
+/* Sideband crop. Reference: obtainRadius() (HilbertTransform.cu:3425-3455) returns (r, c, radius);
+ * get1stOrder_new() (HilbertTransform.cu:3566-3586) crops an M x M window (M = H/4 there). */
+typedef struct ctdm_crop {
+    int row;     /* sideband centre row in the fftshifted H x W spectrum */
+    int col;     /* sideband centre column in the fftshifted spectrum */
+    int radius;  /* half-width of the square passband, |row - H/2| / 3 in the reference */
+    int M;       /* crop side; also the Rytov spectrum side and the volume side */
+} ctdm_crop;
+
+/* Optics of one cell. Reference constants: BatchRecon (HilbertTransform.cu:1840-1862). */
+typedef struct ctdm_optics {
+    double wavelength;  /* vacuum wavelength [m] */
+    double n_med;       /* medium refractive index */
+    double dx;          /* sampling of the cropped field = voxel pitch [m]
+                           (reference: CameraPixelSize/Mag * 1024 / ReconZ) */
+} ctdm_optics;
+
+/* Illumination angle of one hologram [rad] (reference sampleAngleRadX/Y; EdwardSphere takes
+ * f_x0 from ang_y and f_y0 from ang_x, Recon_kernel.cu:175-176 — reproduced as is). */
+typedef struct ctdm_angle { double ang_x, ang_y; } ctdm_angle;
+
+/* ---- unwrapping flags (ctdm_unwrap_dct) ---- */
+#define CTDM_UNWRAP_WRAPDIFF        0x0u  /* RHS = divergence of wrapped forward differences (DCT_Unwrapping.cu:336-417) */
+#define CTDM_UNWRAP_SINCOS          0x1u  /* RHS = cos(phi) L[sin(phi)] - sin(phi) L[cos(phi)] (FastUnwrapping.cu:37-96 idea) */
+#define CTDM_UNWRAP_REFCOMPAT_EIG   0x2u  /* eigenvalues 2cos(4*pi*i/(4N-1)) + 2cos(4*pi*j/(4N-1)) - 4 (DCT_Unwrapping.cu:310) */
+#define CTDM_UNWRAP_REFCOMPAT_IDCT  0x4u  /* (i+1) twiddle of the reference IDCT (DCT_Unwrapping.cu:230-231) */
+#define CTDM_UNWRAP_CONGRUENT       0x8u  /* phi_w + 2*pi*round((phi_ls - phi_w)/2*pi) (FastUnwrapping.cu:333-344) */
+#define CTDM_UNWRAP_REFCOMPAT       (CTDM_UNWRAP_REFCOMPAT_EIG | CTDM_UNWRAP_REFCOMPAT_IDCT)
+
+/* ---- reconstruction flags ---- */
+#define CTDM_DC_HITMASK     0x0u   /* data consistency where the hit count is > 0 (default) */
+#define CTDM_DC_REFCOMPAT   0x1u   /* reference test: F2.re != 0 && F2.im != 0 (Recon_kernel.cu:406) */
+#define CTDM_POS_NO_SHIFT   0x2u   /* skip the final 3-D octant shift (HilbertTransform.cu:1974) */
+#define CTDM_EXEC_PLAIN     0x100u /* run the reference-structured, unfused kernel sequence */
+
+/* ---------------------------------------------------------------- context */
+ctdm_status ctdm_create(ctdm_ctx **ctx, const ctdm_config *cfg, ctdm_stream stream);
+ctdm_status ctdm_destroy(ctdm_ctx *ctx);
+ctdm_status ctdm_set_stream(ctdm_ctx *ctx, ctdm_stream stream);
+ctdm_status ctdm_sync(ctdm_ctx *ctx);
+/* Text of the last error raised on this context (never NULL). */
+const char *ctdm_last_error(const ctdm_ctx *ctx);
+/* Library version string (host). */
+const char *ctdm_version(void);
+
+/* ---------------------------------------------------------------- retrieval (L2) */
+/* Replaces obtainRadius (HilbertTransform.cu:3425-3455) applied to the fftshifted spectrum of the
+ * background (HilbertTransform.cu:1564-1572): argmax of |BG spectrum| over shifted rows
+ * [row_start, H), raster order, first maximum wins; radius = |row - H/2| / 3. row_start <= 0 selects
+ * the reference's 600*H/1024. Writes the crop to HOST memory *out (synchronises the stream). */
+ctdm_status ctdm_find_sideband(ctdm_ctx *ctx, const float *bg, int H, int W, int M, int row_start,
+                               ctdm_crop *out);
+
+/* Replaces extractQPI_AlgorithmA (HilbertTransform.cu:1546-1585) + calcWrapPhase
+ * (HilbertTransform.cu:3618-3639) for `batch` hologram pairs (holo, bg: batch x H x W float32).
+ * Outputs per item, M x M: wrapped = arg(SP) - arg(BG) wrapped to (-pi, pi] (the reference value
+ * mod 2*pi), logamp = ln(|SP| / |BG|). field_opt (may be NULL): SP/BG complex ratio.
+ * Non-finite results are replaced by 0 as in calcWrapPhase. */
+ctdm_status ctdm_retrieve(ctdm_ctx *ctx, const float *holo, const float *bg, int batch, int H, int W,
+                          const ctdm_crop *crop, float *wrapped, float *logamp,
+                          ctdm_complex *field_opt);
+
+/* ---------------------------------------------------------------- unwrapping (L3) */
+/* Replaces DCT_UWLS_Unwrapped (DCT_Unwrapping.cu:10-59; RHS option from FastUnwrapping.cu:5-117):
+ * least-squares unwrapping with Neumann boundaries via an orthonormal DCT-II Poisson solve, for
+ * `batch` M x M maps. flags: CTDM_UNWRAP_*. */
+ctdm_status ctdm_unwrap_dct(ctdm_ctx *ctx, const float *wrapped, float *unwrapped, int batch, int M,
+                            unsigned flags);
+
+/* Adjoint (transpose) of the Jacobian of ctdm_unwrap_dct at `wrapped` (no reference counterpart):
+ * grad_wrapped = J^T grad_unwrapped. Wrapped differences have slope 1 almost everywhere, so for
+ * WRAPDIFF the Jacobian is the linear Poisson operator; SINCOS linearises at `wrapped`. */
+ctdm_status ctdm_unwrap_dct_adjoint(ctdm_ctx *ctx, const float *wrapped, const float *grad_unwrapped,
+                                    float *grad_wrapped, int batch, int M, unsigned flags);
+
+/* Replaces cudaGoldsteinUnwrap2D (Goldstein.cu:5-50; residues :52-103, cuts :105-248, flood fill
+ * :263-641) with a deterministic specification of the same steps, batched over `batch` H x W maps:
+ *   residues  int8  +1/-1/0 (last row/column 0)
+ *   cuts      uint8 1 on branch-cut pixels
+ *   k2pi      int32 integer 2*pi count; INT32_MIN where the flood fill never reached the pixel
+ *   unwrapped float wrapped + 2*pi*k2pi, 0 where unreached (as the reference leaves it)
+ * Any output pointer except `unwrapped` may be NULL. */
+ctdm_status ctdm_goldstein(ctdm_ctx *ctx, const float *wrapped, int batch, int H, int W,
+                           int8_t *residues, uint8_t *cuts, int32_t *k2pi, float *unwrapped);
+
+/* ---------------------------------------------------------------- reconstruction (L4) */
+/* Replaces CopyDataArray/cuCopyArray (Recon_kernel.cu:144-169) + cuFFT2Dshift + cuFFT2D(-1)
+ * (HilbertTransform.cu:1909-1913): u = logamp + i*unwrapped, stored transposed, spatially
+ * fftshifted, forward unnormalised 2-D DFT. A maps of M x M -> A spectra of M x M. */
+ctdm_status ctdm_rytov_spectrum(ctdm_ctx *ctx, const float *unwrapped, const float *logamp, int A,
+                                int M, ctdm_complex *spec);
+
+/* Replaces EdwardSphere/fillEdwardSphere (Recon_kernel.cu:171-229) over all A angles:
+ * F[v] += 4*pi*i*f_z*spec[a][p], hits[v] += 1 on the Ewald cap (ACCUMULATES: the caller zeroes
+ * F and hits). angles: HOST array of A. M must equal V (as in the reference, ReconX = crop side). */
+ctdm_status ctdm_ewald_scatter(ctdm_ctx *ctx, const ctdm_complex *spec, const ctdm_angle *angles,
+                               int A, int M, int V, const ctdm_optics *optics, ctdm_complex *F,
+                               float *hits);
+
+/* Adjoint of ctdm_ewald_scatter (no reference counterpart): spec[a][p] = conj(4*pi*i*f_z) * F[v]
+ * on the cap, 0 off the cap. Overwrites spec. */
+ctdm_status ctdm_ewald_gather(ctdm_ctx *ctx, const ctdm_complex *F, const ctdm_angle *angles, int A,
+                              int M, int V, const ctdm_optics *optics, ctdm_complex *spec);
+
+/* Replaces initial_F_and_F2/Initial_F_F2 (Recon_kernel.cu:241-276): F /= hits where hits != 0,
+ * F *= 1/dx, F_meas = F. F_meas may be NULL. */
+ctdm_status ctdm_normalize_measured(ctdm_ctx *ctx, ctdm_complex *F, const float *hits,
+                                    ctdm_complex *F_meas, int V, double dx);
+
+/* Replaces cuFFT3D (FFT.cu:807-838) + scaleFFT3D (FFT.cu:900-916): in-place 3-D DFT of a V^3
+ * volume; dir = -1 forward unnormalised, dir = +1 inverse scaled by 1/V^3 (no conjugation). */
+ctdm_status ctdm_fft3d(ctdm_ctx *ctx, ctdm_complex *vol, int V, int dir);
+
+/* Replaces the positivity loop of BatchRecon (HilbertTransform.cu:1940-1975): given the real-space
+ * scattering potential f (the inverse 3-D DFT of the normalised measured spectrum; overwritten),
+ * the measured spectrum F_meas and hit counts, runs `iters` iterations of
+ *   f = modify(n); F = FFT3(f); F = F_meas where measured; f = iFFT3(F); n = est_n(f)
+ * (calculate_n3D / Modify_F3D / check_F3D, Recon_kernel.cu:305-412) starting from n = est_n(f),
+ * and writes n (complex RI; positivity Re n >= n_med, Im n >= 0) to `n`, octant-shifted unless
+ * CTDM_POS_NO_SHIFT. iters = IterTime + 1 of the reference. hits may be NULL with
+ * CTDM_DC_REFCOMPAT. CTDM_EXEC_PLAIN runs the unfused sequence. */
+ctdm_status ctdm_positivity(ctdm_ctx *ctx, ctdm_complex *f, const ctdm_complex *F_meas,
+                            const float *hits, ctdm_complex *n, int V, int iters,
+                            const ctdm_optics *optics, unsigned flags);
+
+/* ---------------------------------------------------------------- pipeline */
+/* One cell: A hologram pairs -> one V^3 RI volume (reference: per-frame loop
+ * HilbertTransform.cu:785-1165 followed by BatchRecon, HilbertTransform.cu:1825-2047). */
+typedef struct ctdm_cell {
+    const float *holo;          /* A x H x W sample holograms (device) */
+    const float *bg;            /* A x H x W backgrounds (device) */
+    const ctdm_angle *angles;   /* A angles (HOST) */
+    int A, H, W;
+    int V;                      /* volume side = crop side M */
+    int iters;                  /* positivity iterations (reference IterTime + 1) */
+    unsigned unwrap_flags;      /* CTDM_UNWRAP_* */
+    unsigned recon_flags;       /* CTDM_DC_* | CTDM_POS_NO_SHIFT */
+    ctdm_optics optics;
+    ctdm_crop crop;             /* crop.M == V; crop.radius < 0: locate from bg[0] (reference frame 1) */
+    ctdm_complex *ri;           /* V^3 complex RI output (device) */
+    float *wrapped_opt;         /* optional A x V x V outputs (device), may be NULL */
+    float *unwrapped_opt;
+    float *logamp_opt;
+} ctdm_cell;
+
+typedef struct ctdm_batch_opts {
+    int mode;      /* 0 = plain (reference structure: per angle, unfused, host-synchronous);
+                      1 = batched (all angles and cells per launch, fused passes) */
+    int graph;     /* batched only: capture the launch sequence in a hipGraph and replay it */
+    int autotune;  /* batched only: autotune kernel variants for the sizes present before capture */
+    int reserved[5];
+} ctdm_batch_opts;
+
+ctdm_status ctdm_reconstruct(ctdm_ctx *ctx, ctdm_cell *cells, int ncells,
+                             const ctdm_batch_opts *opts);
+
+/* Times every kernel variant for volume/crop side V (FFT radix plan, pencils per workgroup,
+ * gridding block) on this device and caches the fastest in the context (no reference
+ * counterpart). Synchronises. `report` (HOST, may be NULL) receives a text summary. */
+ctdm_status ctdm_autotune(ctdm_ctx *ctx, const int *sizes, int nsizes, char *report,
+                          size_t report_len);
+
+/* ---------------------------------------------------------------- differentiable fit */
+/* Projected-gradient fit of measured Rytov spectra (no reference counterpart; replaces the
+ * positivity loop when chosen): minimise 0.5*sum_a |G_a(FFT3 f) - spec_a|^2 over the scattering
+ * potential f, where G_a = ctdm_ewald_gather restricted to the cap and scaled by dx / (4*pi*i*f_z)^-1
+ * as documented in DESIGN; each step recomputes FFT3 and its adjoint, then projects f onto the
+ * positivity set (Re n >= n_med, Im n >= 0). f holds the start on entry and the result on exit;
+ * n receives est_n(f) (octant-shifted unless CTDM_POS_NO_SHIFT). */
+ctdm_status ctdm_fit_pgd(ctdm_ctx *ctx, ctdm_complex *f, const ctdm_complex *spec,
+                         const ctdm_angle *angles, int A, int V, const ctdm_optics *optics,
+                         int iters, double step, ctdm_complex *n, unsigned flags);
+
+/* ---------------------------------------------------------------- synthetic inputs (host) */
+/* Seeded synthetic phantoms and holograms (host memory; no reference counterpart, the
+ * reference reads BMP files). See the generator source for the models. */
+typedef struct ctdm_synth_params {
+    int H;              /* hologram side (1024) */
+    double pixel;       /* sample-plane hologram pixel [m] (0.1e-6) */
+    double wavelength;  /* [m] */
+    double n_med;
+    double na_ill;      /* illumination NA of the angle cone */
+    double n_cell, n_nucleoli;
+    double kappa_cell;  /* imaginary RI of the cell (weak absorption) */
+    double semi_axes[3];/* ellipsoid semi-axes [m] */
+    int n_nuc;          /* number of nucleoli */
+    double r_nuc;       /* nucleolus radius [m] */
+    double noise;       /* Gaussian intensity noise sigma */
+    double carrier[2];  /* carrier frequency [cycles / pixel] along x and y */
+} ctdm_synth_params;
+
+void ctdm_synth_default_params(ctdm_synth_params *p);
+/* A angles on a cone (n_med sin theta = na_ill, azimuth 2*pi*a/A) -> angles_host[A];
+ * holo_host/bg_host: A x H x H float32 each (host). */
+ctdm_status ctdm_synth_cell(uint64_t seed, int A, const ctdm_synth_params *p, float *holo_host,
+                            float *bg_host, ctdm_angle *angles_host);
+/* Single off-axis hologram (sphere phase of radius 80 px, peak 12 rad, plus a 2nd-order
+ * background) and a background, H x H each (host). */
+ctdm_status ctdm_synth_hologram(uint64_t seed, int H, float *holo_host, float *bg_host);
+/* `batch` wrapped phase maps of H x W: sum of 8 Gaussians (|peak| <= 40 rad) + U(-0.8pi, 0.8pi)
+ * noise, wrapped to (-pi, pi] (host). */
+ctdm_status ctdm_synth_wrapped(uint64_t seed, int batch, int H, int W, float *out_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CTDM_H */
