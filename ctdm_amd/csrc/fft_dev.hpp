@@ -243,6 +243,15 @@ struct Plan<256> {
     }
 };
 template <>
+struct Plan<384> {
+    static constexpr int T = 16;
+    template <int DIR, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    {
+        Stockham<384, T, DIR, 1, 3, 8, 16>::run(t, s, tw, gl, gs);
+    }
+};
+template <>
 struct Plan<512> {
     static constexpr int T = 32;
     template <int DIR, class GL, class GS>

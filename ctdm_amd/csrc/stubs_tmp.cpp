@@ -2,9 +2,9 @@
 #include "internal.hpp"
 namespace ctdm {
 #define NOTYET(name) fail(CTDM_EUNSUPPORTED, name " not implemented yet")
-void unwrap_dct_batched(Ctx&, const float*, float*, int, int, unsigned) { NOTYET("unwrap_dct"); }
-void unwrap_dct_plain_one(Ctx&, const float*, float*, int, unsigned) { NOTYET("unwrap_dct_plain"); }
-void unwrap_dct_adjoint(Ctx&, const float*, const float*, float*, int, int, unsigned) { NOTYET("unwrap_dct_adjoint"); }
+
+
+
 void goldstein(Ctx&, const float*, int, int, int, int8_t*, uint8_t*, int32_t*, float*) { NOTYET("goldstein"); }
 void rytov_spectrum(Ctx&, const float*, const float*, int, int, float2*) { NOTYET("rytov"); }
 void ewald_scatter(Ctx&, const float2*, const ctdm_angle*, int, int, int, const ctdm_optics&, float2*, float*) { NOTYET("scatter"); }
