@@ -384,9 +384,11 @@ class Context:
 
     def BatchRecon(self, holo, bg, angles, V: int, iters: int, optics: Optics, **kw):
         """Per-cell pipeline ending in the reference's BatchRecon (HilbertTransform.cu:1825)."""
+        mode = kw.pop("mode", 1)
+        graph = kw.pop("graph", False)
         ri = self._torch.empty((V, V, V), dtype=self._torch.complex64, device=holo.device)
         cell = dict(holo=holo, bg=bg, angles=angles, V=V, iters=iters, optics=optics, ri=ri, **kw)
-        self.reconstruct([cell], mode=kw.pop("mode", 1) if "mode" in kw else 1)
+        self.reconstruct([cell], mode=mode, graph=graph)
         return ri
 
 

@@ -1,0 +1,785 @@
+// Tomographic reconstruction for gfx950: Rytov spectra, Ewald-cap direct mapping into a V^3
+// complex64 k-space, 3-D FFTs and the positivity-constrained loop (reference BatchRecon,
+// HilbertTransform.cu:1825-2047, and Recon_kernel.cu:144-412).
+//
+// Geometry (EdwardSphere / fillEdwardSphere, Recon_kernel.cu:171-229): for angle (ang_x, ang_y)
+//   f_x0 = f_m0 sin(ang_y), f_y0 = f_m0 sin(ang_x), f_z0 = sqrt(f_m0^2 - f_x0^2 - f_y0^2)
+// and spectrum pixel (Nx, Ny) with signed frequencies ii, jj in [-V/2, V/2):
+//   f_z = sqrt(f_m0^2 - (ii df + f_x0)^2 - (jj df + f_y0)^2),  Nz = mod(round(( f_z - f_z0) / df), V)
+//   F[Nx, Ny, Nz] += 4 pi i f_z spec[Nx, Ny],  hits[...] += 1   (skipped where f_z is not real)
+// The voxel index is computed in FP64 with explicitly rounded operations (no contraction), exactly
+// as the oracle computes it, so both assign every sample to the same voxel.
+//
+// Loop (HilbertTransform.cu:1940-1972): n = est_n(f); repeat { f = modify(n); F = FFT3(f);
+// F = F_meas on measured voxels; f = iFFT3(F); n = est_n(f) }, then the octant shift (:1974).
+// The fused schedule runs each iteration as 4 pencil passes over the volume:
+//   X: inverse x-FFT, 1/V^3, P = modify o est_n, forward x-FFT  (one read + one write)
+//   Y: forward y-FFT
+//   Z: forward z-FFT, data consistency from a compact per-pencil list, inverse z-FFT
+//   Y: inverse y-FFT
+// The unfused schedule (plain path) launches the reference's kernel sequence: modify, 3 FFT
+// passes, check, 3 FFT passes, scale, est_n.
+#include "fft_dev.hpp"
+#include "internal.hpp"
+
+#include <cmath>
+
+namespace ctdm {
+
+// ---------------------------------------------------------------- geometry (host)
+EwaldGeom ewald_geom(const ctdm_optics& o, int V)
+{
+    EwaldGeom g;
+    g.fm0 = o.n_med / o.wavelength;
+    g.fm02 = g.fm0 * g.fm0;
+    g.dx = o.dx;
+    g.df = 1.0 / ((double)V * o.dx);
+    const double k = 2.0 * M_PI * o.n_med / o.wavelength;
+    g.k2 = k * k;
+    g.n_med = o.n_med;
+    return g;
+}
+
+std::vector<AngleRow> angle_rows(const EwaldGeom& g, const ctdm_angle* ang, int A)
+{
+    CTDM_REQUIRE(ang || A == 0, "angles: null pointer");
+    std::vector<AngleRow> r(A);
+    for (int a = 0; a < A; ++a) {
+        const double fx0 = g.fm0 * std::sin(ang[a].ang_y);  // X/Y swap of Recon_kernel.cu:175-176
+        const double fy0 = g.fm0 * std::sin(ang[a].ang_x);
+        const double s0 = g.fm02 - fx0 * fx0 - fy0 * fy0;
+        r[a] = AngleRow{fx0, fy0, std::sqrt(s0 > 0 ? s0 : 0.0)};
+    }
+    return r;
+}
+
+struct Cap {
+    int nz;
+    float w;  // 4 pi f_z
+    bool ok;
+};
+
+__device__ __forceinline__ Cap ewald_cap(int ii, int jj, double fx0, double fy0, double fz0, double fm02,
+                                         double df, int V)
+{
+    const double fx = __dadd_rn(__dmul_rn((double)ii, df), fx0);
+    const double fy = __dadd_rn(__dmul_rn((double)jj, df), fy0);
+    const double s = __dsub_rn(fm02, __dadd_rn(__dmul_rn(fx, fx), __dmul_rn(fy, fy)));
+    const double s0 = __dsub_rn(__dsub_rn(fm02, __dmul_rn(fx0, fx0)), __dmul_rn(fy0, fy0));
+    Cap c;
+    c.ok = !(s0 < 0.0 || s < 0.0);
+    const double fz = __dsqrt_rn(s > 0.0 ? s : 0.0);
+    const double q = __ddiv_rn(__dsub_rn(fz, fz0), df);
+    const double nz = q > 0.0 ? floor(__dadd_rn(q, 0.5)) : ceil(__dsub_rn(q, 0.5));
+    int m = (int)nz % V;
+    if (m < 0) m += V;
+    c.nz = m;
+    c.w = (float)__dmul_rn(4.0 * M_PI, fz);
+    return c;
+}
+
+__device__ __forceinline__ int signed_freq(int n, int V) { return n < V / 2 ? n : n - V; }
+
+// Device copy of the per-angle rows (fx0, fy0, fz0).
+static const double* upload_rows(Ctx& c, const std::string& name, const std::vector<AngleRow>& rows)
+{
+    double* d = (double*)c.buf(name, sizeof(double) * 3 * std::max<size_t>(rows.size(), 1));
+    // Under graph capture the eager pass has already uploaded identical rows (the graph key holds
+    // the angle values); a pageable host copy must not be recorded.
+    if (!rows.empty() && !c.capturing)
+        CTDM_HIP(hipMemcpyAsync(d, rows.data(), sizeof(double) * 3 * rows.size(), hipMemcpyHostToDevice,
+                                c.stream));
+    return d;
+}
+
+// ---------------------------------------------------------------- Rytov spectrum (batched)
+// u(x, y) = logamp + i phi. Reference: u_sp[y' = x][x' = y] (cuCopyArray, transposed), spatial
+// fftshift, forward 2-D DFT. With U = DFT2(u): spec[kx' + M ky'] = (-1)^(kx'+ky') U(kx = ky', ky = kx').
+// Pass 1 (rows of u): P[a][y][kx] = DFT_x u. Pass 2 (columns kx): DFT_y, store at kx*M + ky.
+template <int M, int G>
+__global__ __launch_bounds__(G* Plan<M>::T) void k_rytov_rows(const float* __restrict__ phi,
+                                                              const float* __restrict__ la, int A,
+                                                              const float2* __restrict__ tw,
+                                                              float2* __restrict__ P)
+{
+    constexpr int T = Plan<M>::T;
+    __shared__ float2 lds[G * LdsSize<M>::value];
+    const int g = threadIdx.x / T, t = threadIdx.x % T;
+    const long long row = (long long)blockIdx.x * G + g;
+    const bool valid = row < (long long)A * M;
+    const long long r = valid ? row : 0;
+    float2* s = lds + g * LdsSize<M>::value;
+    auto gl = [&](int x) -> float2 { return make_float2(la[r * M + x], phi[r * M + x]); };
+    auto gs = [&](int k, float2 v) {
+        if (valid) P[r * M + k] = v;
+    };
+    Plan<M>::template run<-1>(t, s, tw, gl, gs);
+}
+
+template <int M, int B>
+__global__ __launch_bounds__(B* Plan<M>::T) void k_rytov_cols(const float2* __restrict__ P, int A,
+                                                              const float2* __restrict__ tw,
+                                                              float2* __restrict__ spec)
+{
+    constexpr int T = Plan<M>::T;
+    __shared__ float2 lds[B * LdsSize<M>::value];
+    const int b = threadIdx.x % B, t = threadIdx.x / B;
+    const int tiles = M / B;
+    const long long a = blockIdx.x / tiles;
+    const int kx0 = (blockIdx.x % tiles) * B;
+    const float2* col = P + a * M * M + kx0 + b;
+    float2* s = lds + b * LdsSize<M>::value;
+    auto gl = [&](int y) -> float2 { return col[(long long)y * M]; };
+    auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
+    Plan<M>::template run<-1>(t, s, tw, gl, gs);
+    __syncthreads();
+    float2* dst = spec + a * M * M;
+    for (int e = threadIdx.x; e < B * M; e += B * T) {
+        const int bb = e / M, ky = e % M, kx = kx0 + bb;
+        float2 v = lds[bb * LdsSize<M>::value + lds_pad(ky)];
+        if ((kx + ky) & 1) v = make_float2(-v.x, -v.y);
+        dst[(long long)kx * M + ky] = v;
+    }
+}
+
+template <int M>
+static void rytov_launch(Ctx& c, const float* phi, const float* la, int A, float2* spec)
+{
+    constexpr int G = 256 / Plan<M>::T, B = 256 / Plan<M>::T;
+    float2* P = (float2*)c.buf("ry.P", sizeof(float2) * (size_t)A * M * M);
+    hipLaunchKernelGGL((k_rytov_rows<M, G>), dim3(grid1d((long long)A * M, G)), dim3(G * Plan<M>::T), 0, c.stream,
+                       phi, la, A, c.tw(M), P);
+    hipLaunchKernelGGL((k_rytov_cols<M, B>), dim3(A * (M / B)), dim3(B * Plan<M>::T), 0, c.stream, P, A,
+                       c.tw(M), spec);
+    CTDM_CHECK_LAUNCH("rytov");
+}
+
+void rytov_spectrum(Ctx& c, const float* unwrapped, const float* logamp, int A, int M, float2* spec)
+{
+    CTDM_REQUIRE(unwrapped && logamp && spec, "rytov_spectrum: null pointer");
+    if (A == 0) return;
+    switch (M) {
+    case 64: rytov_launch<64>(c, unwrapped, logamp, A, spec); break;
+    case 128: rytov_launch<128>(c, unwrapped, logamp, A, spec); break;
+    case 192: rytov_launch<192>(c, unwrapped, logamp, A, spec); break;
+    case 256: rytov_launch<256>(c, unwrapped, logamp, A, spec); break;
+    default: fail(CTDM_EUNSUPPORTED, "rytov: size has no kernel");
+    }
+}
+
+// Reference-structured Rytov spectrum of one angle: cuCopyArray (Recon_kernel.cu:154-169) with the
+// amplitude map, cuFFT2Dshift, cuFFT2D(-1).
+__global__ void k_copy_array(float2* __restrict__ u, const float* __restrict__ phase, const float* __restrict__ amp,
+                             int M)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= M || j >= M) return;
+    const int idx = i + M * j;
+    u[j + M * i] = make_float2(logf(amp[idx]), phase[idx]);
+}
+
+void rytov_spectrum_plain_one(Ctx& c, const float* phase, const float* amp, int M, float2* u_sp)
+{
+    dim3 blk(32, 8), grd((M + 31) / 32, (M + 7) / 8);
+    hipLaunchKernelGGL(k_copy_array, grd, blk, 0, c.stream, u_sp, phase, amp, M);
+    CTDM_CHECK_LAUNCH("k_copy_array");
+    fftshift2d(c, u_sp, M, M, 1);
+    fft_rows(c, u_sp, M, M, -1, 1.f, false);
+    fft_cols(c, u_sp, M, M, M, 1, 0, -1, 1.f, false);
+}
+
+// ---------------------------------------------------------------- Ewald scatter / gather
+// Owner-computes scatter: one thread per k-space pencil (Nx, Ny) walks the angles in order and
+// updates its own pencil, so no atomics are needed and the summation order is fixed.
+__global__ void k_scatter_owner(const float2* __restrict__ spec, const double* __restrict__ rows, int A,
+                                int V, double fm02, double df, float2* __restrict__ F, float* __restrict__ hits)
+{
+    const long long VV = (long long)V * V;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= VV) return;
+    const int Nx = (int)(p % V), Ny = (int)(p / V);
+    const int ii = signed_freq(Nx, V), jj = signed_freq(Ny, V);
+    for (int a = 0; a < A; ++a) {
+        const Cap cp = ewald_cap(ii, jj, rows[3 * a], rows[3 * a + 1], rows[3 * a + 2], fm02, df, V);
+        if (!cp.ok) continue;
+        const float2 u = spec[(long long)a * VV + p];
+        const long long v = p + VV * cp.nz;
+        float2 f = F[v];
+        f.x += -cp.w * u.y;
+        f.y += cp.w * u.x;
+        F[v] = f;
+        hits[v] += 1.f;
+    }
+}
+
+// fillEdwardSphere of one angle (Recon_kernel.cu:187-229): one thread per spectrum pixel, plain
+// read-modify-write (the voxels of one angle are distinct).
+__global__ void k_scatter_one(const float2* __restrict__ u_sp, double fx0, double fy0, double fz0, int V,
+                              double fm02, double df, float2* __restrict__ F, float* __restrict__ hits)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= V || j >= V) return;
+    const int ii = i - V / 2, jj = j - V / 2;
+    const Cap cp = ewald_cap(ii, jj, fx0, fy0, fz0, fm02, df, V);
+    if (!cp.ok) return;
+    const int Nx = ii < 0 ? ii + V : ii, Ny = jj < 0 ? jj + V : jj;
+    const float2 u = u_sp[Nx + (long long)V * Ny];
+    const long long v = Nx + (long long)V * Ny + (long long)V * V * cp.nz;
+    float2 f = F[v];
+    f.x += -cp.w * u.y;
+    f.y += cp.w * u.x;
+    F[v] = f;
+    hits[v] += 1.f;
+}
+
+// Adjoint of the scatter: spec[a][p] = conj(4 pi i f_z) F[v] on the cap, 0 elsewhere.
+__global__ void k_gather(const float2* __restrict__ F, const double* __restrict__ rows, int A, int V,
+                         double fm02, double df, float2* __restrict__ spec)
+{
+    const long long VV = (long long)V * V;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= VV * A) return;
+    const int a = (int)(g / VV);
+    const long long p = g % VV;
+    const int ii = signed_freq((int)(p % V), V), jj = signed_freq((int)(p / V), V);
+    const Cap cp = ewald_cap(ii, jj, rows[3 * a], rows[3 * a + 1], rows[3 * a + 2], fm02, df, V);
+    float2 out = make_float2(0.f, 0.f);
+    if (cp.ok) {
+        const float2 f = F[p + VV * cp.nz];
+        out = make_float2(cp.w * f.y, -cp.w * f.x);
+    }
+    spec[g] = out;
+}
+
+void ewald_scatter(Ctx& c, const float2* spec, const ctdm_angle* ang, int A, int M, int V,
+                   const ctdm_optics& o, float2* F, float* hits)
+{
+    CTDM_REQUIRE(spec && F && hits, "ewald_scatter: null pointer");
+    CTDM_REQUIRE(M == V && V >= 4 && V % 2 == 0, "ewald_scatter: spectrum side must equal the volume side");
+    const EwaldGeom g = ewald_geom(o, V);
+    const double* rows = upload_rows(c, "ew.rows", angle_rows(g, ang, A));
+    const long long VV = (long long)V * V;
+    hipLaunchKernelGGL(k_scatter_owner, dim3(grid1d(VV, 256)), dim3(256), 0, c.stream, spec, rows, A, V, g.fm02,
+                       g.df, F, hits);
+    CTDM_CHECK_LAUNCH("k_scatter_owner");
+}
+
+void ewald_gather(Ctx& c, const float2* F, const ctdm_angle* ang, int A, int M, int V, const ctdm_optics& o,
+                  float2* spec)
+{
+    CTDM_REQUIRE(spec && F, "ewald_gather: null pointer");
+    CTDM_REQUIRE(M == V && V >= 4 && V % 2 == 0, "ewald_gather: spectrum side must equal the volume side");
+    const EwaldGeom g = ewald_geom(o, V);
+    const double* rows = upload_rows(c, "ew.rows", angle_rows(g, ang, A));
+    const long long n = (long long)V * V * A;
+    hipLaunchKernelGGL(k_gather, dim3(grid1d(n, 256)), dim3(256), 0, c.stream, F, rows, A, V, g.fm02, g.df, spec);
+    CTDM_CHECK_LAUNCH("k_gather");
+}
+
+// ---------------------------------------------------------------- normalisation (Initial_F_F2)
+__global__ void k_normalize(float2* __restrict__ F, const float* __restrict__ C, float2* __restrict__ Fm, long long n,
+                            float dx)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 f = F[i];
+    const float c = C[i];
+    if (c != 0.f) f = make_float2(f.x / c, f.y / c);
+    f = make_float2(f.x / dx, f.y / dx);
+    F[i] = f;
+    if (Fm) Fm[i] = f;
+}
+
+void normalize_measured(Ctx& c, float2* F, const float* hits, float2* Fm, int V, double dx)
+{
+    CTDM_REQUIRE(F && hits && dx > 0, "normalize_measured: bad argument");
+    const long long n = (long long)V * V * V;
+    hipLaunchKernelGGL(k_normalize, dim3(grid1d(n, 256)), dim3(256), 0, c.stream, F, hits, Fm, n, (float)dx);
+    CTDM_CHECK_LAUNCH("k_normalize");
+}
+
+// ---------------------------------------------------------------- 3-D FFT
+static void check_vol(int V)
+{
+    CTDM_REQUIRE(V == 64 || V == 128 || V == 192 || V == 256 || V == 512, "volume side has no kernel");
+}
+
+// x, y, z passes; `scale` folded into the z pass.
+static void fft3d_passes(Ctx& c, float2* vol, int V, int dir, float scale)
+{
+    const long long VV = (long long)V * V;
+    fft_rows(c, vol, V, VV, dir, 1.f, false);
+    fft_cols(c, vol, V, V, V, V, VV, dir, 1.f, false);
+    fft_cols(c, vol, V, V, VV, V, V, dir, scale, false);
+}
+
+void fft3d(Ctx& c, float2* vol, int V, int dir)
+{
+    CTDM_REQUIRE(vol, "fft3d: null pointer");
+    CTDM_REQUIRE(dir == -1 || dir == 1, "fft3d: dir must be -1 or +1");
+    check_vol(V);
+    const double s = dir > 0 ? 1.0 / ((double)V * V * V) : 1.0;
+    fft3d_passes(c, vol, V, dir, (float)s);
+}
+
+// ---------------------------------------------------------------- pointwise operators
+struct Proj {
+    float k2, inv_k2, nm, inv_nm2;
+};
+
+static Proj make_proj(const EwaldGeom& g)
+{
+    return Proj{(float)g.k2, (float)(1.0 / g.k2), (float)g.n_med, (float)(1.0 / (g.n_med * g.n_med))};
+}
+
+// calculate_n3D (Recon_kernel.cu:305-342): n = n_m sqrt(1 - f/k^2) with Re n >= n_m, Im n >= 0.
+// The square root is evaluated in the cancellation-free form (b = d / 2a for c >= 0); it equals the
+// reference's (sqrt((|z|+c)/2), sign(d) sqrt((|z|-c)/2)) in exact arithmetic.
+__device__ __forceinline__ float2 est_n(float2 f, const Proj& p)
+{
+    const float c = 1.f - f.x * p.inv_k2;
+    const float d = -f.y * p.inv_k2;
+    const float r = sqrtf(c * c + d * d);
+    float a, b;
+    if (c >= 0.f) {
+        a = sqrtf(0.5f * (r + c));
+        b = (a > 0.f) ? d / (2.f * a) : 0.f;
+    } else {
+        b = copysignf(sqrtf(0.5f * (r - c)), d);
+        a = (d != 0.f) ? d / (2.f * b) : 0.f;
+    }
+    if (d == 0.f) b = 0.f;
+    float nr = a * p.nm, ni = b * p.nm;
+    if (nr < p.nm) nr = p.nm;
+    if (ni < 0.f) ni = 0.f;
+    return make_float2(nr, ni);
+}
+
+// Modify_F3D (Recon_kernel.cu:370-383) with the caller's n_m^2 (HilbertTransform.cu:1962):
+// f = -k^2 ((n_r^2 - n_i^2)/n_m^2 - 1) + i (-k^2 2 n_r n_i / n_m^2), written without cancellation.
+__device__ __forceinline__ float2 modify_f(float2 n, const Proj& p)
+{
+    const float re = ((n.x - p.nm) * (n.x + p.nm) - n.y * n.y) * p.inv_nm2;
+    const float im = 2.f * n.x * n.y * p.inv_nm2;
+    return make_float2(-p.k2 * re, -p.k2 * im);
+}
+
+__global__ void k_est_n(const float2* __restrict__ f, float2* __restrict__ n, long long cnt, Proj p)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) n[i] = est_n(f[i], p);
+}
+
+__global__ void k_modify(const float2* __restrict__ n, float2* __restrict__ f, long long cnt, Proj p)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) f[i] = modify_f(n[i], p);
+}
+
+// check_F3D (Recon_kernel.cu:396-412) with the hit mask (default) or the reference's test.
+__global__ void k_check(float2* __restrict__ F, const float2* __restrict__ Fm, const float* __restrict__ hits,
+                        long long cnt, int refcompat)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    const float2 m = Fm[i];
+    const bool meas = refcompat ? (m.x != 0.f && m.y != 0.f) : (hits[i] > 0.f);
+    if (meas) F[i] = m;
+}
+
+__global__ void k_scale(float2* __restrict__ F, long long cnt, float s)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) F[i] = make_float2(F[i].x * s, F[i].y * s);
+}
+
+__global__ void k_fill(float* __restrict__ p, long long cnt, float v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) p[i] = v;
+}
+
+static void fill(Ctx& c, void* p, long long nfloats, float v)
+{
+    hipLaunchKernelGGL(k_fill, dim3(grid1d(nfloats, 256)), dim3(256), 0, c.stream, (float*)p, nfloats, v);
+    CTDM_CHECK_LAUNCH("k_fill");
+}
+
+void positivity_plain(Ctx& c, float2* f, const float2* Fm, const float* hits, float2* n, int V, int iters,
+                      const ctdm_optics& o, unsigned flags)
+{
+    CTDM_REQUIRE(f && Fm && n, "positivity: null pointer");
+    const bool ref = flags & CTDM_DC_REFCOMPAT;
+    CTDM_REQUIRE(hits || ref, "positivity: hits needed unless CTDM_DC_REFCOMPAT");
+    check_vol(V);
+    const EwaldGeom g = ewald_geom(o, V);
+    const Proj p = make_proj(g);
+    const long long cnt = (long long)V * V * V;
+    const dim3 grd(grid1d(cnt, 256)), blk(256);
+    const float s = (float)(1.0 / ((double)V * V * V));
+    hipLaunchKernelGGL(k_est_n, grd, blk, 0, c.stream, f, n, cnt, p);
+    for (int it = 0; it < iters; ++it) {
+        hipLaunchKernelGGL(k_modify, grd, blk, 0, c.stream, n, f, cnt, p);
+        fft3d_passes(c, f, V, -1, 1.f);
+        hipLaunchKernelGGL(k_check, grd, blk, 0, c.stream, f, Fm, hits, cnt, (int)ref);
+        fft3d_passes(c, f, V, +1, 1.f);
+        hipLaunchKernelGGL(k_scale, grd, blk, 0, c.stream, f, cnt, s);
+        hipLaunchKernelGGL(k_est_n, grd, blk, 0, c.stream, f, n, cnt, p);
+    }
+    CTDM_CHECK_LAUNCH("positivity_plain");
+    if (!(flags & CTDM_POS_NO_SHIFT)) fftshift3d(c, n, V);
+}
+
+// ---------------------------------------------------------------- fused volume passes
+// Compact data-consistency list: for pencil p = x + V*y, cnt[p] entries (z, value), stored at
+// p*cap + k (z as int16 in idx, value in val).
+struct DcList {
+    int* cnt;
+    short* idx;
+    float2* val;
+    int cap;
+};
+
+// Pencil tile geometry of the volume passes. AXIS 0: rows (x), 1: y, 2: z. B pencils per workgroup.
+template <int V, int AXIS, int B>
+struct Tile {
+    static constexpr int T = Plan<V>::T;
+    static constexpr bool BFAST = AXIS != 0;
+    int b, t;
+    long long base;  // element offset of this thread's pencil
+    long long stride;
+    int px, py;      // pencil coordinates (the two fixed axes)
+    __device__ Tile()
+    {
+        const int tid = threadIdx.x;
+        b = BFAST ? tid % B : tid / T;
+        t = BFAST ? tid / B : tid % T;
+        const long long VV = (long long)V * V;
+        if (AXIS == 0) {  // pencil (y, z): rows
+            const long long r = (long long)blockIdx.x * B + b;
+            px = (int)(r % V);
+            py = (int)(r / V);
+            base = r * V;
+            stride = 1;
+        } else if (AXIS == 1) {  // pencil (x, z)
+            const int tiles = V / B;
+            px = (int)(blockIdx.x % tiles) * B + b;
+            py = (int)(blockIdx.x / tiles);
+            base = px + VV * py;
+            stride = V;
+        } else {  // pencil (x, y)
+            const int tiles = V / B;
+            px = (int)(blockIdx.x % tiles) * B + b;
+            py = (int)(blockIdx.x / tiles);
+            base = px + (long long)V * py;
+            stride = VV;
+        }
+    }
+};
+
+// X pass. MODE 0: P + forward (input real space); 1: inverse, scale, P, forward;
+// 2: inverse, scale, est_n, octant shift -> out; 3: est_n (+ shift) -> out.
+template <int V, int B, int MODE>
+__global__ __launch_bounds__(B* Plan<V>::T) void k_vol_x(float2* __restrict__ W, float2* __restrict__ out,
+                                                         const float2* __restrict__ tw, Proj p, float scale,
+                                                         int shift)
+{
+    __shared__ float2 lds[B * LdsSize<V>::value];
+    Tile<V, 0, B> tl;
+    float2* s = lds + tl.b * LdsSize<V>::value;
+    float2* row = W + tl.base;
+    // destination row of the final n (octant shift)
+    const int h = shift ? V / 2 : 0;
+    const long long orow = ((long long)((tl.px + h) % V) + (long long)V * ((tl.py + h) % V)) * V;
+    auto put_n = [&](int x, float2 v) { out[orow + (x + h) % V] = v; };
+    if constexpr (MODE == 0) {
+        auto gl = [&](int x) -> float2 { return modify_f(est_n(row[x], p), p); };
+        auto gs = [&](int k, float2 v) { row[k] = v; };
+        Plan<V>::template run<-1>(tl.t, s, tw, gl, gs);
+    } else if constexpr (MODE == 3) {
+        for (int x = tl.t; x < V; x += Plan<V>::T) put_n(x, est_n(row[x], p));
+    } else {
+        auto gl = [&](int k) -> float2 { return row[k]; };
+        if constexpr (MODE == 1) {
+            auto gs = [&](int x, float2 v) { s[lds_pad(x)] = modify_f(est_n(cscale(v, scale), p), p); };
+            Plan<V>::template run<+1>(tl.t, s, tw, gl, gs);
+            __syncthreads();
+            auto gl2 = [&](int x) -> float2 { return s[lds_pad(x)]; };
+            auto gs2 = [&](int k, float2 v) { row[k] = v; };
+            Plan<V>::template run<-1>(tl.t, s, tw, gl2, gs2);
+        } else {
+            auto gs = [&](int x, float2 v) { put_n(x, est_n(cscale(v, scale), p)); };
+            Plan<V>::template run<+1>(tl.t, s, tw, gl, gs);
+        }
+    }
+}
+
+// Z pass with data consistency: forward z-FFT, replace measured voxels, inverse z-FFT.
+template <int V, int B>
+__global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ W, const float2* __restrict__ tw,
+                                                           DcList dc)
+{
+    constexpr int T = Plan<V>::T;
+    __shared__ float2 lds[B * LdsSize<V>::value];
+    Tile<V, 2, B> tl;
+    float2* s = lds + tl.b * LdsSize<V>::value;
+    float2* pen = W + tl.base;
+    auto gl = [&](int z) -> float2 { return pen[(long long)z * tl.stride]; };
+    auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
+    Plan<V>::template run<-1>(tl.t, s, tw, gl, gs);
+    __syncthreads();
+    // data consistency, lanes walk the entries of one pencil (contiguous in the list)
+    {
+        const int pb = threadIdx.x / T, k0 = threadIdx.x % T;
+        const long long pencil = (long long)(blockIdx.x % (V / B)) * B + pb + (long long)V * (blockIdx.x / (V / B));
+        const int n = dc.cnt[pencil];
+        float2* sp = lds + pb * LdsSize<V>::value;
+        for (int k = k0; k < n; k += T) {
+            const long long e = pencil * dc.cap + k;
+            sp[lds_pad(dc.idx[e])] = dc.val[e];
+        }
+    }
+    __syncthreads();
+    auto gl2 = [&](int k) -> float2 { return s[lds_pad(k)]; };
+    auto gs2 = [&](int z, float2 v) { pen[(long long)z * tl.stride] = v; };
+    Plan<V>::template run<+1>(tl.t, s, tw, gl2, gs2);
+}
+
+// Y pass (plain FFT along y).
+template <int V, int B, int DIR>
+__global__ __launch_bounds__(B* Plan<V>::T) void k_vol_y(float2* __restrict__ W, const float2* __restrict__ tw)
+{
+    __shared__ float2 lds[B * LdsSize<V>::value];
+    Tile<V, 1, B> tl;
+    float2* s = lds + tl.b * LdsSize<V>::value;
+    float2* pen = W + tl.base;
+    auto gl = [&](int y) -> float2 { return pen[(long long)y * tl.stride]; };
+    auto gs = [&](int k, float2 v) { pen[(long long)k * tl.stride] = v; };
+    Plan<V>::template run<DIR>(tl.t, s, tw, gl, gs);
+}
+
+// Compact list from a dense measured spectrum (positivity API): one thread per pencil.
+__global__ void k_dc_build(const float2* __restrict__ Fm, const float* __restrict__ hits, int V, int refcompat,
+                           DcList dc)
+{
+    const long long VV = (long long)V * V;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= VV) return;
+    int n = 0;
+    for (int z = 0; z < V; ++z) {
+        const long long v = p + VV * z;
+        const float2 m = Fm[v];
+        const bool meas = refcompat ? (m.x != 0.f && m.y != 0.f) : (hits[v] > 0.f);
+        if (meas) {
+            dc.idx[p * dc.cap + n] = (short)z;
+            dc.val[p * dc.cap + n] = m;
+            ++n;
+        }
+    }
+    dc.cnt[p] = n;
+}
+
+// Gridding pass: per k-space pencil (x, y) accumulate every angle's sample in angle order, average
+// by hit count, scale by 1/dx (Initial_F_F2), emit the compact list, then inverse z-FFT.
+template <int V, int B>
+__global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __restrict__ spec,
+                                                            const double* __restrict__ rows, int A,
+                                                            double fm02, double df, float inv_dx,
+                                                            int refcompat, const float2* __restrict__ tw,
+                                                            float2* __restrict__ W, DcList dc)
+{
+    constexpr int T = Plan<V>::T;
+    __shared__ float2 lds[B * LdsSize<V>::value];
+    __shared__ unsigned short hit[B][V];
+    Tile<V, 2, B> tl;
+    float2* s = lds + tl.b * LdsSize<V>::value;
+    for (int z = tl.t; z < V; z += T) {
+        s[lds_pad(z)] = make_float2(0.f, 0.f);
+        hit[tl.b][z] = 0;
+    }
+    __syncthreads();
+    const long long VV = (long long)V * V;
+    const long long pencil = tl.base;  // x + V*y
+    if (tl.t == 0) {
+        const int ii = signed_freq(tl.px, V), jj = signed_freq(tl.py, V);
+        for (int a = 0; a < A; ++a) {
+            const Cap cp = ewald_cap(ii, jj, rows[3 * a], rows[3 * a + 1], rows[3 * a + 2], fm02, df, V);
+            if (!cp.ok) continue;
+            const float2 u = spec[(long long)a * VV + pencil];
+            float2& acc = s[lds_pad(cp.nz)];
+            acc.x += -cp.w * u.y;
+            acc.y += cp.w * u.x;
+            hit[tl.b][cp.nz] += 1;
+        }
+    }
+    __syncthreads();
+    for (int z = tl.t; z < V; z += T) {
+        const int h = hit[tl.b][z];
+        float2 f = s[lds_pad(z)];
+        if (h) f = make_float2(f.x / (float)h, f.y / (float)h);
+        f = make_float2(f.x * inv_dx, f.y * inv_dx);
+        s[lds_pad(z)] = f;
+    }
+    __syncthreads();
+    if (tl.t == 0) {  // compact list in z order
+        int n = 0;
+        for (int z = 0; z < V; ++z) {
+            if (!hit[tl.b][z]) continue;
+            const float2 f = s[lds_pad(z)];
+            if (refcompat && !(f.x != 0.f && f.y != 0.f)) continue;
+            dc.idx[pencil * dc.cap + n] = (short)z;
+            dc.val[pencil * dc.cap + n] = f;
+            ++n;
+        }
+        dc.cnt[pencil] = n;
+    }
+    float2* pen = W + tl.base;
+    auto gl = [&](int k) -> float2 { return s[lds_pad(k)]; };
+    auto gs = [&](int z, float2 v) { pen[VV * z] = v; };
+    Plan<V>::template run<+1>(tl.t, s, tw, gl, gs);
+}
+
+static DcList dc_alloc(Ctx& c, int V, int cap)
+{
+    const size_t VV = (size_t)V * V;
+    DcList d;
+    d.cap = cap;
+    d.cnt = (int*)c.buf("dc.cnt", sizeof(int) * VV);
+    d.idx = (short*)c.buf("dc.idx", sizeof(short) * VV * cap);
+    d.val = (float2*)c.buf("dc.val", sizeof(float2) * VV * cap);
+    return d;
+}
+
+template <int V>
+struct VolOps {
+    static constexpr int B = 16;
+    static constexpr int NT = B * Plan<V>::T;
+    static dim3 grid() { return dim3((unsigned)((long long)V * V / B)); }
+
+    static void x(Ctx& c, int mode, float2* W, float2* out, const Proj& p, float scale, bool shift)
+    {
+        const float2* tw = c.tw(V);
+        switch (mode) {
+        case 0: hipLaunchKernelGGL((k_vol_x<V, B, 0>), grid(), dim3(NT), 0, c.stream, W, out, tw, p, scale, (int)shift); break;
+        case 1: hipLaunchKernelGGL((k_vol_x<V, B, 1>), grid(), dim3(NT), 0, c.stream, W, out, tw, p, scale, (int)shift); break;
+        case 2: hipLaunchKernelGGL((k_vol_x<V, B, 2>), grid(), dim3(NT), 0, c.stream, W, out, tw, p, scale, (int)shift); break;
+        default: hipLaunchKernelGGL((k_vol_x<V, B, 3>), grid(), dim3(NT), 0, c.stream, W, out, tw, p, scale, (int)shift); break;
+        }
+        CTDM_CHECK_LAUNCH("k_vol_x");
+    }
+    static void y(Ctx& c, float2* W, int dir)
+    {
+        if (dir < 0)
+            hipLaunchKernelGGL((k_vol_y<V, B, -1>), grid(), dim3(NT), 0, c.stream, W, c.tw(V));
+        else
+            hipLaunchKernelGGL((k_vol_y<V, B, +1>), grid(), dim3(NT), 0, c.stream, W, c.tw(V));
+        CTDM_CHECK_LAUNCH("k_vol_y");
+    }
+    static void zdc(Ctx& c, float2* W, const DcList& dc)
+    {
+        hipLaunchKernelGGL((k_vol_zdc<V, B>), grid(), dim3(NT), 0, c.stream, W, c.tw(V), dc);
+        CTDM_CHECK_LAUNCH("k_vol_zdc");
+    }
+    static void grid_pass(Ctx& c, const float2* spec, const double* rows, int A, const EwaldGeom& g, bool ref,
+                          float2* W, const DcList& dc)
+    {
+        hipLaunchKernelGGL((k_vol_grid<V, B>), grid(), dim3(NT), 0, c.stream, spec, rows, A, g.fm02, g.df,
+                           (float)(1.0 / g.dx), (int)ref, c.tw(V), W, dc);
+        CTDM_CHECK_LAUNCH("k_vol_grid");
+    }
+    // Loop after the initial inverse transform of the measured spectrum is in W up to the x pass:
+    // `first_x_mode` 0 if W holds real-space f, 1 if W still needs its inverse x-FFT.
+    static void loop(Ctx& c, float2* W, float2* n_out, const DcList& dc, const Proj& p, int iters, bool shift,
+                     int first_x_mode)
+    {
+        const float s = (float)(1.0 / ((double)V * V * V));
+        if (iters == 0) {
+            if (first_x_mode == 0)
+                x(c, 3, W, n_out, p, 1.f, shift);
+            else
+                x(c, 2, W, n_out, p, s, shift);
+            return;
+        }
+        x(c, first_x_mode, W, n_out, p, s, shift);
+        for (int it = 0; it < iters; ++it) {
+            y(c, W, -1);
+            zdc(c, W, dc);
+            y(c, W, +1);
+            x(c, it + 1 < iters ? 1 : 2, W, n_out, p, s, shift);
+        }
+    }
+};
+
+template <int V>
+static void positivity_fused_v(Ctx& c, float2* f, const float2* Fm, const float* hits, float2* n, int iters,
+                               const Proj& p, unsigned flags)
+{
+    const bool ref = flags & CTDM_DC_REFCOMPAT;
+    const DcList dc = dc_alloc(c, V, V);
+    hipLaunchKernelGGL(k_dc_build, dim3(grid1d((long long)V * V, 256)), dim3(256), 0, c.stream, Fm, hits, V,
+                       (int)ref, dc);
+    CTDM_CHECK_LAUNCH("k_dc_build");
+    VolOps<V>::loop(c, f, n, dc, p, iters, !(flags & CTDM_POS_NO_SHIFT), 0);
+}
+
+void positivity_fused(Ctx& c, float2* f, const float2* Fm, const float* hits, float2* n, int V, int iters,
+                      const ctdm_optics& o, unsigned flags)
+{
+    CTDM_REQUIRE(f && Fm && n && f != n, "positivity: bad pointers");
+    const bool ref = flags & CTDM_DC_REFCOMPAT;
+    CTDM_REQUIRE(hits || ref, "positivity: hits needed unless CTDM_DC_REFCOMPAT");
+    CTDM_REQUIRE(iters >= 0, "positivity: negative iteration count");
+    const Proj p = make_proj(ewald_geom(o, V));
+    switch (V) {
+    case 64: positivity_fused_v<64>(c, f, Fm, hits, n, iters, p, flags); break;
+    case 128: positivity_fused_v<128>(c, f, Fm, hits, n, iters, p, flags); break;
+    case 192: positivity_fused_v<192>(c, f, Fm, hits, n, iters, p, flags); break;
+    case 256: positivity_fused_v<256>(c, f, Fm, hits, n, iters, p, flags); break;
+    default: fail(CTDM_EUNSUPPORTED, "fused positivity: volume side has no kernel");
+    }
+}
+
+template <int V>
+static void volume_batched_v(Ctx& c, const float2* spec, const double* rows, int A, const EwaldGeom& g, int iters,
+                             unsigned flags, float2* n_out)
+{
+    const DcList dc = dc_alloc(c, V, std::max(A, 1));
+    float2* W = (float2*)c.buf("vol.W", sizeof(float2) * (size_t)V * V * V);
+    VolOps<V>::grid_pass(c, spec, rows, A, g, flags & CTDM_DC_REFCOMPAT, W, dc);
+    VolOps<V>::y(c, W, +1);
+    VolOps<V>::loop(c, W, n_out, dc, make_proj(g), iters, !(flags & CTDM_POS_NO_SHIFT), 1);
+}
+
+void volume_batched(Ctx& c, const float2* spec, const ctdm_angle* ang, int A, int V, const ctdm_optics& o,
+                    int iters, unsigned flags, float2* n_out)
+{
+    CTDM_REQUIRE(spec && n_out, "volume: null pointer");
+    CTDM_REQUIRE(A >= 0 && A < 32768 && iters >= 0, "volume: bad sizes");
+    const EwaldGeom g = ewald_geom(o, V);
+    const double* rows = upload_rows(c, "vol.rows." + std::to_string((uintptr_t)n_out), angle_rows(g, ang, A));
+    switch (V) {
+    case 64: volume_batched_v<64>(c, spec, rows, A, g, iters, flags, n_out); break;
+    case 128: volume_batched_v<128>(c, spec, rows, A, g, iters, flags, n_out); break;
+    case 192: volume_batched_v<192>(c, spec, rows, A, g, iters, flags, n_out); break;
+    case 256: volume_batched_v<256>(c, spec, rows, A, g, iters, flags, n_out); break;
+    default: fail(CTDM_EUNSUPPORTED, "volume side has no kernel");
+    }
+}
+
+// Scatter of one angle's spectrum, reference-structured (used by the plain pipeline).
+void scatter_plain_one(Ctx& c, const float2* u_sp, const AngleRow& r, int V, const EwaldGeom& g, float2* F,
+                       float* hits)
+{
+    dim3 blk(32, 8), grd((V + 31) / 32, (V + 7) / 8);
+    hipLaunchKernelGGL(k_scatter_one, grd, blk, 0, c.stream, u_sp, r.fx0, r.fy0, r.fz0, V, g.fm02, g.df, F, hits);
+    CTDM_CHECK_LAUNCH("k_scatter_one");
+}
+
+void zero_volume(Ctx& c, float2* F, float* hits, int V)
+{
+    const long long n = (long long)V * V * V;
+    fill(c, F, 2 * n, 0.f);
+    if (hits) fill(c, hits, n, 0.f);
+}
+
+}  // namespace ctdm

@@ -1,0 +1,197 @@
+// Cell pipelines: hologram pairs -> RI volume.
+//
+// plain   (mode 0): the reference's structure rebuilt on this library's kernels — per angle,
+//                   full 2-D transforms, separate shift/crop/scale kernels, the 14-kernel DCT
+//                   unwrapper, per-angle Rytov FFT and scatter, unfused positivity loop (10 volume
+//                   passes per iteration), host-synchronous between stages (the reference's
+//                   cudaMemcpy / cudaThreadSynchronize points, HilbertTransform.cu:785-1165,
+//                   1825-2047, FFT.cu:655-736).
+// batched (mode 1): all angles of a cell per launch, fused passes (3 retrieval, 3 unwrap,
+//                   2 Rytov, 3 + 4*iters volume kernels), optionally captured once into a hipGraph
+//                   and replayed.
+#include <cmath>
+#include <cstring>
+
+#include "internal.hpp"
+
+namespace ctdm {
+
+void scatter_plain_one(Ctx& c, const float2* u_sp, const AngleRow& r, int V, const EwaldGeom& g, float2* F,
+                       float* hits);
+void zero_volume(Ctx& c, float2* F, float* hits, int V);
+
+static void validate_cell(const ctdm_cell& cl)
+{
+    CTDM_REQUIRE(cl.holo && cl.bg && cl.angles && cl.ri, "cell: null pointer");
+    CTDM_REQUIRE(cl.A > 0 && cl.H > 0 && cl.W > 0, "cell: empty");
+    CTDM_REQUIRE(cl.V == 64 || cl.V == 128 || cl.V == 192 || cl.V == 256, "cell: volume side has no kernel");
+    CTDM_REQUIRE(cl.iters >= 0, "cell: negative iteration count");
+    CTDM_REQUIRE(cl.crop.radius < 0 || cl.crop.M == cl.V, "cell: crop side must equal the volume side");
+    CTDM_REQUIRE(cl.optics.wavelength > 0 && cl.optics.n_med > 0 && cl.optics.dx > 0, "cell: bad optics");
+}
+
+static ctdm_crop cell_crop(Ctx& c, const ctdm_cell& cl)
+{
+    if (cl.crop.radius >= 0) return cl.crop;
+    ctdm_crop cr{};
+    find_sideband(c, cl.bg, cl.H, cl.W, cl.V, 0, &cr);  // frame 1 of the reference (HilbertTransform.cu:1572)
+    return cr;
+}
+
+__global__ void k_wrap_log(const float* __restrict__ phase, const float* __restrict__ amp, float* __restrict__ w,
+                           float* __restrict__ la, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (w) {
+        const float p = phase[i];
+        w[i] = p - 6.283185307179586f * rintf(p / 6.283185307179586f);
+    }
+    if (la) {
+        const float l = logf(amp[i]);
+        la[i] = isfinite(l) ? l : 0.f;
+    }
+}
+
+static void run_plain(Ctx& c, ctdm_cell& cl)
+{
+    const int A = cl.A, H = cl.H, W = cl.W, V = cl.V;
+    const size_t img = (size_t)V * V;
+    cl.crop = cell_crop(c, cl);
+    float* phase = (float*)c.buf("pl.phase", sizeof(float) * A * img);
+    float* amp = (float*)c.buf("pl.amp", sizeof(float) * A * img);
+    float* unw = cl.unwrapped_opt ? cl.unwrapped_opt : (float*)c.buf("pl.unw", sizeof(float) * A * img);
+    for (int a = 0; a < A; ++a) {
+        retrieve_plain_one(c, cl.holo + (size_t)a * H * W, cl.bg + (size_t)a * H * W, H, W, cl.crop,
+                           phase + a * img, amp + a * img);
+        CTDM_HIP(hipStreamSynchronize(c.stream));
+        unwrap_dct_plain_one(c, phase + a * img, unw + a * img, V, cl.unwrap_flags);
+        CTDM_HIP(hipStreamSynchronize(c.stream));
+    }
+    if (cl.wrapped_opt || cl.logamp_opt) {
+        const long long n = (long long)A * img;
+        hipLaunchKernelGGL(k_wrap_log, dim3(grid1d(n, 256)), dim3(256), 0, c.stream, phase, amp, cl.wrapped_opt,
+                           cl.logamp_opt, n);
+        CTDM_CHECK_LAUNCH("k_wrap_log");
+    }
+    const EwaldGeom g = ewald_geom(cl.optics, V);
+    const auto rows = angle_rows(g, cl.angles, A);
+    float2* F = (float2*)c.buf("pl.F", sizeof(float2) * img * V);
+    float* hits = (float*)c.buf("pl.hits", sizeof(float) * img * V);
+    float2* Fm = (float2*)c.buf("pl.Fm", sizeof(float2) * img * V);
+    float2* u_sp = (float2*)c.buf("pl.usp", sizeof(float2) * img);
+    zero_volume(c, F, hits, V);
+    for (int a = 0; a < A; ++a) {
+        rytov_spectrum_plain_one(c, unw + a * img, amp + a * img, V, u_sp);
+        scatter_plain_one(c, u_sp, rows[a], V, g, F, hits);
+        CTDM_HIP(hipStreamSynchronize(c.stream));
+    }
+    normalize_measured(c, F, hits, Fm, V, g.dx);
+    fft3d(c, F, V, +1);
+    CTDM_HIP(hipStreamSynchronize(c.stream));
+    positivity_plain(c, F, Fm, hits, (float2*)cl.ri, V, cl.iters, cl.optics, cl.recon_flags);
+    CTDM_HIP(hipStreamSynchronize(c.stream));
+}
+
+static void enqueue_batched(Ctx& c, ctdm_cell& cl)
+{
+    const int A = cl.A, V = cl.V;
+    const size_t n = (size_t)A * V * V;
+    // per-cell buffers (graph replays see stable pointers)
+    const std::string key = std::to_string((uintptr_t)cl.ri);
+    float* wr = cl.wrapped_opt ? cl.wrapped_opt : (float*)c.buf("bt.wr." + key, sizeof(float) * n);
+    float* la = cl.logamp_opt ? cl.logamp_opt : (float*)c.buf("bt.la." + key, sizeof(float) * n);
+    float* un = cl.unwrapped_opt ? cl.unwrapped_opt : (float*)c.buf("bt.un." + key, sizeof(float) * n);
+    float2* spec = (float2*)c.buf("bt.spec." + key, sizeof(float2) * n);
+    retrieve_batched(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.crop, wr, la, nullptr);
+    unwrap_dct_batched(c, wr, un, A, V, cl.unwrap_flags);
+    rytov_spectrum(c, un, la, A, V, spec);
+    volume_batched(c, spec, cl.angles, A, V, cl.optics, cl.iters, cl.recon_flags, (float2*)cl.ri);
+}
+
+static GraphKey graph_key(const ctdm_cell* cells, int ncells)
+{
+    GraphKey k;
+    for (int i = 0; i < ncells; ++i) {
+        const ctdm_cell& cl = cells[i];
+        for (uintptr_t w : {(uintptr_t)cl.holo, (uintptr_t)cl.bg, (uintptr_t)cl.ri, (uintptr_t)cl.wrapped_opt,
+                            (uintptr_t)cl.unwrapped_opt, (uintptr_t)cl.logamp_opt, (uintptr_t)cl.A,
+                            (uintptr_t)cl.H, (uintptr_t)cl.W, (uintptr_t)cl.V, (uintptr_t)cl.iters,
+                            (uintptr_t)cl.unwrap_flags, (uintptr_t)cl.recon_flags, (uintptr_t)cl.crop.row,
+                            (uintptr_t)cl.crop.col, (uintptr_t)cl.crop.radius, (uintptr_t)cl.crop.M})
+            k.words.push_back(w);
+        uint64_t bits[3];
+        std::memcpy(&bits[0], &cl.optics.wavelength, 8);
+        std::memcpy(&bits[1], &cl.optics.n_med, 8);
+        std::memcpy(&bits[2], &cl.optics.dx, 8);
+        for (uint64_t b : bits) k.words.push_back((uintptr_t)b);
+        for (int a = 0; a < cl.A; ++a) {  // angle values, not just the host pointer
+            uint64_t ax, ay;
+            std::memcpy(&ax, &cl.angles[a].ang_x, 8);
+            std::memcpy(&ay, &cl.angles[a].ang_y, 8);
+            k.words.push_back((uintptr_t)ax);
+            k.words.push_back((uintptr_t)ay);
+        }
+    }
+    return k;
+}
+
+void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& opts)
+{
+    CTDM_REQUIRE(ncells >= 0, "reconstruct: negative cell count");
+    for (int i = 0; i < ncells; ++i) validate_cell(cells[i]);
+    if (opts.mode == 0) {
+        for (int i = 0; i < ncells; ++i) run_plain(c, cells[i]);
+        return;
+    }
+    CTDM_REQUIRE(opts.mode == 1, "reconstruct: mode must be 0 (plain) or 1 (batched)");
+    for (int i = 0; i < ncells; ++i) cells[i].crop = cell_crop(c, cells[i]);  // host-synchronous, never captured
+    if (opts.autotune) {
+        std::vector<int> sizes;
+        for (int i = 0; i < ncells; ++i) sizes.push_back(cells[i].V);
+        autotune(c, sizes.data(), (int)sizes.size());
+    }
+    if (!opts.graph) {
+        for (int i = 0; i < ncells; ++i) enqueue_batched(c, cells[i]);
+        return;
+    }
+    const GraphKey key = graph_key(cells, ncells);
+    auto it = c.graphs.find(key);
+    if (it == c.graphs.end()) {
+        // Eager pass: allocates every buffer and table the sequence needs and produces the result.
+        for (int i = 0; i < ncells; ++i) enqueue_batched(c, cells[i]);
+        CTDM_HIP(hipStreamSynchronize(c.stream));
+        hipStream_t cs = nullptr;
+        CTDM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipStream_t saved = c.stream;
+        c.stream = cs;
+        c.capturing = true;
+        GraphEntry ge;
+        try {
+            CTDM_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            try {
+                for (int i = 0; i < ncells; ++i) enqueue_batched(c, cells[i]);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                hipStreamEndCapture(cs, &g);
+                if (g) hipGraphDestroy(g);
+                throw;
+            }
+            CTDM_HIP(hipStreamEndCapture(cs, &ge.graph));
+            CTDM_HIP(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0));
+        } catch (...) {
+            c.capturing = false;
+            c.stream = saved;
+            hipStreamDestroy(cs);
+            throw;
+        }
+        c.capturing = false;
+        c.stream = saved;
+        CTDM_HIP(hipStreamDestroy(cs));
+        c.graphs[key] = ge;
+        return;
+    }
+    CTDM_HIP(hipGraphLaunch(it->second.exec, c.stream));
+}
+
+}  // namespace ctdm

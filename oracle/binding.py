@@ -52,7 +52,19 @@ def _declare(L):
             getattr(L, name).argtypes = args
 
 
-_EXTRA: dict = {}
+_EXTRA: dict = {
+    "orc_rytov_spectrum": [_dp, _dp, C.c_int, C.c_int, _dp],
+    "orc_ewald_scatter": [_dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp],
+    "orc_ewald_gather": [_dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp],
+    "orc_ewald_index": [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _ip],
+    "orc_normalize": [_dp, _dp, _dp, C.c_int, C.c_double],
+    "orc_est_n": [_dp, _dp, C.c_long, C.c_double, C.c_double],
+    "orc_modify_f": [_dp, _dp, C.c_long, C.c_double, C.c_double],
+    "orc_positivity": [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint],
+    "orc_reconstruct_cell": [_fp, _fp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                             C.c_double, C.c_double, C.c_double, C.c_int, C.c_uint, C.c_uint, _dp,
+                             _dp, _dp, _dp],
+}
 
 
 def _p(a: np.ndarray | None, t):
@@ -118,3 +130,98 @@ def unwrap_dct(x: np.ndarray, flags: int = 0) -> np.ndarray:
     out = np.zeros_like(a)
     lib().orc_unwrap_dct(_p(a, _dp), _p(out, _dp), a.shape[0], a.shape[-1], flags)
     return out
+
+
+# ---------------------------------------------------------------- reconstruction
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _cp(a):
+    return None if a is None else a.view(np.float64).ctypes.data_as(_dp)
+
+
+def rytov_spectrum(phase, logamp):
+    ph = np.ascontiguousarray(phase, np.float64)
+    la = np.ascontiguousarray(logamp, np.float64)
+    A, M = ph.shape[0], ph.shape[-1]
+    out = np.zeros((A, M, M), np.complex128)
+    lib().orc_rytov_spectrum(_p(ph, _dp), _p(la, _dp), A, M, _cp(out))
+    return out
+
+
+def _ang(angles):
+    return np.ascontiguousarray(np.asarray(angles, np.float64).reshape(-1, 2))
+
+
+def ewald_scatter(spec, angles, V, wavelength, n_med, dx, F=None, hits=None):
+    s = _c(spec)
+    a = _ang(angles)
+    F = np.zeros((V, V, V), np.complex128) if F is None else _c(F).copy()
+    hits = np.zeros((V, V, V)) if hits is None else np.ascontiguousarray(hits, np.float64).copy()
+    lib().orc_ewald_scatter(_cp(s), _p(a, _dp), len(a), V, wavelength, n_med, dx, _cp(F), _p(hits, _dp))
+    return F, hits
+
+
+def ewald_gather(F, angles, wavelength, n_med, dx):
+    Fc = _c(F)
+    a = _ang(angles)
+    V = Fc.shape[0]
+    out = np.zeros((len(a), V, V), np.complex128)
+    lib().orc_ewald_gather(_cp(Fc), _p(a, _dp), len(a), V, wavelength, n_med, dx, _cp(out))
+    return out
+
+
+def ewald_index(angles, V, wavelength, n_med, dx):
+    a = _ang(angles)
+    out = np.zeros((len(a), V, V), np.int32)
+    lib().orc_ewald_index(_p(a, _dp), len(a), V, wavelength, n_med, dx, _p(out, _ip))
+    return out
+
+
+def normalize(F, hits, dx):
+    Fc = _c(F).copy()
+    h = np.ascontiguousarray(hits, np.float64)
+    Fm = np.zeros_like(Fc)
+    lib().orc_normalize(_cp(Fc), _p(h, _dp), _cp(Fm), Fc.shape[0], dx)
+    return Fc, Fm
+
+
+def est_n(f, wavelength, n_med):
+    fc = _c(f)
+    out = np.zeros_like(fc)
+    lib().orc_est_n(_cp(fc), _cp(out), fc.size, wavelength, n_med)
+    return out
+
+
+def modify_f(n, wavelength, n_med):
+    nc = _c(n)
+    out = np.zeros_like(nc)
+    lib().orc_modify_f(_cp(nc), _cp(out), nc.size, wavelength, n_med)
+    return out
+
+
+def positivity(f, Fm, hits, iters, wavelength, n_med, flags=0):
+    fc = _c(f).copy()
+    mc = _c(Fm)
+    h = None if hits is None else np.ascontiguousarray(hits, np.float64)
+    n = np.zeros_like(fc)
+    lib().orc_positivity(_cp(fc), _cp(mc), _p(h, _dp), _cp(n), fc.shape[0], iters, wavelength, n_med, flags)
+    return n
+
+
+def reconstruct_cell(holo, bg, angles, V, crop, wavelength, n_med, dx, iters, unwrap_flags=0,
+                     recon_flags=0, outputs=False):
+    h = np.ascontiguousarray(holo, np.float32)
+    b = np.ascontiguousarray(bg, np.float32)
+    a = _ang(angles)
+    A, H = h.shape[0], h.shape[-1]
+    ri = np.zeros((V, V, V), np.complex128)
+    extra = [np.zeros((A, V, V)) for _ in range(3)] if outputs else [None] * 3
+    row, col, rad = crop[:3]
+    lib().orc_reconstruct_cell(_p(h, _fp), _p(b, _fp), _p(a, _dp), A, H, V, row, col, rad, wavelength,
+                               n_med, dx, iters, unwrap_flags, recon_flags, _cp(ri),
+                               *[_p(e, _dp) for e in extra])
+    if outputs:
+        return ri, dict(zip(("wrapped", "unwrapped", "logamp"), extra))
+    return ri

@@ -90,11 +90,40 @@ void fft_rec(const cd* in, cd* out, int n, long stride, int dir, const std::vect
     }
 }
 
+// Iterative radix-2 transform for powers of two (bit reversal + butterflies).
+void fft_pow2(cd* a, int n, int dir)
+{
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    const auto& tw = twiddles(n);
+    for (int len = 2; len <= n; len <<= 1) {
+        const int half = len >> 1, step = n / len;
+        for (int i = 0; i < n; i += len)
+            for (int k = 0; k < half; ++k) {
+                const cd w = dir < 0 ? tw[k * step] : std::conj(tw[k * step]);
+                const cd u = a[i + k], v = a[i + k + half] * w;
+                a[i + k] = u + v;
+                a[i + k + half] = u - v;
+            }
+    }
+}
+
 void fft1(cd* data, int n, long stride, int dir)
 {
-    const auto& tw = twiddles(n);
-    std::vector<cd> in(n), out(n);
+    thread_local std::vector<cd> in, out;
+    in.resize(n);
     for (int i = 0; i < n; ++i) in[i] = data[i * stride];
+    if ((n & (n - 1)) == 0) {
+        fft_pow2(in.data(), n, dir);
+        for (int i = 0; i < n; ++i) data[i * stride] = in[i];
+        return;
+    }
+    out.resize(n);
+    const auto& tw = twiddles(n);
     fft_rec(in.data(), out.data(), n, 1, dir, tw, 1);
     for (int i = 0; i < n; ++i) data[i * stride] = out[i];
 }
@@ -462,6 +491,262 @@ void orc_unwrap_dct(const double* in, double* out, int batch, int N, unsigned fl
     for (int b = 0; b < batch; ++b)
         orc_dct_chain(in + (size_t)b * N * N, N, flags, 0, nullptr, nullptr, nullptr,
                       out + (size_t)b * N * N);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- reconstruction (BatchRecon)
+namespace {
+
+struct Geom {
+    double fm0, fm02, df, dx, k2, nm;
+};
+
+// BatchRecon constants (HilbertTransform.cu:1840-1862) in FP64.
+Geom geom(double wavelength, double n_med, double dx, int V)
+{
+    Geom g;
+    g.fm0 = n_med / wavelength;
+    g.fm02 = g.fm0 * g.fm0;
+    g.dx = dx;
+    g.df = 1.0 / ((double)V * dx);
+    const double k = 2.0 * PI * n_med / wavelength;
+    g.k2 = k * k;
+    g.nm = n_med;
+    return g;
+}
+
+struct Row {
+    double fx0, fy0, fz0;
+};
+
+// EdwardSphere (Recon_kernel.cu:171-185): f_x0 from AngY, f_y0 from AngX.
+Row angle_row(const Geom& g, double ang_x, double ang_y)
+{
+    const double fx0 = g.fm0 * std::sin(ang_y);
+    const double fy0 = g.fm0 * std::sin(ang_x);
+    const double s0 = g.fm02 - fx0 * fx0 - fy0 * fy0;
+    return Row{fx0, fy0, std::sqrt(s0 > 0 ? s0 : 0.0)};
+}
+
+// fillEdwardSphere (Recon_kernel.cu:187-229) geometry for signed frequencies (ii, jj):
+// returns false off the cap; nz = mod(cuRound((f_z - f_z0)/df), V) with cuRound (:236-239).
+bool cap(const Geom& g, const Row& r, int ii, int jj, int V, int* nz, double* w)
+{
+    const double fx = (double)ii * g.df + r.fx0;
+    const double fy = (double)jj * g.df + r.fy0;
+    const double s = g.fm02 - (fx * fx + fy * fy);
+    const double s0 = g.fm02 - r.fx0 * r.fx0 - r.fy0 * r.fy0;
+    if (s0 < 0.0 || s < 0.0) return false;
+    const double fz = std::sqrt(s);
+    const double q = (fz - r.fz0) / g.df;
+    const double rn = q > 0.0 ? std::floor(q + 0.5) : std::ceil(q - 0.5);
+    int m = (int)rn % V;
+    if (m < 0) m += V;
+    *nz = m;
+    *w = 4.0 * PI * fz;
+    return true;
+}
+
+// calculate_n3D (Recon_kernel.cu:305-342)
+cd est_n(cd f, const Geom& g)
+{
+    const double c = f.real() / (-g.k2) + 1.0;
+    const double d = f.imag() / (-g.k2);
+    const double r = std::sqrt(c * c + d * d);
+    const double a = std::sqrt((r + c) / 2.0);
+    const double b = (d != 0.0) ? std::sqrt(std::max(0.0, (r - c) / 2.0)) * (d / std::fabs(d)) : 0.0;
+    double nr = a * g.nm, ni = b * g.nm;
+    if (nr < g.nm) nr = g.nm;
+    if (ni < 0) ni = 0;
+    return cd(nr, ni);
+}
+
+// Modify_F3D (Recon_kernel.cu:370-383) called with n_m^2 (HilbertTransform.cu:1962).
+cd modify_f(cd n, const Geom& g)
+{
+    const double nm2 = g.nm * g.nm;
+    return cd(((n.real() * n.real() - n.imag() * n.imag()) / nm2 - 1.0) * (-g.k2),
+              (n.real() * n.imag() * 2.0) / nm2 * (-g.k2));
+}
+
+}  // namespace
+
+extern "C" {
+
+// cuCopyArray (Recon_kernel.cu:154-169) + cuFFT2Dshift + cuFFT2D(-1) (HilbertTransform.cu:1909-1913)
+// for A maps: spec[a] (interleaved, M x M).
+void orc_rytov_spectrum(const double* phase, const double* logamp, int A, int M, double* spec)
+{
+#pragma omp parallel for schedule(dynamic)
+    for (int a = 0; a < A; ++a) {
+        std::vector<cd> u((size_t)M * M);
+        const double* ph = phase + (size_t)a * M * M;
+        const double* la = logamp + (size_t)a * M * M;
+        for (int j = 0; j < M; ++j)
+            for (int i = 0; i < M; ++i) u[j + (size_t)M * i] = cd(la[i + (size_t)M * j], ph[i + (size_t)M * j]);
+        shift2(u.data(), M, M);
+        for (int y = 0; y < M; ++y) fft1(u.data() + (size_t)y * M, M, 1, -1);
+        for (int x = 0; x < M; ++x) fft1(u.data() + x, M, M, -1);
+        std::memcpy(spec + (size_t)a * 2 * M * M, u.data(), sizeof(cd) * M * M);
+    }
+}
+
+// EdwardSphere/fillEdwardSphere over all A angles in order (accumulates into F, hits).
+void orc_ewald_scatter(const double* spec, const double* angles, int A, int V, double wavelength, double n_med,
+                       double dx, double* F, double* hits)
+{
+    const Geom g = geom(wavelength, n_med, dx, V);
+    cd* Fc = reinterpret_cast<cd*>(F);
+    const cd* S = reinterpret_cast<const cd*>(spec);
+    const long VV = (long)V * V;
+    for (int a = 0; a < A; ++a) {
+        const Row r = angle_row(g, angles[2 * a], angles[2 * a + 1]);
+        for (int j = 0; j < V; ++j)
+            for (int i = 0; i < V; ++i) {
+                const int ii = i - V / 2, jj = j - V / 2;
+                int nz;
+                double w;
+                if (!cap(g, r, ii, jj, V, &nz, &w)) continue;
+                const int Nx = ii < 0 ? ii + V : ii, Ny = jj < 0 ? jj + V : jj;
+                const cd u = S[(long)a * VV + Nx + (long)V * Ny];
+                const long v = Nx + (long)V * Ny + VV * nz;
+                Fc[v] += cd(-w * u.imag(), w * u.real());
+                hits[v] += 1.0;
+            }
+    }
+}
+
+// Adjoint of orc_ewald_scatter: spec[a][p] = conj(4 pi i f_z) F[v] on the cap, 0 elsewhere.
+void orc_ewald_gather(const double* F, const double* angles, int A, int V, double wavelength, double n_med,
+                      double dx, double* spec)
+{
+    const Geom g = geom(wavelength, n_med, dx, V);
+    const cd* Fc = reinterpret_cast<const cd*>(F);
+    cd* S = reinterpret_cast<cd*>(spec);
+    const long VV = (long)V * V;
+    for (int a = 0; a < A; ++a) {
+        const Row r = angle_row(g, angles[2 * a], angles[2 * a + 1]);
+        for (long p = 0; p < VV; ++p) {
+            const int Nx = (int)(p % V), Ny = (int)(p / V);
+            const int ii = Nx < V / 2 ? Nx : Nx - V, jj = Ny < V / 2 ? Ny : Ny - V;
+            int nz;
+            double w;
+            S[a * VV + p] = cap(g, r, ii, jj, V, &nz, &w) ? std::conj(cd(0.0, w)) * Fc[p + VV * nz] : cd(0, 0);
+        }
+    }
+}
+
+// Voxel index of every (angle, spectrum pixel): -1 off the cap (for index-parity tests).
+void orc_ewald_index(const double* angles, int A, int V, double wavelength, double n_med, double dx, int* nz_out)
+{
+    const Geom g = geom(wavelength, n_med, dx, V);
+    const long VV = (long)V * V;
+    for (int a = 0; a < A; ++a) {
+        const Row r = angle_row(g, angles[2 * a], angles[2 * a + 1]);
+        for (long p = 0; p < VV; ++p) {
+            const int Nx = (int)(p % V), Ny = (int)(p / V);
+            const int ii = Nx < V / 2 ? Nx : Nx - V, jj = Ny < V / 2 ? Ny : Ny - V;
+            int nz;
+            double w;
+            nz_out[a * VV + p] = cap(g, r, ii, jj, V, &nz, &w) ? nz : -1;
+        }
+    }
+}
+
+// Initial_F_F2 (Recon_kernel.cu:253-276): F /= C where C != 0; F2 = F / dx; F = F2.
+void orc_normalize(double* F, const double* hits, double* Fm, int V, double dx)
+{
+    cd* Fc = reinterpret_cast<cd*>(F);
+    cd* Mc = reinterpret_cast<cd*>(Fm);
+    const long n = (long)V * V * V;
+    for (long i = 0; i < n; ++i) {
+        if (hits[i] != 0.0) Fc[i] /= hits[i];
+        Fc[i] /= dx;
+        if (Mc) Mc[i] = Fc[i];
+    }
+}
+
+void orc_est_n(const double* f, double* n, long cnt, double wavelength, double n_med)
+{
+    const Geom g = geom(wavelength, n_med, 1.0, 1);
+    for (long i = 0; i < cnt; ++i) {
+        const cd v = est_n(cd(f[2 * i], f[2 * i + 1]), g);
+        n[2 * i] = v.real();
+        n[2 * i + 1] = v.imag();
+    }
+}
+
+void orc_modify_f(const double* n, double* f, long cnt, double wavelength, double n_med)
+{
+    const Geom g = geom(wavelength, n_med, 1.0, 1);
+    for (long i = 0; i < cnt; ++i) {
+        const cd v = modify_f(cd(n[2 * i], n[2 * i + 1]), g);
+        f[2 * i] = v.real();
+        f[2 * i + 1] = v.imag();
+    }
+}
+
+// Positivity loop of BatchRecon (HilbertTransform.cu:1940-1975), flags: 1 = reference mask
+// (F2.re != 0 && F2.im != 0, Recon_kernel.cu:406) instead of hits > 0; 2 = no final octant shift.
+void orc_positivity(double* f, const double* Fm, const double* hits, double* n, int V, int iters,
+                    double wavelength, double n_med, unsigned flags)
+{
+    const Geom g = geom(wavelength, n_med, 1.0, V);
+    cd* fc = reinterpret_cast<cd*>(f);
+    const cd* mc = reinterpret_cast<const cd*>(Fm);
+    cd* nc = reinterpret_cast<cd*>(n);
+    const long cnt = (long)V * V * V;
+    const double s = 1.0 / (double)cnt;
+    const bool ref = flags & 1u;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < cnt; ++i) nc[i] = est_n(fc[i], g);
+    for (int it = 0; it < iters; ++it) {
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < cnt; ++i) fc[i] = modify_f(nc[i], g);
+        fft3(fc, V, -1);
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < cnt; ++i) {
+            const bool meas = ref ? (mc[i].real() != 0.0 && mc[i].imag() != 0.0) : (hits[i] > 0.0);
+            if (meas) fc[i] = mc[i];
+        }
+        fft3(fc, V, +1);
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < cnt; ++i) nc[i] = est_n(fc[i] * s, g);
+    }
+    if (!(flags & 2u)) shift3(nc, V);
+}
+
+// Whole cell: A hologram pairs (float32, A x H x H) -> RI volume (interleaved complex, V^3).
+// crop = (row, col, radius); angles = (ang_x, ang_y) pairs; unwrap_flags as orc_unwrap_dct;
+// recon_flags as orc_positivity. Optional per-angle outputs (V x V each): wrapped (-pi, pi],
+// unwrapped, logamp.
+void orc_reconstruct_cell(const float* holo, const float* bg, const double* angles, int A, int H, int V,
+                          int row, int col, int radius, double wavelength, double n_med, double dx, int iters,
+                          unsigned unwrap_flags, unsigned recon_flags, double* ri, double* wrapped_out,
+                          double* unwrapped_out, double* logamp_out)
+{
+    const size_t img = (size_t)V * V, hh = (size_t)H * H;
+    std::vector<double> phase(A * img), la(A * img), unw(A * img);
+    for (int a = 0; a < A; ++a) {
+        std::vector<double> pr(img), am(img), lg(img), wr(img);
+        orc_retrieve(holo + a * hh, bg + a * hh, H, H, row, col, radius, V, pr.data(), am.data(), wr.data(),
+                     lg.data(), nullptr);
+        std::memcpy(phase.data() + a * img, pr.data(), sizeof(double) * img);
+        std::memcpy(la.data() + a * img, lg.data(), sizeof(double) * img);
+        if (wrapped_out) std::memcpy(wrapped_out + a * img, wr.data(), sizeof(double) * img);
+    }
+    orc_unwrap_dct(phase.data(), unw.data(), A, V, unwrap_flags);
+    if (unwrapped_out) std::memcpy(unwrapped_out, unw.data(), sizeof(double) * A * img);
+    if (logamp_out) std::memcpy(logamp_out, la.data(), sizeof(double) * A * img);
+    std::vector<double> spec(2 * A * img);
+    orc_rytov_spectrum(unw.data(), la.data(), A, V, spec.data());
+    const size_t vol = img * V;
+    std::vector<double> F(2 * vol, 0.0), hits(vol, 0.0), Fm(2 * vol);
+    orc_ewald_scatter(spec.data(), angles, A, V, wavelength, n_med, dx, F.data(), hits.data());
+    orc_normalize(F.data(), hits.data(), Fm.data(), V, dx);
+    orc_fft3d(F.data(), V, +1);
+    orc_positivity(F.data(), Fm.data(), hits.data(), ri, V, iters, wavelength, n_med, recon_flags);
 }
 
 }  // extern "C"

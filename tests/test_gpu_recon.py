@@ -1,0 +1,194 @@
+"""GPU parity of the reconstruction stage against the FP64 oracle (and rocFFT via torch.fft for the
+3-D transform). Bars: Ewald voxel assignment and hit counts bit-exact; spectra / volumes within FP32
+rounding of the oracle; RI (complex n) within 1e-5 absolute of the oracle."""
+import numpy as np
+import pytest
+import torch
+
+import ctdm_amd
+
+pytestmark = pytest.mark.gpu
+LAM, NM = 532e-9, 1.337
+
+
+def _angles(A, na=1.0):
+    s = na / NM
+    az = 2 * np.pi * np.arange(A) / A
+    return np.stack([np.arcsin(s * np.sin(az)), np.arcsin(s * np.cos(az))], axis=1)
+
+
+def _cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _relmax(got, ref):
+    return float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-30))
+
+
+@pytest.mark.parametrize("M", [64, 128, 192, 256])
+def test_rytov_spectrum_matches_oracle(ctx, orc, M):
+    rng = np.random.default_rng(M)
+    ph = rng.standard_normal((3, M, M)).astype(np.float32)
+    la = (0.1 * rng.standard_normal((3, M, M))).astype(np.float32)
+    got = ctx.rytov_spectrum(_cuda(ph), _cuda(la)).cpu().numpy()
+    ref = orc.rytov_spectrum(ph, la)
+    assert _relmax(got, ref) <= 2e-6
+
+
+@pytest.mark.parametrize("V,A", [(64, 60), (256, 60), (128, 7)])
+def test_scatter_voxels_and_hits_bit_exact(ctx, orc, V, A):
+    o = ctdm_amd.optics_for(V)
+    ang = _angles(A)
+    spec = torch.ones((A, V, V), dtype=torch.complex64, device="cuda")
+    _, hits = ctx.ewald_scatter(spec, ang, V, o)
+    idx = orc.ewald_index(ang, V, LAM, NM, o.dx)
+    ref = np.zeros(V * V * V)
+    for a in range(A):
+        on = idx[a].ravel() >= 0
+        p = np.arange(V * V)[on]
+        np.add.at(ref, p + V * V * idx[a].ravel()[on], 1.0)
+    assert np.array_equal(hits.cpu().numpy().ravel(), ref)
+
+
+def test_scatter_and_gather_values(ctx, orc):
+    V, A = 64, 9
+    o = ctdm_amd.optics_for(V)
+    ang = _angles(A)
+    rng = np.random.default_rng(5)
+    spec = (rng.standard_normal((A, V, V)) + 1j * rng.standard_normal((A, V, V))).astype(np.complex64)
+    F, hits = ctx.ewald_scatter(_cuda(spec), ang, V, o)
+    Fr, hr = orc.ewald_scatter(spec, ang, V, LAM, NM, o.dx)
+    assert np.array_equal(hits.cpu().numpy(), hr)
+    assert _relmax(F.cpu().numpy(), Fr) <= 1e-6
+    vol = (rng.standard_normal((V, V, V)) + 1j * rng.standard_normal((V, V, V))).astype(np.complex64)
+    G = ctx.ewald_gather(_cuda(vol), ang, V, o).cpu().numpy()
+    Gr = orc.ewald_gather(vol, ang, LAM, NM, o.dx)
+    assert _relmax(G, Gr) <= 1e-6
+    # adjointness on the device values: <S u, v> = <u, G v>
+    lhs = np.vdot(F.cpu().numpy().astype(np.complex128), vol.astype(np.complex128))
+    rhs = np.vdot(spec.astype(np.complex128), G.astype(np.complex128))
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
+@pytest.mark.parametrize("V", [64, 128, 192, 256])
+def test_fft3d_matches_rocfft_and_oracle(ctx, orc, V):
+    g = torch.Generator(device="cuda").manual_seed(V)
+    x = torch.randn((V, V, V), dtype=torch.complex64, device="cuda", generator=g)
+    ref = torch.fft.fftn(x)  # rocFFT; tensor [z][y][x] == our x + V*y + V*V*z layout
+    got = ctx.fft3d(x.clone(), -1)
+    err = (got - ref).abs().max().item() / ref.abs().max().item()
+    assert err <= 2e-6, err
+    inv = ctx.fft3d(ref.clone(), +1)
+    assert (inv - x).abs().max().item() <= 1e-5
+    if V <= 64:
+        o = orc.fft3d(x.cpu().numpy(), -1)
+        assert _relmax(got.cpu().numpy(), o) <= 2e-6
+
+
+def _positivity_case(V, seed):
+    rng = np.random.default_rng(seed)
+    z, y, x = np.mgrid[0:V, 0:V, 0:V]
+    c = V / 2
+    r2 = ((x - c) / (0.3 * V)) ** 2 + ((y - c) / (0.25 * V)) ** 2 + ((z - c) / (0.2 * V)) ** 2
+    n = NM + 0.023 * (r2 <= 1) + 1j * 2e-4 * (r2 <= 1)
+    k2 = (2 * np.pi * NM / LAM) ** 2
+    f = -k2 * ((n / NM) ** 2 - 1)
+    F = np.fft.fftn(f)
+    mask = rng.uniform(size=(V, V, V)) < 0.25
+    mask[0, 0, 0] = True
+    Fm = np.where(mask, F, 0)
+    f0 = np.fft.ifftn(Fm)
+    return f0.astype(np.complex64), Fm.astype(np.complex64), mask.astype(np.float32)
+
+
+@pytest.mark.parametrize("V", [64, 128])
+def test_positivity_fused_plain_oracle(ctx, orc, V):
+    f0, Fm, hits = _positivity_case(V, V)
+    o = ctdm_amd.optics_for(V)
+    ref = orc.positivity(f0, Fm, hits, 5, LAM, NM, flags=0)
+    outs = []
+    for flags in (0, ctdm_amd.EXEC_PLAIN):
+        n = ctx.positivity(_cuda(f0), _cuda(Fm), _cuda(hits), 5, o, flags).cpu().numpy()
+        err = np.abs(n - ref).max()
+        print("positivity flags", flags, "max |dn|", err)
+        assert err <= 1e-5
+        outs.append(n)
+    assert np.abs(outs[0] - outs[1]).max() <= 1e-5
+    # reference mask mode agrees with hit mask when measured values are generic
+    nref = ctx.positivity(_cuda(f0), _cuda(Fm), _cuda(hits), 5, o, ctdm_amd.DC_REFCOMPAT).cpu().numpy()
+    assert np.abs(nref - outs[0]).max() <= 1e-5
+    # iters = 0: est_n of the input (shifted)
+    n0 = ctx.positivity(_cuda(f0), _cuda(Fm), _cuda(hits), 0, o).cpu().numpy()
+    assert np.abs(n0 - orc.positivity(f0, Fm, hits, 0, LAM, NM)).max() <= 1e-5
+
+
+def _cell(seed, A, H, V):
+    p = ctdm_amd.synth_params(H=H)
+    holo, bg, ang = ctdm_amd.synth_cell(seed, A, p)
+    return holo, bg, ang
+
+
+def _run(ctx, holo, bg, ang, V, iters, mode, graph=False, flags=0, **kw):
+    H = holo.shape[-1]
+    ri = torch.empty((V, V, V), dtype=torch.complex64, device="cuda")
+    cell = dict(holo=_cuda(holo), bg=_cuda(bg), angles=ang, V=V, iters=iters,
+                optics=ctdm_amd.optics_for(V, H), ri=ri, unwrap_flags=flags, **kw)
+    ctx.reconstruct([cell], mode=mode, graph=graph)
+    torch.cuda.synchronize()
+    return ri, cell
+
+
+@pytest.mark.parametrize("A,H,V,iters", [(12, 256, 64, 3), (20, 512, 128, 4)])
+def test_cell_batched_plain_oracle(ctx, orc, A, H, V, iters):
+    holo, bg, ang = _cell(A + V, A, H, V)
+    rb, cb = _run(ctx, holo, bg, ang, V, iters, mode=1)
+    rp, _ = _run(ctx, holo, bg, ang, V, iters, mode=0)
+    cr = cb["crop_used"]
+    assert (cr.row, cr.col) == (3 * H // 4, 3 * H // 4)
+    o = ctdm_amd.optics_for(V, H)
+    ref = orc.reconstruct_cell(holo, bg, ang, V, cr.astuple(), LAM, NM, o.dx, iters)
+    eb = np.abs(rb.cpu().numpy() - ref).max()
+    ep = np.abs(rp.cpu().numpy() - ref).max()
+    print(f"cell A={A} V={V}: batched max|dn| {eb:.2e}, plain {ep:.2e}, RI range {ref.real.min():.4f}..{ref.real.max():.4f}")
+    assert eb <= 1e-5 and ep <= 1e-5
+    assert ref.real.max() > NM + 1e-3  # the phantom is visible
+
+
+def test_graph_replay_equals_eager(ctx):
+    A, H, V = 10, 256, 64
+    holo, bg, ang = _cell(3, A, H, V)
+    eager, _ = _run(ctx, holo, bg, ang, V, 3, mode=1)
+    ri = torch.empty((V, V, V), dtype=torch.complex64, device="cuda")
+    hd, bd = _cuda(holo), _cuda(bg)
+    cell = dict(holo=hd, bg=bd, angles=ang, V=V, iters=3, optics=ctdm_amd.optics_for(V, H), ri=ri)
+    for rep in range(4):  # first call: eager + capture, then replays
+        ri.zero_()
+        ctx.reconstruct([cell], mode=1, graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(ri, eager), rep
+        _ = (hd * 2).sum().item()  # unrelated eager work between replays
+    # new data in the same buffers: the replay must follow it
+    holo2, bg2, _ = _cell(4, A, H, V)
+    hd.copy_(_cuda(holo2))
+    bd.copy_(_cuda(bg2))
+    ctx.reconstruct([cell], mode=1, graph=True)
+    torch.cuda.synchronize()
+    fresh, _ = _run(ctx, holo2, bg2, ang, V, 3, mode=1)
+    assert torch.equal(ri, fresh)
+    assert not torch.equal(ri, eager)
+
+
+def test_config2_full_size(ctx, orc):
+    """60 angles of 1024^2 holograms -> 256^3 RI, 10 positivity iterations (the headline workload)."""
+    A, H, V, iters = 60, 1024, 256, 10
+    holo, bg, ang = _cell(2, A, H, V)
+    rb, cb = _run(ctx, holo, bg, ang, V, iters, mode=1, graph=True)
+    cr = cb["crop_used"]
+    assert cr.astuple() == (768, 768, 85, 256)
+    o = ctdm_amd.optics_for(V, H)
+    ref = orc.reconstruct_cell(holo, bg, ang, V, cr.astuple(), LAM, NM, o.dx, iters)
+    eb = np.abs(rb.cpu().numpy() - ref).max()
+    rp, _ = _run(ctx, holo, bg, ang, V, iters, mode=0)
+    ep = np.abs(rp.cpu().numpy() - ref).max()
+    print(f"config2: batched max|dn| {eb:.2e}, plain {ep:.2e}; RI {ref.real.min():.4f}..{ref.real.max():.4f}")
+    assert eb <= 1e-5 and ep <= 1e-5

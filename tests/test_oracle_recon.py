@@ -1,0 +1,89 @@
+"""Pins the oracle's reconstruction restatement (Rytov spectrum, Ewald mapping, normalisation,
+positivity loop) with properties that a wrong sign, index or dropped term would break:
+- Rytov spectrum = numpy fft2 of the fftshifted transposed field;
+- every mapped sample lies on the Ewald sphere of its angle and its voxel is the nearest along z;
+- scatter and gather are adjoint (complex inner product, FP64);
+- est_n o modify is the identity on feasible n; the projection is idempotent;
+- a fully measured spectrum of a feasible object is a fixed point of the positivity loop."""
+import numpy as np
+import pytest
+
+LAM, NM = 532e-9, 1.337
+
+
+def _angles(A, na=1.0):
+    s = na / NM
+    az = 2 * np.pi * np.arange(A) / A
+    return np.stack([np.arcsin(s * np.sin(az)), np.arcsin(s * np.cos(az))], axis=1)
+
+
+def test_rytov_spectrum_matches_numpy(orc):
+    rng = np.random.default_rng(0)
+    ph = rng.standard_normal((2, 32, 32))
+    la = 0.1 * rng.standard_normal((2, 32, 32))
+    got = orc.rytov_spectrum(ph, la)
+    for a in range(2):
+        ref = np.fft.fft2(np.fft.fftshift((la[a] + 1j * ph[a]).T))
+        np.testing.assert_allclose(got[a], ref, atol=1e-10)
+
+
+@pytest.mark.parametrize("V", [32, 64])
+def test_cap_geometry(orc, V):
+    dx = 0.1e-6 * 1024 / 256 * (256 / V)
+    ang = _angles(6)
+    idx = orc.ewald_index(ang, V, LAM, NM, dx)
+    fm0 = NM / LAM
+    df = 1.0 / (V * dx)
+    f = np.fft.fftfreq(V, 1.0 / V)  # signed frequency of array index (numpy order == mod layout)
+    for a, (ax, ay) in enumerate(ang):
+        fx0, fy0 = fm0 * np.sin(ay), fm0 * np.sin(ax)
+        fz0 = np.sqrt(fm0 ** 2 - fx0 ** 2 - fy0 ** 2)
+        FX = f[None, :] * df + fx0   # index [Ny][Nx]
+        FY = f[:, None] * df + fy0
+        s = fm0 ** 2 - FX ** 2 - FY ** 2
+        on = s >= 0
+        assert np.array_equal(idx[a] >= 0, on)
+        fz = np.sqrt(np.where(on, s, 0))
+        nz = idx[a].astype(np.float64)
+        kz = np.where(nz >= V // 2, nz - V, nz)  # signed voxel index
+        # the voxel centre kz*df is the nearest grid point to fz - fz0 (mod V)
+        d = (fz - fz0) / df - kz
+        d = d - V * np.round(d / V)
+        assert np.all(np.abs(d[on]) <= 0.5 + 1e-9)
+
+
+def test_scatter_gather_adjoint(orc):
+    V, A = 32, 5
+    dx = 0.4e-6 * 8
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal((A, V, V)) + 1j * rng.standard_normal((A, V, V))
+    Fr = rng.standard_normal((V, V, V)) + 1j * rng.standard_normal((V, V, V))
+    Su, _ = orc.ewald_scatter(u, _angles(A), V, LAM, NM, dx)
+    GF = orc.ewald_gather(Fr, _angles(A), LAM, NM, dx)
+    lhs = np.vdot(Su, Fr)
+    rhs = np.vdot(u, GF)
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+def test_est_n_modify_identity_and_idempotence(orc):
+    rng = np.random.default_rng(2)
+    n = (NM + rng.uniform(0, 0.1, 1000)) + 1j * rng.uniform(0, 0.01, 1000)
+    f = orc.modify_f(n, LAM, NM)
+    # the literal sqrt((|z| - c)/2) form cancels when Im n is small: ~1e-11 in FP64
+    np.testing.assert_allclose(orc.est_n(f, LAM, NM), n, rtol=0, atol=1e-10)
+    g = rng.standard_normal(1000) * 1e13 + 1j * rng.standard_normal(1000) * 1e12
+    p1 = orc.modify_f(orc.est_n(g, LAM, NM), LAM, NM)
+    p2 = orc.modify_f(orc.est_n(p1, LAM, NM), LAM, NM)
+    np.testing.assert_allclose(p2, p1, rtol=0, atol=1e-9 * np.abs(p1).max())
+
+
+def test_positivity_fixed_point(orc):
+    V = 16
+    rng = np.random.default_rng(3)
+    n = (NM + rng.uniform(0, 0.05, (V, V, V))) + 1j * rng.uniform(0, 0.005, (V, V, V))
+    f = orc.modify_f(n, LAM, NM)
+    F = np.fft.fftn(f)
+    out = orc.positivity(f, F, np.ones((V, V, V)), 3, LAM, NM, flags=2)
+    np.testing.assert_allclose(out, n, atol=1e-12)
+    shifted = orc.positivity(f, F, np.ones((V, V, V)), 1, LAM, NM, flags=0)
+    np.testing.assert_allclose(shifted, np.fft.fftshift(n), atol=1e-12)
