@@ -103,6 +103,7 @@ SIGNATURES = {
     "ctdm_positivity": ([_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(Optics), C.c_uint], C.c_int),
     "ctdm_reconstruct": ([_vp, C.POINTER(Cell), C.c_int, C.POINTER(BatchOpts)], C.c_int),
     "ctdm_autotune": ([_vp, C.POINTER(C.c_int), C.c_int, C.c_char_p, C.c_size_t], C.c_int),
+    "ctdm_time_pass": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "ctdm_fit_pgd": ([_vp, _vp, _vp, C.POINTER(Angle), C.c_int, C.c_int, C.POINTER(Optics), C.c_int, C.c_double, _vp, C.c_uint], C.c_int),
     "ctdm_synth_default_params": ([C.POINTER(SynthParams)], None),
     "ctdm_synth_cell": ([C.c_uint64, C.c_int, C.POINTER(SynthParams), _vp, _vp, C.POINTER(Angle)], C.c_int),
@@ -353,6 +354,12 @@ class Context:
         for i, d in enumerate(cells):  # report the crop the pipeline used
             d["crop_used"] = arr[i].crop
         return cells
+
+    def time_pass(self, pass_kind: int, V: int, pencils: int = 0, reps: int = 20):
+        """(average ms per launch, algorithmic bytes per launch) of a fused volume pass."""
+        ms, nb = C.c_double(), C.c_double()
+        self._call("ctdm_time_pass", pass_kind, V, pencils, reps, C.byref(ms), C.byref(nb))
+        return ms.value, nb.value
 
     def autotune(self, sizes) -> str:
         s = (C.c_int * len(sizes))(*sizes)

@@ -36,9 +36,13 @@ void* Ctx::buf(const std::string& name, size_t bytes)
     CTDM_REQUIRE(!capturing, "scratch buffer '" + name + "' grown during graph capture");
     if (b.ptr) {
         CTDM_HIP(hipStreamSynchronize(stream));
-        CTDM_HIP(hipFree(b.ptr));
+        if (b.pinned)
+            retired.push_back(b.ptr);
+        else
+            CTDM_HIP(hipFree(b.ptr));
         b.ptr = nullptr;
         b.bytes = 0;
+        b.pinned = false;
     }
     CTDM_HIP(hipMalloc(&b.ptr, bytes ? bytes : 1));
     b.bytes = bytes;
@@ -117,6 +121,7 @@ ctdm_status ctdm_destroy(ctdm_ctx* ctx)
     for (auto& kv : ctx->c.twiddles) hipFree(kv.second);
     for (auto& kv : ctx->c.scratch)
         if (kv.second.ptr) hipFree(kv.second.ptr);
+    for (void* p : ctx->c.retired) hipFree(p);
     delete ctx;
     return CTDM_OK;
 }
@@ -126,9 +131,6 @@ ctdm_status ctdm_set_stream(ctdm_ctx* ctx, ctdm_stream stream)
     return guarded(ctx, [&](ctdm::Ctx& c) {
         CTDM_HIP(hipStreamSynchronize(c.stream));
         c.stream = (hipStream_t)stream;
-        for (auto& kv : c.graphs) {  // graphs were captured on the previous stream's pointers only
-            (void)kv;
-        }
     });
 }
 
@@ -261,6 +263,16 @@ ctdm_status ctdm_autotune(ctdm_ctx* ctx, const int* sizes, int nsizes, char* rep
             std::strncpy(report, r.c_str(), len - 1);
             report[len - 1] = 0;
         }
+    });
+}
+
+ctdm_status ctdm_time_pass(ctdm_ctx* ctx, int pass, int V, int pencils, int reps, double* avg_ms,
+                           double* bytes)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        CTDM_REQUIRE(avg_ms && bytes, "time_pass: null output");
+        if (pencils <= 0) pencils = c.tuning(V).pencils[pass < 0 || pass > 3 ? 0 : pass];
+        *avg_ms = ctdm::time_pass(c, pass, V, pencils, reps, bytes);
     });
 }
 

@@ -49,15 +49,16 @@ struct Error {
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool pinned = false;  // referenced by a captured graph: retire instead of freeing on growth
 };
 
 struct Ctx;
 
 // Per-size kernel choices picked by the autotuner (defaults are the untuned picks).
 struct Tune {
-    int vol_pencils = 16;  // pencils per workgroup in the volume passes (8 or 16)
-    int grid_block = 256;  // threads per workgroup of the pencil gridding kernel
+    int pencils[4] = {16, 16, 16, 16};  // pencils per workgroup of the x, y, z+dc and gridding passes
     bool tuned = false;
+    double ms[4][2] = {};               // measured ms per launch for B = 8, 16 (after tuning)
 };
 
 struct GraphKey {
@@ -78,8 +79,11 @@ struct Ctx {
     std::map<std::string, DevBuf> scratch;
     std::map<int, Tune> tune;
     std::map<GraphKey, GraphEntry> graphs;
+    std::vector<void*> retired;  // pinned buffers replaced by larger ones (freed with the context)
     bool capturing = false;
     int max_lds = 65536;
+    int dc_V = 0;    // volume side the cached data-consistency list ("dc.*") was built for
+    int dc_cap = 0;  // its entries per pencil
 
     const float2* tw(int n);
     // Scratch buffer `name` of at least `bytes` bytes (contents undefined).
@@ -144,6 +148,9 @@ void volume_batched(Ctx& c, const float2* spec, const ctdm_angle* ang, int A, in
                     const ctdm_optics& o, int iters, unsigned flags, float2* n_out);
 void fit_pgd(Ctx& c, float2* f, const float2* spec, const ctdm_angle* ang, int A, int V,
              const ctdm_optics& o, int iters, double step, float2* n, unsigned flags);
+
+// Average ms of one launch of volume pass `kind` (0 x, 1 y, 2 z+dc) with B pencils per workgroup.
+double time_pass(Ctx& c, int kind, int V, int B, int reps, double* bytes);
 
 // Pipeline (pipeline.cpp)
 void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& opts);

@@ -536,7 +536,8 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
         float2* sp = lds + pb * LdsSize<V>::value;
         for (int k = k0; k < n; k += T) {
             const long long e = pencil * dc.cap + k;
-            sp[lds_pad(dc.idx[e])] = dc.val[e];
+            const int z = dc.idx[e];
+            if (z >= 0 && z < V) sp[lds_pad(z)] = dc.val[e];
         }
     }
     __syncthreads();
@@ -642,6 +643,8 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
 static DcList dc_alloc(Ctx& c, int V, int cap)
 {
     const size_t VV = (size_t)V * V;
+    c.dc_V = V;
+    c.dc_cap = cap;
     DcList d;
     d.cap = cap;
     d.cnt = (int*)c.buf("dc.cnt", sizeof(int) * VV);
@@ -650,9 +653,9 @@ static DcList dc_alloc(Ctx& c, int V, int cap)
     return d;
 }
 
-template <int V>
+// Launchers of the fused passes for one pencils-per-workgroup choice B.
+template <int V, int B>
 struct VolOps {
-    static constexpr int B = 16;
     static constexpr int NT = B * Plan<V>::T;
     static dim3 grid() { return dim3((unsigned)((long long)V * V / B)); }
 
@@ -687,6 +690,35 @@ struct VolOps {
                            (float)(1.0 / g.dx), (int)ref, c.tw(V), W, dc);
         CTDM_CHECK_LAUNCH("k_vol_grid");
     }
+};
+
+// Runtime choice of B per pass kind (autotuned; 16 by default).
+enum PassKind { PASS_X = 0, PASS_Y = 1, PASS_ZDC = 2, PASS_GRID = 3, NPASS = 4 };
+
+template <int V>
+struct Vol {
+    static int pick(Ctx& c, PassKind k) { return c.tuning(V).pencils[k] == 8 ? 8 : 16; }
+    static void x(Ctx& c, int mode, float2* W, float2* out, const Proj& p, float scale, bool shift)
+    {
+        if (pick(c, PASS_X) == 8) VolOps<V, 8>::x(c, mode, W, out, p, scale, shift);
+        else VolOps<V, 16>::x(c, mode, W, out, p, scale, shift);
+    }
+    static void y(Ctx& c, float2* W, int dir)
+    {
+        if (pick(c, PASS_Y) == 8) VolOps<V, 8>::y(c, W, dir);
+        else VolOps<V, 16>::y(c, W, dir);
+    }
+    static void zdc(Ctx& c, float2* W, const DcList& dc)
+    {
+        if (pick(c, PASS_ZDC) == 8) VolOps<V, 8>::zdc(c, W, dc);
+        else VolOps<V, 16>::zdc(c, W, dc);
+    }
+    static void grid_pass(Ctx& c, const float2* spec, const double* rows, int A, const EwaldGeom& g, bool ref,
+                          float2* W, const DcList& dc)
+    {
+        if (pick(c, PASS_GRID) == 8) VolOps<V, 8>::grid_pass(c, spec, rows, A, g, ref, W, dc);
+        else VolOps<V, 16>::grid_pass(c, spec, rows, A, g, ref, W, dc);
+    }
     // Loop after the initial inverse transform of the measured spectrum is in W up to the x pass:
     // `first_x_mode` 0 if W holds real-space f, 1 if W still needs its inverse x-FFT.
     static void loop(Ctx& c, float2* W, float2* n_out, const DcList& dc, const Proj& p, int iters, bool shift,
@@ -719,7 +751,7 @@ static void positivity_fused_v(Ctx& c, float2* f, const float2* Fm, const float*
     hipLaunchKernelGGL(k_dc_build, dim3(grid1d((long long)V * V, 256)), dim3(256), 0, c.stream, Fm, hits, V,
                        (int)ref, dc);
     CTDM_CHECK_LAUNCH("k_dc_build");
-    VolOps<V>::loop(c, f, n, dc, p, iters, !(flags & CTDM_POS_NO_SHIFT), 0);
+    Vol<V>::loop(c, f, n, dc, p, iters, !(flags & CTDM_POS_NO_SHIFT), 0);
 }
 
 void positivity_fused(Ctx& c, float2* f, const float2* Fm, const float* hits, float2* n, int V, int iters,
@@ -745,9 +777,9 @@ static void volume_batched_v(Ctx& c, const float2* spec, const double* rows, int
 {
     const DcList dc = dc_alloc(c, V, std::max(A, 1));
     float2* W = (float2*)c.buf("vol.W", sizeof(float2) * (size_t)V * V * V);
-    VolOps<V>::grid_pass(c, spec, rows, A, g, flags & CTDM_DC_REFCOMPAT, W, dc);
-    VolOps<V>::y(c, W, +1);
-    VolOps<V>::loop(c, W, n_out, dc, make_proj(g), iters, !(flags & CTDM_POS_NO_SHIFT), 1);
+    Vol<V>::grid_pass(c, spec, rows, A, g, flags & CTDM_DC_REFCOMPAT, W, dc);
+    Vol<V>::y(c, W, +1);
+    Vol<V>::loop(c, W, n_out, dc, make_proj(g), iters, !(flags & CTDM_POS_NO_SHIFT), 1);
 }
 
 void volume_batched(Ctx& c, const float2* spec, const ctdm_angle* ang, int A, int V, const ctdm_optics& o,
@@ -780,6 +812,75 @@ void zero_volume(Ctx& c, float2* F, float* hits, int V)
     const long long n = (long long)V * V * V;
     fill(c, F, 2 * n, 0.f);
     if (hits) fill(c, hits, n, 0.f);
+}
+
+// ---------------------------------------------------------------- pass timing (autotuner, bench)
+// Average duration of one launch of pass `kind` with B pencils per workgroup, timed with hipEvents on
+// the context stream over `reps` launches. Runs on the context's volume buffer and data-consistency
+// list (left by the last batched reconstruction of side V, or zero-initialised here). Returns the
+// algorithmic bytes of one launch (logical inputs read once + outputs written once).
+template <int V>
+static double time_pass_v(Ctx& c, int kind, int B, int reps, double* bytes)
+{
+    const size_t vol = (size_t)V * V * V, VV = (size_t)V * V;
+    const std::string wname = "vol.W";
+    const bool have = c.dc_V == V && c.scratch.count(wname) && c.scratch[wname].bytes >= sizeof(float2) * vol;
+    const int cap = have ? std::max(1, c.dc_cap) : 1;
+    float2* W = (float2*)c.buf(wname, sizeof(float2) * vol);
+    const DcList dc = dc_alloc(c, V, cap);
+    if (!have) {
+        fill(c, W, 2 * (long long)vol, 1.f);
+        fill(c, dc.cnt, (long long)VV, 0.f);  // all-zero bit pattern == integer 0
+    }
+    float2* nbuf = (float2*)c.buf("tp.n", sizeof(float2) * vol);
+    const Proj p = make_proj(ewald_geom(ctdm_optics{532e-9, 1.337, 0.4e-6}, V));
+    // bytes of one launch
+    const double vb = 2.0 * sizeof(float2) * vol;
+    double extra = 0;
+    if (kind == PASS_ZDC) {
+        std::vector<int> h(VV);
+        CTDM_HIP(hipMemcpy(h.data(), dc.cnt, sizeof(int) * VV, hipMemcpyDeviceToHost));
+        double n = 0;
+        for (int v : h) n += v;
+        extra = sizeof(int) * (double)VV + n * (sizeof(float2) + sizeof(short));
+    }
+    *bytes = vb + extra;
+    auto launch = [&](int r) {
+        switch (kind) {
+        case PASS_X: B == 8 ? VolOps<V, 8>::x(c, 1, W, nbuf, p, 1.f / (float)vol, true)
+                            : VolOps<V, 16>::x(c, 1, W, nbuf, p, 1.f / (float)vol, true); break;
+        case PASS_Y: B == 8 ? VolOps<V, 8>::y(c, W, (r & 1) ? 1 : -1) : VolOps<V, 16>::y(c, W, (r & 1) ? 1 : -1); break;
+        case PASS_ZDC: B == 8 ? VolOps<V, 8>::zdc(c, W, dc) : VolOps<V, 16>::zdc(c, W, dc); break;
+        default: fail(CTDM_EINVAL, "time_pass: unknown pass");
+        }
+    };
+    launch(0);  // warm-up
+    hipEvent_t e0, e1;
+    CTDM_HIP(hipEventCreate(&e0));
+    CTDM_HIP(hipEventCreate(&e1));
+    CTDM_HIP(hipEventRecord(e0, c.stream));
+    for (int r = 0; r < reps; ++r) launch(r + 1);
+    CTDM_HIP(hipEventRecord(e1, c.stream));
+    CTDM_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    CTDM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return ms / reps;
+}
+
+double time_pass(Ctx& c, int kind, int V, int B, int reps, double* bytes)
+{
+    CTDM_REQUIRE(!c.capturing, "time_pass during capture");
+    CTDM_REQUIRE(reps > 0 && (B == 8 || B == 16), "time_pass: bad arguments");
+    CTDM_REQUIRE(kind >= 0 && kind < PASS_GRID, "time_pass: pass must be 0 (x), 1 (y) or 2 (z+dc)");
+    switch (V) {
+    case 64: return time_pass_v<64>(c, kind, B, reps, bytes);
+    case 128: return time_pass_v<128>(c, kind, B, reps, bytes);
+    case 192: return time_pass_v<192>(c, kind, B, reps, bytes);
+    case 256: return time_pass_v<256>(c, kind, B, reps, bytes);
+    default: fail(CTDM_EUNSUPPORTED, "time_pass: volume side has no kernel");
+    }
 }
 
 }  // namespace ctdm

@@ -210,6 +210,7 @@ static void row_real(Ctx& c, const float* src0, int n0, const float* src1, int n
                      int k_lo, int nout, float2* out)
 {
     switch (W) {
+    case 256: launch_row_real<256>(c, src0, n0, src1, nimg, H, k_lo, nout, out); break;
     case 512: launch_row_real<512>(c, src0, n0, src1, nimg, H, k_lo, nout, out); break;
     case 1024: launch_row_real<1024>(c, src0, n0, src1, nimg, H, k_lo, nout, out); break;
     default: fail(CTDM_EUNSUPPORTED, "hologram width " + std::to_string(W) + " has no kernel");
@@ -275,6 +276,7 @@ void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int
     float2* G = (float2*)c.buf("ret.G", sizeof(float2) * (size_t)nimg * M * pb.n);
     row_real(c, holo, batch, bg, nimg, H, W, c0 + pb.lo, pb.n, R);
     switch (H) {
+    case 256: col_band_m<256>(c, M, R, nimg, pb.n, r0, qb, G); break;
     case 512: col_band_m<512>(c, M, R, nimg, pb.n, r0, qb, G); break;
     case 1024: col_band_m<1024>(c, M, R, nimg, pb.n, r0, qb, G); break;
     default: fail(CTDM_EUNSUPPORTED, "hologram height " + std::to_string(H) + " has no kernel");
@@ -291,7 +293,7 @@ void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int
 void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, ctdm_crop* out)
 {
     CTDM_REQUIRE(bg && out, "find_sideband: null pointer");
-    CTDM_REQUIRE(H == W && (H == 512 || H == 1024), "find_sideband: square 512 or 1024 holograms");
+    CTDM_REQUIRE(H == W && (H == 256 || H == 512 || H == 1024), "find_sideband: square 256, 512 or 1024 holograms");
     if (row_start <= 0) row_start = 600 * H / 1024;
     CTDM_REQUIRE(row_start < H, "find_sideband: row_start beyond the spectrum");
     float2* spec = (float2*)c.buf("sb.spec", sizeof(float2) * (size_t)H * W);

@@ -188,6 +188,7 @@ void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& op
         c.capturing = false;
         c.stream = saved;
         CTDM_HIP(hipStreamDestroy(cs));
+        for (auto& kv : c.scratch) kv.second.pinned = true;  // the graph holds these pointers
         c.graphs[key] = ge;
         return;
     }

@@ -208,6 +208,14 @@ ctdm_status ctdm_reconstruct(ctdm_ctx *ctx, ctdm_cell *cells, int ncells,
 ctdm_status ctdm_autotune(ctdm_ctx *ctx, const int *sizes, int nsizes, char *report,
                           size_t report_len);
 
+/* Average duration [ms] of one launch of a fused volume pass (pass 0: x with projection, 1: y,
+ * 2: z with data consistency) at volume side V with `pencils` pencils per workgroup (8 or 16;
+ * <= 0 = the tuned choice), timed with hipEvents on the context stream over `reps` launches on the
+ * context's volume buffer (no reference counterpart: the measuring primitive of ctdm_autotune).
+ * *bytes receives the algorithmic bytes of one launch. Synchronises. */
+ctdm_status ctdm_time_pass(ctdm_ctx *ctx, int pass, int V, int pencils, int reps, double *avg_ms,
+                           double *bytes);
+
 /* ---------------------------------------------------------------- differentiable fit */
 /* Projected-gradient fit of measured Rytov spectra (no reference counterpart; replaces the
  * positivity loop when chosen): minimise 0.5*sum_a |G_a(FFT3 f) - spec_a|^2 over the scattering

@@ -192,3 +192,16 @@ def test_config2_full_size(ctx, orc):
     ep = np.abs(rp.cpu().numpy() - ref).max()
     print(f"config2: batched max|dn| {eb:.2e}, plain {ep:.2e}; RI {ref.real.min():.4f}..{ref.real.max():.4f}")
     assert eb <= 1e-5 and ep <= 1e-5
+
+
+def test_autotune_keeps_results_and_times_passes(ctx):
+    A, H, V = 8, 256, 64
+    holo, bg, ang = _cell(9, A, H, V)
+    before, _ = _run(ctx, holo, bg, ang, V, 2, mode=1)
+    report = ctx.autotune([V])
+    assert "V=64" in report and report.count("->") == 3
+    after, _ = _run(ctx, holo, bg, ang, V, 2, mode=1)
+    assert torch.equal(before, after)  # the pencil grouping does not change any value
+    for k in (0, 1, 2):
+        ms, nbytes = ctx.time_pass(k, V, 0, 5)
+        assert ms > 0 and nbytes >= 2 * 8 * V ** 3
