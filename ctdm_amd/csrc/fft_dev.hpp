@@ -136,6 +136,14 @@ struct LdsSize {
     static constexpr int value = N + N / 16;
 };
 
+// Copies a per-N twiddle table into LDS at kernel entry (the copy overlaps the first data loads;
+// the first Stockham stage uses no twiddles and its barrier publishes the table).
+template <int N>
+__device__ __forceinline__ void stage_twiddles(float2* dst, const float2* __restrict__ src)
+{
+    for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[i];
+}
+
 // Twiddle from a per-N table tw[m] = exp(-2*pi*i*m/N) (float2, computed in double on the host).
 template <int DIR>
 __device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m)

@@ -21,6 +21,8 @@ __global__ __launch_bounds__(B* Plan<N>::T) void k_fft_pencils(
 {
     constexpr int T = Plan<N>::T;
     __shared__ float2 lds[B * LdsSize<N>::value];
+    __shared__ float2 twl[N];
+    stage_twiddles<N>(twl, tw);
     const int tid = threadIdx.x;
     const int b = BFAST ? (tid % B) : (tid / T);
     const int t = BFAST ? (tid / B) : (tid % T);
@@ -37,7 +39,7 @@ __global__ __launch_bounds__(B* Plan<N>::T) void k_fft_pencils(
         if (conj_out) v.y = -v.y;
         base[(long long)i * stride_n] = v;
     };
-    Plan<N>::template run<DIR>(t, s, tw, gl, gs);
+    Plan<N>::template run<DIR>(t, s, twl, gl, gs);
 }
 
 template <int N, int B, bool BFAST>

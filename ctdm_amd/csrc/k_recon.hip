@@ -485,6 +485,8 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_x(float2* __restrict__ W,
                                                          int shift)
 {
     __shared__ float2 lds[B * LdsSize<V>::value];
+    __shared__ float2 twl[V];
+    stage_twiddles<V>(twl, tw);
     Tile<V, 0, B> tl;
     float2* s = lds + tl.b * LdsSize<V>::value;
     float2* row = W + tl.base;
@@ -495,47 +497,68 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_x(float2* __restrict__ W,
     if constexpr (MODE == 0) {
         auto gl = [&](int x) -> float2 { return modify_f(est_n(row[x], p), p); };
         auto gs = [&](int k, float2 v) { row[k] = v; };
-        Plan<V>::template run<-1>(tl.t, s, tw, gl, gs);
+        Plan<V>::template run<-1>(tl.t, s, twl, gl, gs);
     } else if constexpr (MODE == 3) {
         for (int x = tl.t; x < V; x += Plan<V>::T) put_n(x, est_n(row[x], p));
     } else {
         auto gl = [&](int k) -> float2 { return row[k]; };
         if constexpr (MODE == 1) {
             auto gs = [&](int x, float2 v) { s[lds_pad(x)] = modify_f(est_n(cscale(v, scale), p), p); };
-            Plan<V>::template run<+1>(tl.t, s, tw, gl, gs);
+            Plan<V>::template run<+1>(tl.t, s, twl, gl, gs);
             __syncthreads();
             auto gl2 = [&](int x) -> float2 { return s[lds_pad(x)]; };
             auto gs2 = [&](int k, float2 v) { row[k] = v; };
-            Plan<V>::template run<-1>(tl.t, s, tw, gl2, gs2);
+            Plan<V>::template run<-1>(tl.t, s, twl, gl2, gs2);
         } else {
             auto gs = [&](int x, float2 v) { put_n(x, est_n(cscale(v, scale), p)); };
-            Plan<V>::template run<+1>(tl.t, s, tw, gl, gs);
+            Plan<V>::template run<+1>(tl.t, s, twl, gl, gs);
         }
     }
 }
 
 // Z pass with data consistency: forward z-FFT, replace measured voxels, inverse z-FFT.
+// The pencil's list entries are fetched at kernel entry (DC_PRE per thread, independent of the
+// count) so their latency overlaps the data loads; longer lists finish in a loop.
 template <int V, int B>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ W, const float2* __restrict__ tw,
                                                            DcList dc)
 {
     constexpr int T = Plan<V>::T;
+    constexpr int DC_PRE = 4;
     __shared__ float2 lds[B * LdsSize<V>::value];
+    __shared__ float2 twl[V];
+    stage_twiddles<V>(twl, tw);
+    // list prefetch: lanes walk the entries of one pencil (contiguous in the list)
+    const int pb = threadIdx.x / T, k0 = threadIdx.x % T;
+    const long long dpencil = (long long)(blockIdx.x % (V / B)) * B + pb + (long long)V * (blockIdx.x / (V / B));
+    const int n = dc.cnt[dpencil];
+    short pz[DC_PRE];
+    float2 pv[DC_PRE];
+#pragma unroll
+    for (int r = 0; r < DC_PRE; ++r) {
+        const int k = k0 + r * T;
+        const bool in = k < dc.cap;
+        const long long e = dpencil * dc.cap + (in ? k : 0);
+        pz[r] = dc.idx[e];
+        pv[r] = dc.val[e];
+    }
     Tile<V, 2, B> tl;
     float2* s = lds + tl.b * LdsSize<V>::value;
     float2* pen = W + tl.base;
     auto gl = [&](int z) -> float2 { return pen[(long long)z * tl.stride]; };
     auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
-    Plan<V>::template run<-1>(tl.t, s, tw, gl, gs);
+    Plan<V>::template run<-1>(tl.t, s, twl, gl, gs);
     __syncthreads();
-    // data consistency, lanes walk the entries of one pencil (contiguous in the list)
     {
-        const int pb = threadIdx.x / T, k0 = threadIdx.x % T;
-        const long long pencil = (long long)(blockIdx.x % (V / B)) * B + pb + (long long)V * (blockIdx.x / (V / B));
-        const int n = dc.cnt[pencil];
         float2* sp = lds + pb * LdsSize<V>::value;
-        for (int k = k0; k < n; k += T) {
-            const long long e = pencil * dc.cap + k;
+#pragma unroll
+        for (int r = 0; r < DC_PRE; ++r) {
+            const int k = k0 + r * T;
+            const int z = pz[r];
+            if (k < n && z >= 0 && z < V) sp[lds_pad(z)] = pv[r];
+        }
+        for (int k = k0 + DC_PRE * T; k < n; k += T) {
+            const long long e = dpencil * dc.cap + k;
             const int z = dc.idx[e];
             if (z >= 0 && z < V) sp[lds_pad(z)] = dc.val[e];
         }
@@ -543,7 +566,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
     __syncthreads();
     auto gl2 = [&](int k) -> float2 { return s[lds_pad(k)]; };
     auto gs2 = [&](int z, float2 v) { pen[(long long)z * tl.stride] = v; };
-    Plan<V>::template run<+1>(tl.t, s, tw, gl2, gs2);
+    Plan<V>::template run<+1>(tl.t, s, twl, gl2, gs2);
 }
 
 // Y pass (plain FFT along y).
@@ -551,12 +574,14 @@ template <int V, int B, int DIR>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_y(float2* __restrict__ W, const float2* __restrict__ tw)
 {
     __shared__ float2 lds[B * LdsSize<V>::value];
+    __shared__ float2 twl[V];
+    stage_twiddles<V>(twl, tw);
     Tile<V, 1, B> tl;
     float2* s = lds + tl.b * LdsSize<V>::value;
     float2* pen = W + tl.base;
     auto gl = [&](int y) -> float2 { return pen[(long long)y * tl.stride]; };
     auto gs = [&](int k, float2 v) { pen[(long long)k * tl.stride] = v; };
-    Plan<V>::template run<DIR>(tl.t, s, tw, gl, gs);
+    Plan<V>::template run<DIR>(tl.t, s, twl, gl, gs);
 }
 
 // Compact list from a dense measured spectrum (positivity API): one thread per pencil.
@@ -582,6 +607,11 @@ __global__ void k_dc_build(const float2* __restrict__ Fm, const float* __restric
 
 // Gridding pass: per k-space pencil (x, y) accumulate every angle's sample in angle order, average
 // by hit count, scale by 1/dx (Initial_F_F2), emit the compact list, then inverse z-FFT.
+// The pencil's T threads evaluate the caps of interleaved angles into an LDS list (chunks of
+// GRID_CHUNK angles); thread t then owns the z range [t V/T, (t+1) V/T) and adds the list entries
+// that land there in angle order, so the sums are deterministic.
+constexpr int GRID_CHUNK = 64;
+
 template <int V, int B>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __restrict__ spec,
                                                             const double* __restrict__ rows, int A,
@@ -590,54 +620,81 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
                                                             float2* __restrict__ W, DcList dc)
 {
     constexpr int T = Plan<V>::T;
+    constexpr int ZR = V / T;  // z values owned by one thread
     __shared__ float2 lds[B * LdsSize<V>::value];
     __shared__ unsigned short hit[B][V];
+    __shared__ short lnz[B][GRID_CHUNK];
+    __shared__ float2 lval[B][GRID_CHUNK];
+    __shared__ int lcnt[B][T];
+    __shared__ float2 twl[V];
+    stage_twiddles<V>(twl, tw);
     Tile<V, 2, B> tl;
     float2* s = lds + tl.b * LdsSize<V>::value;
     for (int z = tl.t; z < V; z += T) {
         s[lds_pad(z)] = make_float2(0.f, 0.f);
         hit[tl.b][z] = 0;
     }
-    __syncthreads();
+    __syncthreads();  // zeroing (interleaved z) precedes the owner-range accumulation
     const long long VV = (long long)V * V;
     const long long pencil = tl.base;  // x + V*y
-    if (tl.t == 0) {
-        const int ii = signed_freq(tl.px, V), jj = signed_freq(tl.py, V);
-        for (int a = 0; a < A; ++a) {
+    const int ii = signed_freq(tl.px, V), jj = signed_freq(tl.py, V);
+    const int z0 = tl.t * ZR;
+    for (int a0 = 0; a0 < A; a0 += GRID_CHUNK) {
+        const int na = min(GRID_CHUNK, A - a0);
+        for (int k = tl.t; k < na; k += T) {
+            const int a = a0 + k;
             const Cap cp = ewald_cap(ii, jj, rows[3 * a], rows[3 * a + 1], rows[3 * a + 2], fm02, df, V);
-            if (!cp.ok) continue;
-            const float2 u = spec[(long long)a * VV + pencil];
-            float2& acc = s[lds_pad(cp.nz)];
-            acc.x += -cp.w * u.y;
-            acc.y += cp.w * u.x;
-            hit[tl.b][cp.nz] += 1;
+            short nz = -1;
+            float2 val = make_float2(0.f, 0.f);
+            if (cp.ok) {
+                const float2 u = spec[(long long)a * VV + pencil];
+                nz = (short)cp.nz;
+                val = make_float2(-cp.w * u.y, cp.w * u.x);
+            }
+            lnz[tl.b][k] = nz;
+            lval[tl.b][k] = val;
         }
+        __syncthreads();
+        for (int k = 0; k < na; ++k) {
+            const int nz = lnz[tl.b][k];
+            if (nz >= z0 && nz < z0 + ZR) {
+                float2& acc = s[lds_pad(nz)];
+                const float2 v = lval[tl.b][k];
+                acc.x += v.x;
+                acc.y += v.y;
+                hit[tl.b][nz] += 1;
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    for (int z = tl.t; z < V; z += T) {
+    // normalise the owned range and count list entries
+    int cnt = 0;
+    for (int z = z0; z < z0 + ZR; ++z) {
         const int h = hit[tl.b][z];
         float2 f = s[lds_pad(z)];
         if (h) f = make_float2(f.x / (float)h, f.y / (float)h);
         f = make_float2(f.x * inv_dx, f.y * inv_dx);
         s[lds_pad(z)] = f;
+        if (h && !(refcompat && !(f.x != 0.f && f.y != 0.f))) ++cnt;
     }
+    lcnt[tl.b][tl.t] = cnt;
     __syncthreads();
-    if (tl.t == 0) {  // compact list in z order
-        int n = 0;
-        for (int z = 0; z < V; ++z) {
-            if (!hit[tl.b][z]) continue;
-            const float2 f = s[lds_pad(z)];
-            if (refcompat && !(f.x != 0.f && f.y != 0.f)) continue;
-            dc.idx[pencil * dc.cap + n] = (short)z;
-            dc.val[pencil * dc.cap + n] = f;
-            ++n;
+    int off = 0;
+    for (int k = 0; k < tl.t; ++k) off += lcnt[tl.b][k];
+    for (int z = z0; z < z0 + ZR; ++z) {
+        const int h = hit[tl.b][z];
+        const float2 f = s[lds_pad(z)];
+        if (h && !(refcompat && !(f.x != 0.f && f.y != 0.f))) {
+            dc.idx[pencil * dc.cap + off] = (short)z;
+            dc.val[pencil * dc.cap + off] = f;
+            ++off;
         }
-        dc.cnt[pencil] = n;
     }
+    if (tl.t == T - 1) dc.cnt[pencil] = off;
     float2* pen = W + tl.base;
     auto gl = [&](int k) -> float2 { return s[lds_pad(k)]; };
     auto gs = [&](int z, float2 v) { pen[VV * z] = v; };
-    Plan<V>::template run<+1>(tl.t, s, tw, gl, gs);
+    Plan<V>::template run<+1>(tl.t, s, twl, gl, gs);
 }
 
 static DcList dc_alloc(Ctx& c, int V, int cap)
