@@ -59,22 +59,25 @@ struct Cap {
     bool ok;
 };
 
+// Plain operators under `contract(off)`: the FP64 expressions are rounded exactly as written (the
+// host oracle is compiled with -ffp-contract=off), so both pick the same voxel on rounding ties.
 __device__ __forceinline__ Cap ewald_cap(int ii, int jj, double fx0, double fy0, double fz0, double fm02,
                                          double df, int V)
 {
-    const double fx = __dadd_rn(__dmul_rn((double)ii, df), fx0);
-    const double fy = __dadd_rn(__dmul_rn((double)jj, df), fy0);
-    const double s = __dsub_rn(fm02, __dadd_rn(__dmul_rn(fx, fx), __dmul_rn(fy, fy)));
-    const double s0 = __dsub_rn(__dsub_rn(fm02, __dmul_rn(fx0, fx0)), __dmul_rn(fy0, fy0));
+#pragma clang fp contract(off)
+    const double fx = (double)ii * df + fx0;
+    const double fy = (double)jj * df + fy0;
+    const double s = fm02 - (fx * fx + fy * fy);
+    const double s0 = fm02 - fx0 * fx0 - fy0 * fy0;
     Cap c;
     c.ok = !(s0 < 0.0 || s < 0.0);
-    const double fz = __dsqrt_rn(s > 0.0 ? s : 0.0);
-    const double q = __ddiv_rn(__dsub_rn(fz, fz0), df);
-    const double nz = q > 0.0 ? floor(__dadd_rn(q, 0.5)) : ceil(__dsub_rn(q, 0.5));
+    const double fz = __builtin_sqrt(s > 0.0 ? s : 0.0);
+    const double q = (fz - fz0) / df;
+    const double nz = q > 0.0 ? floor(q + 0.5) : ceil(q - 0.5);
     int m = (int)nz % V;
     if (m < 0) m += V;
     c.nz = m;
-    c.w = (float)__dmul_rn(4.0 * M_PI, fz);
+    c.w = (float)(4.0 * M_PI * fz);
     return c;
 }
 

@@ -2,6 +2,5 @@
 #include "internal.hpp"
 namespace ctdm {
 #define NOTYET(name) fail(CTDM_EUNSUPPORTED, name " not implemented yet")
-void goldstein(Ctx&, const float*, int, int, int, int8_t*, uint8_t*, int32_t*, float*) { NOTYET("goldstein"); }
 void fit_pgd(Ctx&, float2*, const float2*, const ctdm_angle*, int, int, const ctdm_optics&, int, double, float2*, unsigned) { NOTYET("fit_pgd"); }
 }

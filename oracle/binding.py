@@ -61,6 +61,10 @@ _EXTRA: dict = {
     "orc_est_n": [_dp, _dp, C.c_long, C.c_double, C.c_double],
     "orc_modify_f": [_dp, _dp, C.c_long, C.c_double, C.c_double],
     "orc_positivity": [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint],
+    "orc_residues": [_fp, C.c_int, C.c_int, C.c_void_p],
+    "orc_branch_cuts": [C.c_void_p, C.c_int, C.c_int, C.c_void_p],
+    "orc_flood_fill": [_fp, C.c_void_p, C.c_int, C.c_int, _ip, _fp],
+    "orc_goldstein": [_fp, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, _ip, _fp],
     "orc_reconstruct_cell": [_fp, _fp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                              C.c_double, C.c_double, C.c_double, C.c_int, C.c_uint, C.c_uint, _dp,
                              _dp, _dp, _dp],
@@ -225,3 +229,43 @@ def reconstruct_cell(holo, bg, angles, V, crop, wavelength, n_med, dx, iters, un
     if outputs:
         return ri, dict(zip(("wrapped", "unwrapped", "logamp"), extra))
     return ri
+
+
+# ---------------------------------------------------------------- Goldstein
+def residues(phi):
+    p = np.ascontiguousarray(phi, np.float32)
+    H, W = p.shape
+    out = np.zeros((H, W), np.int8)
+    lib().orc_residues(_p(p, _fp), H, W, out.ctypes.data)
+    return out
+
+
+def branch_cuts(res):
+    r = np.ascontiguousarray(res, np.int8)
+    H, W = r.shape
+    out = np.zeros((H, W), np.uint8)
+    lib().orc_branch_cuts(r.ctypes.data, H, W, out.ctypes.data)
+    return out
+
+
+def flood_fill(phi, cuts):
+    p = np.ascontiguousarray(phi, np.float32)
+    c = np.ascontiguousarray(cuts, np.uint8)
+    H, W = p.shape
+    k = np.zeros((H, W), np.int32)
+    un = np.zeros((H, W), np.float32)
+    lib().orc_flood_fill(_p(p, _fp), c.ctypes.data, H, W, _p(k, _ip), _p(un, _fp))
+    return k, un
+
+
+def goldstein(phi):
+    p = np.ascontiguousarray(phi, np.float32)
+    if p.ndim == 2:
+        p = p[None]
+    B, H, W = p.shape
+    res = np.zeros((B, H, W), np.int8)
+    cut = np.zeros((B, H, W), np.uint8)
+    k = np.zeros((B, H, W), np.int32)
+    un = np.zeros((B, H, W), np.float32)
+    lib().orc_goldstein(_p(p, _fp), B, H, W, res.ctypes.data, cut.ctypes.data, _p(k, _ip), _p(un, _fp))
+    return res, cut, k, un

@@ -750,3 +750,205 @@ void orc_reconstruct_cell(const float* holo, const float* bg, const double* angl
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- Goldstein branch cuts
+// Deterministic specification of cudaGoldsteinUnwrap2D (Goldstein.cu:5-50). The reference's GPU
+// branch-cut step races (every residue searches in parallel while others clear charges), so the
+// specification serialises the same search in raster order; the flood fill is level-synchronous
+// with the reference's neighbour priority (right, left, below, above; cuFF_Internal_1,
+// Goldstein.cu:415-525), which makes every integer 2*pi count independent of thread order.
+namespace {
+
+// cudaMod (Goldstein.cu:1193-1201)
+float cuda_mod(float a, float b)
+{
+    float r = std::fmod(a, b);
+    if (r < 0) r = r + b;
+    return r;
+}
+
+// One wrapped difference of cudaPhaseResidues: cudaMod(a - b + M_PI, M_PI*2) - M_PI with the
+// reference's float / double promotions (float difference, double shift, float modulo).
+float wrap_step(float a, float b)
+{
+    const float d = a - b;
+    const float x = (float)((double)d + PI);
+    const float m = cuda_mod(x, (float)(PI * 2.0));
+    return (float)((double)m - PI);
+}
+
+// cuPlaceBranchCutInternal (Goldstein.cu:226-248) with signed coordinates.
+void place_cut(uint8_t* cut, int W, int r1, int c1, int r2, int c2)
+{
+    cut[r1 + (long)W * c1] = 1;
+    cut[r2 + (long)W * c2] = 1;
+    const float dR = (float)(r2 - r1), dC = (float)(c2 - c1);
+    const float rad = std::sqrt(dR * dR + dC * dC);
+    const float ct = dR / rad, st = dC / rad;
+    const int n = (int)std::nearbyint(rad);
+    const int rlo = std::min(r1, r2), rhi = std::max(r1, r2), clo = std::min(c1, c2), chi = std::max(c1, c2);
+    for (int ii = 1; ii <= n; ++ii) {
+        const int rf = r1 + (int)std::nearbyint((float)ii * ct);
+        const int cf = c1 + (int)std::nearbyint((float)ii * st);
+        if (rf >= rlo && rf <= rhi && cf >= clo && cf <= chi) cut[rf + (long)W * cf] = 1;
+    }
+}
+
+const float INV_2PI_F = (float)(1.0 / (2.0 * PI));
+const float TWO_PI_F = (float)(2.0 * PI);
+
+int step2pi(float d) { return (int)std::nearbyint(d * INV_2PI_F); }
+
+}  // namespace
+
+extern "C" {
+
+// Residue charges (cudaPhaseResidues, Goldstein.cu:52-103): q = sum of the four wrapped differences
+// around the loop (i,j) -> (i+1,j) -> (i+1,j+1) -> (i,j+1); +1 if q >= 6, -1 if q <= -6; the last
+// row and column are 0.
+void orc_residues(const float* phi, int H, int W, int8_t* res)
+{
+    for (int j = 0; j < H; ++j)
+        for (int i = 0; i < W; ++i) {
+            int8_t q = 0;
+            if (i < W - 1 && j < H - 1) {
+                const float a = phi[i + (long)W * j], b = phi[(i + 1) + (long)W * j];
+                const float c = phi[(i + 1) + (long)W * (j + 1)], d = phi[i + (long)W * (j + 1)];
+                const float s = ((wrap_step(a, b) + wrap_step(b, c)) + wrap_step(c, d)) + wrap_step(d, a);
+                q = s >= 6.0f ? 1 : (s <= -6.0f ? -1 : 0);
+            }
+            res[i + (long)W * j] = q;
+        }
+}
+
+// Branch cuts (cuBranchCuts / cudaBC, Goldstein.cu:105-224) serialised in raster order: each
+// residue still unbalanced grows a box r = 1 .. min(901, W/2) - 1, scans its perimeter column by
+// column (first and last column top to bottom, inner columns top and bottom only) and stops at the
+// first point that is on the image border (cut straight to the border along its row or column) or
+// an unbalanced residue of opposite charge (cut to it, both balanced). Residues on the border are
+// balanced by a one-pixel cut.
+void orc_branch_cuts(const int8_t* res, int H, int W, uint8_t* cut)
+{
+    std::vector<int8_t> m(res, res + (size_t)H * W);
+    std::memset(cut, 0, (size_t)H * W);
+    const int R = std::min(901, W / 2);
+    for (int j = 0; j < H; ++j)
+        for (int i = 0; i < W; ++i) {
+            const long idx = i + (long)W * j;
+            if (m[idx] == 0) continue;
+            if (i <= 0 || i >= W - 1 || j <= 0 || j >= H - 1) {
+                cut[idx] = 1;
+                m[idx] = 0;
+                continue;
+            }
+            bool done = false;
+            for (int r = 1; r < R && !done; ++r) {
+                const int m1 = std::max(i - r, 0), m2 = std::min(i + r, W - 1);
+                const int n1 = std::max(j - r, 0), n2 = std::min(j + r, H - 1);
+                for (int mm = m1; mm <= m2 && !done; ++mm) {
+                    const int ndel = (mm == m1 || mm == m2) ? 1 : (n2 - n1);
+                    for (int nn = n1; nn <= n2 && !done; nn += ndel) {
+                        if (mm <= 0 || mm >= W - 1) {
+                            place_cut(cut, W, i, j, mm, j);
+                            m[idx] = 0;
+                            done = true;
+                        } else if (nn <= 0 || nn >= H - 1) {
+                            place_cut(cut, W, i, j, i, nn);
+                            m[idx] = 0;
+                            done = true;
+                        } else if (m[idx] * m[mm + (long)W * nn] == -1) {
+                            place_cut(cut, W, i, j, mm, nn);
+                            m[idx] = 0;
+                            m[mm + (long)W * nn] = 0;
+                            done = true;
+                        }
+                    }
+                }
+            }
+        }
+}
+
+// Flood fill (cudaFloodFill, Goldstein.cu:263-353) as integer 2*pi counts:
+//   seed (W/2 - t, min(20, H-1) + t) for the first t whose pixel is not a cut (cuFF_SeedPoints
+//   walks diagonally, :372-413); breadth-first levels over non-cut 4-neighbours; a pixel found at
+//   level d+1 takes as parent its first neighbour at level d in the order right, left, below,
+//   above, and k = k_parent - rint((phi - phi_parent) / 2 pi);
+//   then every interior cut pixel with a non-cut neighbour (cuFFcheckArray, :625-641) takes k from
+//   its first reached non-cut neighbour in the same order (cuFF_Internal_2, :527-623).
+// k = INT32_MIN where nothing reached the pixel; unwrapped = phi + 2 pi k there, 0 elsewhere.
+void orc_flood_fill(const float* phi, const uint8_t* cut, int H, int W, int32_t* k, float* unwrapped)
+{
+    const long n = (long)H * W;
+    std::vector<int> level(n, -1);
+    for (long p = 0; p < n; ++p) k[p] = INT32_MIN;
+    int sx = W / 2, sy = std::min(20, H - 1);
+    while (sx >= 0 && sy < H && cut[sx + (long)W * sy]) {
+        --sx;
+        ++sy;
+    }
+    const int dx4[4] = {1, -1, 0, 0}, dy4[4] = {0, 0, 1, -1};
+    if (sx >= 0 && sy < H) {
+        std::vector<long> cur{sx + (long)W * sy}, next;
+        level[cur[0]] = 0;
+        k[cur[0]] = 0;
+        for (int d = 0; !cur.empty(); ++d) {
+            next.clear();
+            for (long p : cur) {
+                const int x = (int)(p % W), y = (int)(p / W);
+                for (int e = 0; e < 4; ++e) {
+                    const int nx = x + dx4[e], ny = y + dy4[e];
+                    if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
+                    const long q = nx + (long)W * ny;
+                    if (cut[q] || level[q] != -1) continue;
+                    level[q] = d + 1;
+                    next.push_back(q);
+                }
+            }
+            for (long q : next) {
+                const int x = (int)(q % W), y = (int)(q / W);
+                for (int e = 0; e < 4; ++e) {
+                    const int nx = x + dx4[e], ny = y + dy4[e];
+                    if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
+                    const long pp = nx + (long)W * ny;
+                    if (level[pp] == d) {
+                        k[q] = k[pp] - step2pi(phi[q] - phi[pp]);
+                        break;
+                    }
+                }
+            }
+            cur.swap(next);
+        }
+    }
+    std::vector<int32_t> kf(k, k + n);
+    for (int y = 1; y < H - 1; ++y)
+        for (int x = 1; x < W - 1; ++x) {
+            const long q = x + (long)W * y;
+            if (!cut[q]) continue;
+            if (cut[q + 1] && cut[q - 1] && cut[q + W] && cut[q - W]) continue;
+            for (int e = 0; e < 4; ++e) {
+                const long pp = (x + dx4[e]) + (long)W * (y + dy4[e]);
+                if (!cut[pp] && level[pp] >= 0) {
+                    kf[q] = k[pp] - step2pi(phi[q] - phi[pp]);
+                    break;
+                }
+            }
+        }
+    for (long p = 0; p < n; ++p) {
+        k[p] = kf[p];
+        unwrapped[p] = kf[p] == INT32_MIN ? 0.0f : phi[p] + (float)kf[p] * TWO_PI_F;
+    }
+}
+
+void orc_goldstein(const float* phi, int batch, int H, int W, int8_t* res, uint8_t* cut, int32_t* k,
+                   float* unwrapped)
+{
+#pragma omp parallel for schedule(dynamic)
+    for (int b = 0; b < batch; ++b) {
+        const size_t o = (size_t)b * H * W;
+        orc_residues(phi + o, H, W, res + o);
+        orc_branch_cuts(res + o, H, W, cut + o);
+        orc_flood_fill(phi + o, cut + o, H, W, k + o, unwrapped + o);
+    }
+}
+
+}  // extern "C"
