@@ -78,7 +78,7 @@ class SynthParams(C.Structure):
                 ("n_med", C.c_double), ("na_ill", C.c_double), ("n_cell", C.c_double),
                 ("n_nucleoli", C.c_double), ("kappa_cell", C.c_double),
                 ("semi_axes", C.c_double * 3), ("n_nuc", C.c_int), ("r_nuc", C.c_double),
-                ("noise", C.c_double), ("carrier", C.c_double * 2)]
+                ("noise", C.c_double), ("carrier", C.c_double * 2), ("field_noise", C.c_double)]
 
 
 _vp = C.c_void_p

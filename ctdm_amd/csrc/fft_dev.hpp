@@ -215,6 +215,15 @@ struct Stockham {
 template <int N>
 struct Plan;
 template <>
+struct Plan<32> {
+    static constexpr int T = 4;
+    template <int DIR, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    {
+        Stockham<32, T, DIR, 1, 8, 4>::run(t, s, tw, gl, gs);
+    }
+};
+template <>
 struct Plan<64> {
     static constexpr int T = 8;
     template <int DIR, class GL, class GS>

@@ -73,6 +73,7 @@ static void dispatch(Ctx& c, int n, float2* data, long long stride_n, long long 
                      float scale, bool conj_out)
 {
     switch (n) {
+    case 32: launch_pencils<32, RowB<32>::value, BFAST>(c, data, stride_n, stride_b, npencils, nplanes, stride_plane, dir, scale, conj_out); break;
     case 64: launch_pencils<64, RowB<64>::value, BFAST>(c, data, stride_n, stride_b, npencils, nplanes, stride_plane, dir, scale, conj_out); break;
     case 128: launch_pencils<128, RowB<128>::value, BFAST>(c, data, stride_n, stride_b, npencils, nplanes, stride_plane, dir, scale, conj_out); break;
     case 192: launch_pencils<192, RowB<192>::value, BFAST>(c, data, stride_n, stride_b, npencils, nplanes, stride_plane, dir, scale, conj_out); break;

@@ -83,6 +83,9 @@ __device__ __forceinline__ Cap ewald_cap(int ii, int jj, double fx0, double fy0,
 
 __device__ __forceinline__ int signed_freq(int n, int V) { return n < V / 2 ? n : n - V; }
 
+// LDS chunk of per-pencil samples staged at a time by the gridding and fit kernels.
+constexpr int GRID_CHUNK = 64;
+
 // Device copy of the per-angle rows (fx0, fy0, fz0).
 static const double* upload_rows(Ctx& c, const std::string& name, const std::vector<AngleRow>& rows)
 {
@@ -148,7 +151,8 @@ __global__ __launch_bounds__(B* Plan<M>::T) void k_rytov_cols(const float2* __re
 template <int M>
 static void rytov_launch(Ctx& c, const float* phi, const float* la, int A, float2* spec)
 {
-    constexpr int G = 256 / Plan<M>::T, B = 256 / Plan<M>::T;
+    constexpr int G = 256 / Plan<M>::T;
+    constexpr int B = (256 / Plan<M>::T) < M ? (256 / Plan<M>::T) : M;  // columns per workgroup, <= M
     float2* P = (float2*)c.buf("ry.P", sizeof(float2) * (size_t)A * M * M);
     hipLaunchKernelGGL((k_rytov_rows<M, G>), dim3(grid1d((long long)A * M, G)), dim3(G * Plan<M>::T), 0, c.stream,
                        phi, la, A, c.tw(M), P);
@@ -162,6 +166,7 @@ void rytov_spectrum(Ctx& c, const float* unwrapped, const float* logamp, int A, 
     CTDM_REQUIRE(unwrapped && logamp && spec, "rytov_spectrum: null pointer");
     if (A == 0) return;
     switch (M) {
+    case 32: rytov_launch<32>(c, unwrapped, logamp, A, spec); break;
     case 64: rytov_launch<64>(c, unwrapped, logamp, A, spec); break;
     case 128: rytov_launch<128>(c, unwrapped, logamp, A, spec); break;
     case 192: rytov_launch<192>(c, unwrapped, logamp, A, spec); break;
@@ -304,7 +309,7 @@ void normalize_measured(Ctx& c, float2* F, const float* hits, float2* Fm, int V,
 // ---------------------------------------------------------------- 3-D FFT
 static void check_vol(int V)
 {
-    CTDM_REQUIRE(V == 64 || V == 128 || V == 192 || V == 256 || V == 512, "volume side has no kernel");
+    CTDM_REQUIRE(V == 32 || V == 64 || V == 128 || V == 192 || V == 256 || V == 512, "volume side has no kernel");
 }
 
 // x, y, z passes; `scale` folded into the z pass.
@@ -572,6 +577,102 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
     Plan<V>::template run<+1>(tl.t, s, twl, gl2, gs2);
 }
 
+// ---------------------------------------------------------------- projected-gradient fit
+// Measurement list of the fit: for pencil p = x + V*y, every (angle, cap sample) landing in the pencil
+// in angle order: z index and m = 4 pi i f_z spec / dx (the angle's own k-space value before hit
+// averaging). With G the unit-weight Ewald gather (F -> F[v_a(p)]) and G^T its scatter, the
+// objective J(f) = 1/2 sum_a |G_a FFT3(f) - m_a|^2 has the k-space gradient R = G^T (G F - m);
+// the fit steps F <- F - tau R / hits (hit-preconditioned) on every pencil, which for tau = 1 is
+// the reference's replacement by the averaged measurement (check_F3D).
+struct MeasList {
+    int* cnt;
+    short* idx;
+    float2* val;
+    int cap;
+};
+
+__global__ void k_pgd_build(const float2* __restrict__ spec, const double* __restrict__ rows, int A, int V,
+                            double fm02, double df, float inv_dx, MeasList ml)
+{
+    const long long VV = (long long)V * V;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= VV) return;
+    const int ii = signed_freq((int)(p % V), V), jj = signed_freq((int)(p / V), V);
+    int n = 0;
+    for (int a = 0; a < A; ++a) {
+        const Cap cp = ewald_cap(ii, jj, rows[3 * a], rows[3 * a + 1], rows[3 * a + 2], fm02, df, V);
+        if (!cp.ok) continue;
+        const float2 u = spec[(long long)a * VV + p];
+        ml.idx[p * ml.cap + n] = (short)cp.nz;
+        ml.val[p * ml.cap + n] = make_float2(-cp.w * u.y * inv_dx, cp.w * u.x * inv_dx);
+        ++n;
+    }
+    ml.cnt[p] = n;
+}
+
+// Z pass of the fit: forward z-FFT, R = G^T (G F - m) accumulated per pencil in angle order (thread t
+// owns z range [t V/T, (t+1) V/T)), F -= tau R / hits, inverse z-FFT.
+template <int V, int B>
+__global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__ W, const float2* __restrict__ tw,
+                                                            MeasList ml, float tau)
+{
+    constexpr int T = Plan<V>::T;
+    constexpr int ZR = V / T;
+    __shared__ float2 lds[B * LdsSize<V>::value];
+    __shared__ float2 twl[V];
+    __shared__ float2 racc[B][V];
+    __shared__ unsigned short hit[B][V];
+    __shared__ short lnz[B][GRID_CHUNK];
+    __shared__ float2 lval[B][GRID_CHUNK];
+    stage_twiddles<V>(twl, tw);
+    Tile<V, 2, B> tl;
+    float2* s = lds + tl.b * LdsSize<V>::value;
+    for (int z = tl.t; z < V; z += T) {
+        racc[tl.b][z] = make_float2(0.f, 0.f);
+        hit[tl.b][z] = 0;
+    }
+    float2* pen = W + tl.base;
+    auto gl = [&](int z) -> float2 { return pen[(long long)z * tl.stride]; };
+    auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
+    Plan<V>::template run<-1>(tl.t, s, twl, gl, gs);
+    __syncthreads();
+    const long long pencil = tl.base;  // x + V*y
+    const int n = ml.cnt[pencil];
+    const int z0 = tl.t * ZR;
+    for (int e0 = 0; e0 < n; e0 += GRID_CHUNK) {
+        const int ne = min(GRID_CHUNK, n - e0);
+        for (int k = tl.t; k < ne; k += T) {
+            lnz[tl.b][k] = ml.idx[pencil * ml.cap + e0 + k];
+            lval[tl.b][k] = ml.val[pencil * ml.cap + e0 + k];
+        }
+        __syncthreads();
+        for (int k = 0; k < ne; ++k) {
+            const int z = lnz[tl.b][k];
+            if (z >= z0 && z < z0 + ZR) {
+                const float2 F = s[lds_pad(z)], m = lval[tl.b][k];
+                racc[tl.b][z].x += F.x - m.x;
+                racc[tl.b][z].y += F.y - m.y;
+                hit[tl.b][z] += 1;
+            }
+        }
+        __syncthreads();
+    }
+    for (int z = z0; z < z0 + ZR; ++z) {
+        const int h = hit[tl.b][z];
+        if (h) {
+            const float c = tau / (float)h;
+            float2 F = s[lds_pad(z)];
+            F.x -= c * racc[tl.b][z].x;
+            F.y -= c * racc[tl.b][z].y;
+            s[lds_pad(z)] = F;
+        }
+    }
+    __syncthreads();
+    auto gl2 = [&](int k) -> float2 { return s[lds_pad(k)]; };
+    auto gs2 = [&](int z, float2 v) { pen[(long long)z * tl.stride] = v; };
+    Plan<V>::template run<+1>(tl.t, s, twl, gl2, gs2);
+}
+
 // Y pass (plain FFT along y).
 template <int V, int B, int DIR>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_y(float2* __restrict__ W, const float2* __restrict__ tw)
@@ -613,8 +714,6 @@ __global__ void k_dc_build(const float2* __restrict__ Fm, const float* __restric
 // The pencil's T threads evaluate the caps of interleaved angles into an LDS list (chunks of
 // GRID_CHUNK angles); thread t then owns the z range [t V/T, (t+1) V/T) and adds the list entries
 // that land there in angle order, so the sums are deterministic.
-constexpr int GRID_CHUNK = 64;
-
 template <int V, int B>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __restrict__ spec,
                                                             const double* __restrict__ rows, int A,
@@ -743,6 +842,11 @@ struct VolOps {
         hipLaunchKernelGGL((k_vol_zdc<V, B>), grid(), dim3(NT), 0, c.stream, W, c.tw(V), dc);
         CTDM_CHECK_LAUNCH("k_vol_zdc");
     }
+    static void zpgd(Ctx& c, float2* W, const MeasList& ml, float tau)
+    {
+        hipLaunchKernelGGL((k_vol_zpgd<V, B>), grid(), dim3(NT), 0, c.stream, W, c.tw(V), ml, tau);
+        CTDM_CHECK_LAUNCH("k_vol_zpgd");
+    }
     static void grid_pass(Ctx& c, const float2* spec, const double* rows, int A, const EwaldGeom& g, bool ref,
                           float2* W, const DcList& dc)
     {
@@ -779,10 +883,16 @@ struct Vol {
         if (pick(c, PASS_GRID) == 8) VolOps<V, 8>::grid_pass(c, spec, rows, A, g, ref, W, dc);
         else VolOps<V, 16>::grid_pass(c, spec, rows, A, g, ref, W, dc);
     }
+    static void zpgd(Ctx& c, float2* W, const MeasList& ml, float tau)
+    {
+        if (pick(c, PASS_ZDC) == 8) VolOps<V, 8>::zpgd(c, W, ml, tau);
+        else VolOps<V, 16>::zpgd(c, W, ml, tau);
+    }
     // Loop after the initial inverse transform of the measured spectrum is in W up to the x pass:
-    // `first_x_mode` 0 if W holds real-space f, 1 if W still needs its inverse x-FFT.
+    // `first_x_mode` 0 if W holds real-space f, 1 if W still needs its inverse x-FFT. The z step is
+    // the data-consistency replacement (dc) or, when `ml` is given, the fit's gradient step.
     static void loop(Ctx& c, float2* W, float2* n_out, const DcList& dc, const Proj& p, int iters, bool shift,
-                     int first_x_mode)
+                     int first_x_mode, const MeasList* ml = nullptr, float tau = 1.f)
     {
         const float s = (float)(1.0 / ((double)V * V * V));
         if (iters == 0) {
@@ -795,7 +905,10 @@ struct Vol {
         x(c, first_x_mode, W, n_out, p, s, shift);
         for (int it = 0; it < iters; ++it) {
             y(c, W, -1);
-            zdc(c, W, dc);
+            if (ml)
+                zpgd(c, W, *ml, tau);
+            else
+                zdc(c, W, dc);
             y(c, W, +1);
             x(c, it + 1 < iters ? 1 : 2, W, n_out, p, s, shift);
         }
@@ -823,6 +936,7 @@ void positivity_fused(Ctx& c, float2* f, const float2* Fm, const float* hits, fl
     CTDM_REQUIRE(iters >= 0, "positivity: negative iteration count");
     const Proj p = make_proj(ewald_geom(o, V));
     switch (V) {
+    case 32: positivity_fused_v<32>(c, f, Fm, hits, n, iters, p, flags); break;
     case 64: positivity_fused_v<64>(c, f, Fm, hits, n, iters, p, flags); break;
     case 128: positivity_fused_v<128>(c, f, Fm, hits, n, iters, p, flags); break;
     case 192: positivity_fused_v<192>(c, f, Fm, hits, n, iters, p, flags); break;
@@ -850,11 +964,47 @@ void volume_batched(Ctx& c, const float2* spec, const ctdm_angle* ang, int A, in
     const EwaldGeom g = ewald_geom(o, V);
     const double* rows = upload_rows(c, "vol.rows." + std::to_string((uintptr_t)n_out), angle_rows(g, ang, A));
     switch (V) {
+    case 32: volume_batched_v<32>(c, spec, rows, A, g, iters, flags, n_out); break;
     case 64: volume_batched_v<64>(c, spec, rows, A, g, iters, flags, n_out); break;
     case 128: volume_batched_v<128>(c, spec, rows, A, g, iters, flags, n_out); break;
     case 192: volume_batched_v<192>(c, spec, rows, A, g, iters, flags, n_out); break;
     case 256: volume_batched_v<256>(c, spec, rows, A, g, iters, flags, n_out); break;
     default: fail(CTDM_EUNSUPPORTED, "volume side has no kernel");
+    }
+}
+
+template <int V>
+static void fit_pgd_v(Ctx& c, float2* f, const float2* spec, const double* rows, int A, const EwaldGeom& g,
+                      int iters, float tau, float2* n, unsigned flags)
+{
+    const size_t VV = (size_t)V * V;
+    MeasList ml;
+    ml.cap = std::max(A, 1);
+    ml.cnt = (int*)c.buf("pgd.cnt", sizeof(int) * VV);
+    ml.idx = (short*)c.buf("pgd.idx", sizeof(short) * VV * ml.cap);
+    ml.val = (float2*)c.buf("pgd.val", sizeof(float2) * VV * ml.cap);
+    hipLaunchKernelGGL(k_pgd_build, dim3(grid1d((long long)VV, 256)), dim3(256), 0, c.stream, spec, rows, A, V,
+                       g.fm02, g.df, (float)(1.0 / g.dx), ml);
+    CTDM_CHECK_LAUNCH("k_pgd_build");
+    DcList none{};
+    Vol<V>::loop(c, f, n, none, make_proj(g), iters, !(flags & CTDM_POS_NO_SHIFT), 0, &ml, tau);
+}
+
+void fit_pgd(Ctx& c, float2* f, const float2* spec, const ctdm_angle* ang, int A, int V, const ctdm_optics& o,
+             int iters, double step, float2* n, unsigned flags)
+{
+    CTDM_REQUIRE(f && spec && n && f != n, "fit_pgd: bad pointers");
+    CTDM_REQUIRE(A > 0 && A < 32768 && iters >= 0, "fit_pgd: bad sizes");
+    CTDM_REQUIRE(step > 0 && step <= 2, "fit_pgd: step must lie in (0, 2]");
+    const EwaldGeom g = ewald_geom(o, V);
+    const double* rows = upload_rows(c, "pgd.rows", angle_rows(g, ang, A));
+    switch (V) {
+    case 32: fit_pgd_v<32>(c, f, spec, rows, A, g, iters, (float)step, n, flags); break;
+    case 64: fit_pgd_v<64>(c, f, spec, rows, A, g, iters, (float)step, n, flags); break;
+    case 128: fit_pgd_v<128>(c, f, spec, rows, A, g, iters, (float)step, n, flags); break;
+    case 192: fit_pgd_v<192>(c, f, spec, rows, A, g, iters, (float)step, n, flags); break;
+    case 256: fit_pgd_v<256>(c, f, spec, rows, A, g, iters, (float)step, n, flags); break;
+    default: fail(CTDM_EUNSUPPORTED, "fit_pgd: volume side has no kernel");
     }
 }
 
@@ -935,6 +1085,7 @@ double time_pass(Ctx& c, int kind, int V, int B, int reps, double* bytes)
     CTDM_REQUIRE(reps > 0 && (B == 8 || B == 16), "time_pass: bad arguments");
     CTDM_REQUIRE(kind >= 0 && kind < PASS_GRID, "time_pass: pass must be 0 (x), 1 (y) or 2 (z+dc)");
     switch (V) {
+    case 32: return time_pass_v<32>(c, kind, B, reps, bytes);
     case 64: return time_pass_v<64>(c, kind, B, reps, bytes);
     case 128: return time_pass_v<128>(c, kind, B, reps, bytes);
     case 192: return time_pass_v<192>(c, kind, B, reps, bytes);

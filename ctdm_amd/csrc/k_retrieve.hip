@@ -243,6 +243,7 @@ static void col_band_m(Ctx& c, int M, const float2* in, int nimg, int nin, int r
                        float2* out)
 {
     switch (M) {
+    case 32: launch_col_band<H, 32>(c, in, nimg, nin, r0, qb, out); break;
     case 64: launch_col_band<H, 64>(c, in, nimg, nin, r0, qb, out); break;
     case 128: launch_col_band<H, 128>(c, in, nimg, nin, r0, qb, out); break;
     case 192: launch_col_band<H, 192>(c, in, nimg, nin, r0, qb, out); break;
@@ -282,6 +283,7 @@ void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int
     default: fail(CTDM_EUNSUPPORTED, "hologram height " + std::to_string(H) + " has no kernel");
     }
     switch (M) {
+    case 32: launch_row_final<32>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;
     case 64: launch_row_final<64>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;
     case 128: launch_row_final<128>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;
     case 192: launch_row_final<192>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;

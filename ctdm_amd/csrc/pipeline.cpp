@@ -24,7 +24,8 @@ static void validate_cell(const ctdm_cell& cl)
 {
     CTDM_REQUIRE(cl.holo && cl.bg && cl.angles && cl.ri, "cell: null pointer");
     CTDM_REQUIRE(cl.A > 0 && cl.H > 0 && cl.W > 0, "cell: empty");
-    CTDM_REQUIRE(cl.V == 64 || cl.V == 128 || cl.V == 192 || cl.V == 256, "cell: volume side has no kernel");
+    CTDM_REQUIRE(cl.V == 32 || cl.V == 64 || cl.V == 128 || cl.V == 192 || cl.V == 256,
+                 "cell: volume side has no kernel");
     CTDM_REQUIRE(cl.iters >= 0, "cell: negative iteration count");
     CTDM_REQUIRE(cl.crop.radius < 0 || cl.crop.M == cl.V, "cell: crop side must equal the volume side");
     CTDM_REQUIRE(cl.optics.wavelength > 0 && cl.optics.n_med > 0 && cl.optics.dx > 0, "cell: bad optics");

@@ -178,8 +178,15 @@ ctdm_status ctdm_synth_cell(uint64_t seed, int A, const ctdm_synth_params* p, fl
             const double Aamp = std::exp(la);
             const double carrier = 2.0 * kPi * (p->carrier[0] * x + p->carrier[1] * y);
             const uint64_t idx = ((uint64_t)a * H + y) * H + x;
-            hp[x] = (float)(Aamp * Aamp + 1.0 + 2.0 * Aamp * std::cos(carrier + phi) +
-                            p->noise * gauss(seed, 3, idx));
+            // sample field E = A exp(i phi) (+ complex noise); I = |E|^2 + 1 + 2 Re(E exp(i carrier))
+            double er = Aamp * std::cos(phi), ei = Aamp * std::sin(phi);
+            if (p->field_noise > 0) {
+                const double s = p->field_noise / std::sqrt(2.0);
+                er += s * gauss(seed, 6, idx);
+                ei += s * gauss(seed, 7, idx);
+            }
+            const double cc = std::cos(carrier), ss = std::sin(carrier);
+            hp[x] = (float)(er * er + ei * ei + 1.0 + 2.0 * (er * cc - ei * ss) + p->noise * gauss(seed, 3, idx));
             bp[x] = (float)(2.0 + 2.0 * std::cos(carrier) + p->noise * gauss(seed, 4, idx));
         }
     });

@@ -217,12 +217,15 @@ ctdm_status ctdm_time_pass(ctdm_ctx *ctx, int pass, int V, int pencils, int reps
                            double *bytes);
 
 /* ---------------------------------------------------------------- differentiable fit */
-/* Projected-gradient fit of measured Rytov spectra (no reference counterpart; replaces the
- * positivity loop when chosen): minimise 0.5*sum_a |G_a(FFT3 f) - spec_a|^2 over the scattering
- * potential f, where G_a = ctdm_ewald_gather restricted to the cap and scaled by dx / (4*pi*i*f_z)^-1
- * as documented in DESIGN; each step recomputes FFT3 and its adjoint, then projects f onto the
- * positivity set (Re n >= n_med, Im n >= 0). f holds the start on entry and the result on exit;
- * n receives est_n(f) (octant-shifted unless CTDM_POS_NO_SHIFT). */
+/* Projected-gradient fit of measured Rytov spectra (no reference counterpart; an alternative to the
+ * data-consistency loop of ctdm_positivity). Minimises
+ *     J(f) = 1/2 sum_a sum_p |FFT3(f)[v_a(p)] - m_a(p)|^2,   m_a(p) = 4*pi*i*f_z * spec_a(p) / dx,
+ * over the scattering potential f (v_a(p) and f_z as in ctdm_ewald_scatter). Each of `iters` steps
+ * recomputes F = FFT3(f), forms the gradient R = G^T (G F - m) with the hand-written gather G and its
+ * transpose, takes F -= step * R / hits (hit-preconditioned), returns to real space and projects onto
+ * the positivity set (Re n >= n_med, Im n >= 0). step = 1 reproduces ctdm_positivity fed with the
+ * averaged measurement. spec: A x V x V Rytov spectra (ctdm_rytov_spectrum). f: start potential,
+ * used as workspace (overwritten). n: est_n of the result, octant-shifted unless CTDM_POS_NO_SHIFT. */
 ctdm_status ctdm_fit_pgd(ctdm_ctx *ctx, ctdm_complex *f, const ctdm_complex *spec,
                          const ctdm_angle *angles, int A, int V, const ctdm_optics *optics,
                          int iters, double step, ctdm_complex *n, unsigned flags);
@@ -243,6 +246,7 @@ typedef struct ctdm_synth_params {
     double r_nuc;       /* nucleolus radius [m] */
     double noise;       /* Gaussian intensity noise sigma */
     double carrier[2];  /* carrier frequency [cycles / pixel] along x and y */
+    double field_noise; /* sigma of complex Gaussian noise added to the sample field A exp(i phi) */
 } ctdm_synth_params;
 
 void ctdm_synth_default_params(ctdm_synth_params *p);

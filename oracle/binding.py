@@ -65,6 +65,8 @@ _EXTRA: dict = {
     "orc_branch_cuts": [C.c_void_p, C.c_int, C.c_int, C.c_void_p],
     "orc_flood_fill": [_fp, C.c_void_p, C.c_int, C.c_int, _ip, _fp],
     "orc_goldstein": [_fp, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, _ip, _fp],
+    "orc_fit_pgd": [_dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                    C.c_double, _dp, C.c_uint],
     "orc_reconstruct_cell": [_fp, _fp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                              C.c_double, C.c_double, C.c_double, C.c_int, C.c_uint, C.c_uint, _dp,
                              _dp, _dp, _dp],
@@ -269,3 +271,13 @@ def goldstein(phi):
     un = np.zeros((B, H, W), np.float32)
     lib().orc_goldstein(_p(p, _fp), B, H, W, res.ctypes.data, cut.ctypes.data, _p(k, _ip), _p(un, _fp))
     return res, cut, k, un
+
+
+def fit_pgd(f0, spec, angles, wavelength, n_med, dx, iters, step, flags=0):
+    f = _c(f0)
+    sp = _c(spec)
+    a = _ang(angles)
+    n = np.zeros_like(f)
+    lib().orc_fit_pgd(_cp(f), _cp(sp), _p(a, _dp), len(a), f.shape[0], wavelength, n_med, dx, iters, step,
+                      _cp(n), flags)
+    return n

@@ -952,3 +952,59 @@ void orc_goldstein(const float* phi, int batch, int H, int W, int8_t* res, uint8
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- projected-gradient fit
+extern "C" {
+
+// Hit-preconditioned projected gradient on J(f) = 1/2 sum_a sum_p |F[v_a(p)] - m_a(p)|^2 with
+// F = FFT3(f) and m_a(p) = 4 pi i f_z spec_a(p) / dx (the sample the reference scatters,
+// Recon_kernel.cu:224-225, before the 1/dx of Initial_F_F2, :269-270): starting from n = est_n(f),
+// each iteration is f = modify(n); F = FFT3(f); R = G^T (G F - m); F -= step * R / hits;
+// f = iFFT3(F); n = est_n(f). step = 1 reproduces orc_positivity with the averaged measurement.
+void orc_fit_pgd(const double* f0, const double* spec, const double* angles, int A, int V, double wavelength,
+                 double n_med, double dx, int iters, double step, double* n, unsigned flags)
+{
+    const Geom g = geom(wavelength, n_med, dx, V);
+    const long VV = (long)V * V, cnt = VV * V;
+    const cd* S = reinterpret_cast<const cd*>(spec);
+    std::vector<long> ev;
+    std::vector<cd> em;
+    std::vector<double> hits(cnt, 0.0);
+    for (int a = 0; a < A; ++a) {
+        const Row r = angle_row(g, angles[2 * a], angles[2 * a + 1]);
+        for (long p = 0; p < VV; ++p) {
+            const int Nx = (int)(p % V), Ny = (int)(p / V);
+            const int ii = Nx < V / 2 ? Nx : Nx - V, jj = Ny < V / 2 ? Ny : Ny - V;
+            int nz;
+            double w;
+            if (!cap(g, r, ii, jj, V, &nz, &w)) continue;
+            const cd u = S[(long)a * VV + p];
+            ev.push_back(p + VV * nz);
+            em.push_back(cd(-w * u.imag(), w * u.real()) / dx);
+            hits[p + VV * nz] += 1.0;
+        }
+    }
+    std::vector<cd> f(reinterpret_cast<const cd*>(f0), reinterpret_cast<const cd*>(f0) + cnt), R(cnt);
+    cd* nc = reinterpret_cast<cd*>(n);
+    const double s = 1.0 / (double)cnt;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < cnt; ++i) nc[i] = est_n(f[i], g);
+    for (int it = 0; it < iters; ++it) {
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < cnt; ++i) {
+            f[i] = modify_f(nc[i], g);
+            R[i] = 0.0;
+        }
+        fft3(f.data(), V, -1);
+        for (size_t e = 0; e < ev.size(); ++e) R[ev[e]] += f[ev[e]] - em[e];
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < cnt; ++i)
+            if (hits[i] > 0) f[i] -= step * R[i] / hits[i];
+        fft3(f.data(), V, +1);
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < cnt; ++i) nc[i] = est_n(f[i] * s, g);
+    }
+    if (!(flags & 2u)) shift3(nc, V);
+}
+
+}  // extern "C"
