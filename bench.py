@@ -44,6 +44,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-config4", action="store_true", help="skip the 8-cell uneven batch")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -135,6 +136,8 @@ def main() -> int:
                "ri_max_err_vs_gpu": err}
         log(f"cpu baseline {tc:.1f} s/volume, GPU-vs-oracle max|dn| {err:.2e}")
 
+    config4 = None if args.no_config4 else run_config4(ctx, args)
+
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "volumes/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
@@ -145,11 +148,78 @@ def main() -> int:
                    "iters": ITERS, "mode": "batched, hipGraph replay"},
         "plain_value": round(plain_value, 3), "speedup_vs_plain": round(value / plain_value, 2),
         "ri_max_diff_batched_vs_plain": dn, "wall_ms_per_step": round(wall, 4),
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "config4": config4,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def run_config4(ctx, args):
+    """Eight cells, A = {30,45,60,60,90,90,120,120}, V cyclic over {128,192,256}, 1024^2 holograms,
+    10 iterations: one captured graph for all cells vs the plain per-cell pipeline."""
+    import torch
+
+    import ctdm_amd
+
+    angles = [30, 45, 60, 60, 90, 90, 120, 120]
+    sides = [128, 192, 256, 128, 192, 256, 128, 192]
+    log("step: config4 synth (615 hologram pairs)")
+    cells = []
+    for i, (A_, V_) in enumerate(zip(angles, sides)):
+        holo, bg, ang = ctdm_amd.synth_cell(40 + i, A_)
+        cells.append(dict(holo=torch.from_numpy(holo).cuda(), bg=torch.from_numpy(bg).cuda(), angles=ang,
+                          V=V_, iters=ITERS, optics=ctdm_amd.optics_for(V_, H),
+                          ri=torch.empty((V_, V_, V_), dtype=torch.complex64, device="cuda")))
+        del holo, bg
+    if not args.no_tune:
+        log(ctx.autotune(sorted(set(sides))).strip())
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    log("step: config4 batched graph")
+    ctx.reconstruct(cells, mode=1, graph=True)
+    ctx.reconstruct(cells, mode=1, graph=True)
+    torch.cuda.synchronize()
+    reps = 5
+    e0, e1 = ev(), ev()
+    e0.record()
+    for _ in range(reps):
+        ctx.reconstruct(cells, mode=1, graph=True)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_batch = e0.elapsed_time(e1) / reps
+    outs = [c["ri"].clone() for c in cells]
+    per_cell = []
+    for c in cells:  # each cell alone, its own captured graph
+        ctx.reconstruct([c], mode=1, graph=True)
+        torch.cuda.synchronize()
+        e0, e1 = ev(), ev()
+        e0.record()
+        for _ in range(reps):
+            ctx.reconstruct([c], mode=1, graph=True)
+        e1.record()
+        torch.cuda.synchronize()
+        per_cell.append(round(e0.elapsed_time(e1) / reps, 3))
+    log("step: config4 plain")
+    plain_ms, dn = 0.0, 0.0
+    for c, ob in zip(cells, outs):
+        rp = torch.empty_like(ob)
+        cp = dict(c, ri=rp)
+        t0 = time.perf_counter()
+        ctx.reconstruct([cp], mode=0)
+        torch.cuda.synchronize()
+        plain_ms += (time.perf_counter() - t0) * 1e3
+        dn = max(dn, (rp - ob).abs().max().item())
+    res = {"cells": [{"A": a, "V": v} for a, v in zip(angles, sides)],
+           "batched_ms_all_cells": round(ms_batch, 3), "volumes_per_s": round(8000.0 / ms_batch, 2),
+           "per_cell_latency_ms": per_cell, "plain_ms_all_cells": round(plain_ms, 2),
+           "plain_volumes_per_s": round(8000.0 / plain_ms, 2), "speedup_vs_plain": round(plain_ms / ms_batch, 2),
+           "ri_max_diff_batched_vs_plain": dn}
+    log(f"config4: batched {ms_batch:.2f} ms for 8 cells ({res['volumes_per_s']} vol/s), plain {plain_ms:.1f} ms, "
+        f"per-cell {per_cell}")
+    for c in cells:
+        del c["holo"], c["bg"]
+    torch.cuda.empty_cache()
+    return res
 
 
 if __name__ == "__main__":
