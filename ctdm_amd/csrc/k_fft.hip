@@ -78,6 +78,7 @@ static void dispatch(Ctx& c, int n, float2* data, long long stride_n, long long 
     case 128: launch_pencils<128, RowB<128>::value, BFAST>(c, data, stride_n, stride_b, npencils, nplanes, stride_plane, dir, scale, conj_out); break;
     case 192: launch_pencils<192, RowB<192>::value, BFAST>(c, data, stride_n, stride_b, npencils, nplanes, stride_plane, dir, scale, conj_out); break;
     case 256: launch_pencils<256, RowB<256>::value, BFAST>(c, data, stride_n, stride_b, npencils, nplanes, stride_plane, dir, scale, conj_out); break;
+    case 384: launch_pencils<384, RowB<384>::value, BFAST>(c, data, stride_n, stride_b, npencils, nplanes, stride_plane, dir, scale, conj_out); break;
     case 512: launch_pencils<512, RowB<512>::value, BFAST>(c, data, stride_n, stride_b, npencils, nplanes, stride_plane, dir, scale, conj_out); break;
     case 1024: launch_pencils<1024, RowB<1024>::value, BFAST>(c, data, stride_n, stride_b, npencils, nplanes, stride_plane, dir, scale, conj_out); break;
     default: fail(CTDM_EUNSUPPORTED, "fft length " + std::to_string(n) + " has no kernel");

@@ -492,7 +492,7 @@ __global__ void k_congruent(float* __restrict__ o, const float* __restrict__ w, 
 
 void unwrap_dct_plain_one(Ctx& c, const float* wrapped, float* out, int N, unsigned flags)
 {
-    CTDM_REQUIRE(N == 32 || N == 64 || N == 128 || N == 256, "plain unwrap: N in {32, 64, 128, 256}");
+    CTDM_REQUIRE(N == 32 || N == 64 || N == 128 || N == 192 || N == 256, "plain unwrap: N in {32, 64, 128, 192, 256}");
     const size_t n = (size_t)N * N;
     float* inX = (float*)c.buf("puw.inX", sizeof(float) * n);
     float* inY = (float*)c.buf("puw.inY", sizeof(float) * n);

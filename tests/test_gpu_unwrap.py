@@ -68,7 +68,7 @@ def test_reference_mode_reproduces_reference_dumps(ctx):
 
 @pytest.mark.parametrize("flags", [0, ctdm_amd.UNWRAP_SINCOS, ctdm_amd.UNWRAP_REFCOMPAT,
                                    ctdm_amd.UNWRAP_CONGRUENT])
-@pytest.mark.parametrize("N", [128, 256])
+@pytest.mark.parametrize("N", [128, 192, 256])
 def test_plain_equals_batched(ctx, N, flags):
     w = torch.from_numpy(_maps(N, 2, 7 * N + flags)).cuda()
     a = ctx.unwrap_dct(w, flags)
