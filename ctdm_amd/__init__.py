@@ -101,6 +101,7 @@ SIGNATURES = {
     "ctdm_normalize_measured": ([_vp, _vp, _vp, _vp, C.c_int, C.c_double], C.c_int),
     "ctdm_fft3d": ([_vp, _vp, C.c_int, C.c_int], C.c_int),
     "ctdm_positivity": ([_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(Optics), C.c_uint], C.c_int),
+    "ctdm_recon_from_maps": ([_vp, _vp, _vp, C.POINTER(Angle), C.c_int, C.c_int, C.POINTER(Optics), C.c_int, C.c_uint, _vp], C.c_int),
     "ctdm_reconstruct": ([_vp, C.POINTER(Cell), C.c_int, C.POINTER(BatchOpts)], C.c_int),
     "ctdm_autotune": ([_vp, C.POINTER(C.c_int), C.c_int, C.c_char_p, C.c_size_t], C.c_int),
     "ctdm_time_pass": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
@@ -332,6 +333,14 @@ class Context:
         self._call("ctdm_fit_pgd", _ptr(f), _ptr(spec), arr, len(arr), f.shape[0],
                    C.byref(optics), iters, step, _ptr(n), flags)
         return n
+
+    def recon_from_maps(self, unwrapped, logamp, angles, optics: Optics, iters: int, flags: int = 0):
+        A, V, _ = unwrapped.shape
+        arr = angles_array(angles)
+        ri = self._torch.empty((V, V, V), dtype=self._torch.complex64, device=unwrapped.device)
+        self._call("ctdm_recon_from_maps", _ptr(unwrapped), _ptr(logamp), arr, A, V, C.byref(optics), iters,
+                   flags, _ptr(ri))
+        return ri
 
     # -- pipeline
     def reconstruct(self, cells: list[dict], mode: int = 1, graph: bool = False,

@@ -245,6 +245,16 @@ ctdm_status ctdm_positivity(ctdm_ctx* ctx, ctdm_complex* f, const ctdm_complex* 
     });
 }
 
+ctdm_status ctdm_recon_from_maps(ctdm_ctx* ctx, const float* unwrapped, const float* logamp,
+                                 const ctdm_angle* angles, int A, int V, const ctdm_optics* optics, int iters,
+                                 unsigned recon_flags, ctdm_complex* ri)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        check_optics(optics);
+        ctdm::recon_from_maps(c, unwrapped, logamp, angles, A, V, *optics, iters, recon_flags, (float2*)ri);
+    });
+}
+
 ctdm_status ctdm_reconstruct(ctdm_ctx* ctx, ctdm_cell* cells, int ncells, const ctdm_batch_opts* opts)
 {
     return guarded(ctx, [&](ctdm::Ctx& c) {

@@ -94,6 +94,45 @@ static void run_plain(Ctx& c, ctdm_cell& cl)
     CTDM_HIP(hipStreamSynchronize(c.stream));
 }
 
+__global__ void k_exp(const float* __restrict__ la, float* __restrict__ amp, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) amp[i] = expf(la[i]);
+}
+
+void recon_from_maps(Ctx& c, const float* unwrapped, const float* logamp, const ctdm_angle* ang, int A, int V,
+                     const ctdm_optics& o, int iters, unsigned flags, float2* ri)
+{
+    CTDM_REQUIRE(unwrapped && logamp && ang && ri, "recon_from_maps: null pointer");
+    CTDM_REQUIRE(A > 0 && iters >= 0, "recon_from_maps: bad sizes");
+    CTDM_REQUIRE(V == 32 || V == 64 || V == 128 || V == 192 || V == 256, "recon_from_maps: volume side has no kernel");
+    const size_t img = (size_t)V * V;
+    if (flags & CTDM_EXEC_PLAIN) {  // BatchRecon's own sequence: per angle copy/shift/FFT/scatter, unfused loop
+        const EwaldGeom g = ewald_geom(o, V);
+        const auto rows = angle_rows(g, ang, A);
+        float* amp = (float*)c.buf("rm.amp", sizeof(float) * A * img);
+        hipLaunchKernelGGL(k_exp, dim3(grid1d((long long)A * img, 256)), dim3(256), 0, c.stream, logamp, amp,
+                           (long long)A * img);
+        CTDM_CHECK_LAUNCH("k_exp");
+        float2* F = (float2*)c.buf("pl.F", sizeof(float2) * img * V);
+        float* hits = (float*)c.buf("pl.hits", sizeof(float) * img * V);
+        float2* Fm = (float2*)c.buf("pl.Fm", sizeof(float2) * img * V);
+        float2* u_sp = (float2*)c.buf("pl.usp", sizeof(float2) * img);
+        zero_volume(c, F, hits, V);
+        for (int a = 0; a < A; ++a) {
+            rytov_spectrum_plain_one(c, unwrapped + a * img, amp + a * img, V, u_sp);
+            scatter_plain_one(c, u_sp, rows[a], V, g, F, hits);
+        }
+        normalize_measured(c, F, hits, Fm, V, g.dx);
+        fft3d(c, F, V, +1);
+        positivity_plain(c, F, Fm, hits, ri, V, iters, o, flags);
+        return;
+    }
+    float2* spec = (float2*)c.buf("rm.spec", sizeof(float2) * A * img);
+    rytov_spectrum(c, unwrapped, logamp, A, V, spec);
+    volume_batched(c, spec, ang, A, V, o, iters, flags, ri);
+}
+
 static void enqueue_batched(Ctx& c, ctdm_cell& cl)
 {
     const int A = cl.A, V = cl.V;

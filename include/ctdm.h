@@ -171,6 +171,16 @@ ctdm_status ctdm_positivity(ctdm_ctx *ctx, ctdm_complex *f, const ctdm_complex *
                             const float *hits, ctdm_complex *n, int V, int iters,
                             const ctdm_optics *optics, unsigned flags);
 
+/* Replaces BatchRecon (HilbertTransform.cu:1825-2047) from its own inputs: A unwrapped-phase and
+ * log-amplitude maps (V x V each, device) and angles (HOST) -> Rytov spectra, Ewald gridding with hit
+ * averaging, inverse 3-D DFT and `iters` positivity iterations; writes the octant-shifted complex
+ * RI to `ri` (V^3, device). The reference takes the amplitude and logs it inside (cuCopyArray,
+ * Recon_kernel.cu:165); here the log-amplitude comes in. recon_flags: CTDM_DC_* |
+ * CTDM_POS_NO_SHIFT | CTDM_EXEC_PLAIN (reference-structured kernel sequence). */
+ctdm_status ctdm_recon_from_maps(ctdm_ctx *ctx, const float *unwrapped, const float *logamp,
+                                 const ctdm_angle *angles, int A, int V, const ctdm_optics *optics,
+                                 int iters, unsigned recon_flags, ctdm_complex *ri);
+
 /* ---------------------------------------------------------------- pipeline */
 /* One cell: A hologram pairs -> one V^3 RI volume (reference: per-frame loop
  * HilbertTransform.cu:785-1165 followed by BatchRecon, HilbertTransform.cu:1825-2047). */

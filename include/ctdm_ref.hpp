@@ -1,0 +1,99 @@
+// ctdm_ref.hpp — C++ host layer over the libctdm C ABI that keeps the reference program's function
+// names and argument meaning, so the reference's per-frame loop (HilbertTransform.cu:785-1165) and
+// BatchRecon call (:1243) map one to one. Differences from the reference, all deliberate:
+//   - every pointer is device memory owned by the caller (the reference mixes host inputs and
+//     driver-owned globals, HilbertTransform.cu:480-574, 1558-1559);
+//   - errors throw ctdm_ref::Error instead of exit() (checkCudaErrors);
+//   - the context carries the stream, scratch and FFT tables the reference kept in globals.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ctdm.h"
+
+namespace ctdm_ref {
+
+struct Error : std::runtime_error {
+    ctdm_status status;
+    Error(ctdm_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+class Context {
+public:
+    explicit Context(int device = 0, ctdm_stream stream = nullptr)
+    {
+        ctdm_config cfg{};
+        cfg.device = device;
+        const ctdm_status s = ctdm_create(&ctx_, &cfg, stream);
+        if (s != CTDM_OK) throw Error(s, "ctdm_create failed");
+    }
+    ~Context() { ctdm_destroy(ctx_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    ctdm_ctx* get() const { return ctx_; }
+    void check(ctdm_status s) const
+    {
+        if (s != CTDM_OK) throw Error(s, ctdm_last_error(ctx_));
+    }
+    void sync() const { check(ctdm_sync(ctx_)); }
+
+private:
+    ctdm_ctx* ctx_ = nullptr;
+};
+
+// extractQPI_AlgorithmA(SP, BG, cuSP_FFT, cuBG_FFT, Nx, Ny, frame) (HilbertTransform.cu:1546)
+// followed by calcWrapPhase (HilbertTransform.cu:3618): one Nx x Ny hologram pair in, the Nx/4
+// (or crop.M, when set) wrapped phase and log-amplitude out. On frame 1 the sideband is located from
+// BG and stored in `crop` for the following frames, as the reference does (HilbertTransform.cu:1572).
+inline void extractQPI_AlgorithmA(const Context& c, const float* SP, const float* BG, int Nx, int Ny, int frame,
+                                  ctdm_crop& crop, float* wrapped, float* logamp)
+{
+    if (frame == 1 || crop.radius < 0) {
+        const int M = crop.M > 0 ? crop.M : Nx / 4;
+        c.check(ctdm_find_sideband(c.get(), BG, Ny, Nx, M, 0, &crop));
+    }
+    c.check(ctdm_retrieve(c.get(), SP, BG, 1, Ny, Nx, &crop, wrapped, logamp, nullptr));
+}
+
+// DCT_UWLS_Unwrapped(float *ImgDst, float *ImgSrc, int sizeX, int sizeY) (DCT_Unwrapping.cu:10).
+// flags = CTDM_UNWRAP_REFCOMPAT reproduces the reference's eigenvalues and IDCT exactly.
+inline void DCT_UWLS_Unwrapped(const Context& c, float* ImgDst, const float* ImgSrc, int sizeX, int sizeY,
+                               unsigned flags = CTDM_UNWRAP_WRAPDIFF)
+{
+    if (sizeX != sizeY) throw Error(CTDM_EINVAL, "DCT_UWLS_Unwrapped: square maps only");
+    c.check(ctdm_unwrap_dct(c.get(), ImgSrc, ImgDst, 1, sizeX, flags));
+}
+
+// FastUnwrapping(float *ImgSrc, float *ImgDst, int sizeX, int sizeY) (FastUnwrapping.cu:5):
+// sin/cos-Laplacian right-hand side, Poisson solve, congruence rounding.
+inline void FastUnwrapping(const Context& c, const float* ImgSrc, float* ImgDst, int sizeX, int sizeY)
+{
+    if (sizeX != sizeY) throw Error(CTDM_EINVAL, "FastUnwrapping: square maps only");
+    c.check(ctdm_unwrap_dct(c.get(), ImgSrc, ImgDst, 1, sizeX, CTDM_UNWRAP_SINCOS | CTDM_UNWRAP_CONGRUENT));
+}
+
+// cudaGoldsteinUnwrap2D(float *cu_phi, float *cu_UnWrapPhase, int sizeX, int sizeY, int frameNumber)
+// (Goldstein.cu:5). The frame number only labelled the reference's debug output.
+inline void cudaGoldsteinUnwrap2D(const Context& c, const float* cu_phi, float* cu_UnWrapPhase, int sizeX,
+                                  int sizeY, int /*frameNumber*/)
+{
+    c.check(ctdm_goldstein(c.get(), cu_phi, 1, sizeY, sizeX, nullptr, nullptr, nullptr, cu_UnWrapPhase));
+}
+
+// BatchRecon(float *PhaseStack, float *AmpStack, float *sampleAngleRadX, float *sampleAngleRadY,
+// int ReconFrame) (HilbertTransform.cu:1825): ReconFrame unwrapped-phase and log-amplitude maps of
+// V x V (device), angles on the host, result = octant-shifted complex RI volume (device, V^3) that
+// the reference wrote to a .rimap file (HilbertTransform.cu:1974-2041).
+inline void BatchRecon(const Context& c, const float* PhaseStack, const float* LogAmpStack,
+                       const float* sampleAngleRadX, const float* sampleAngleRadY, int ReconFrame, int V,
+                       const ctdm_optics& optics, int iters, ctdm_complex* ri, unsigned recon_flags = 0)
+{
+    std::vector<ctdm_angle> ang(ReconFrame);
+    for (int i = 0; i < ReconFrame; ++i) ang[i] = ctdm_angle{sampleAngleRadX[i], sampleAngleRadY[i]};
+    c.check(ctdm_recon_from_maps(c.get(), PhaseStack, LogAmpStack, ang.data(), ReconFrame, V, &optics, iters,
+                                 recon_flags, ri));
+}
+
+}  // namespace ctdm_ref

@@ -205,3 +205,22 @@ def test_autotune_keeps_results_and_times_passes(ctx):
     for k in (0, 1, 2):
         ms, nbytes = ctx.time_pass(k, V, 0, 5)
         assert ms > 0 and nbytes >= 2 * 8 * V ** 3
+
+
+@pytest.mark.parametrize("flags", [0, ctdm_amd.EXEC_PLAIN])
+def test_recon_from_maps_matches_pipeline_and_oracle(ctx, orc, flags):
+    """BatchRecon boundary: the unwrapped-phase and log-amplitude maps the pipeline writes, fed back
+    through ctdm_recon_from_maps, give the pipeline's RI to FP32 rounding, and the oracle's to 1e-5."""
+    A, H, V, iters = 12, 256, 64, 3
+    holo, bg, ang = _cell(21, A, H, V)
+    un = torch.empty((A, V, V), dtype=torch.float32, device="cuda")
+    la = torch.empty_like(un)
+    rb, cb = _run(ctx, holo, bg, ang, V, iters, mode=1, unwrapped=un, logamp=la)
+    o = ctdm_amd.optics_for(V, H)
+    got = ctx.recon_from_maps(un, la, ang, o, iters, flags)
+    torch.cuda.synchronize()
+    d = (got - rb).abs().max().item()
+    print("recon_from_maps flags", flags, "vs pipeline", d)
+    assert d <= 1e-6
+    ref = orc.reconstruct_cell(holo, bg, ang, V, cb["crop_used"].astuple(), LAM, NM, o.dx, iters)
+    assert np.abs(got.cpu().numpy() - ref).max() <= 1e-5
