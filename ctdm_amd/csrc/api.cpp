@@ -268,7 +268,7 @@ ctdm_status ctdm_reconstruct(ctdm_ctx* ctx, ctdm_cell* cells, int ncells, const 
 ctdm_status ctdm_autotune(ctdm_ctx* ctx, const int* sizes, int nsizes, char* report, size_t len)
 {
     return guarded(ctx, [&](ctdm::Ctx& c) {
-        const std::string r = ctdm::autotune(c, sizes, nsizes);
+        const std::string r = ctdm::autotune(c, sizes, nsizes, true);
         if (report && len) {
             std::strncpy(report, r.c_str(), len - 1);
             report[len - 1] = 0;

@@ -1,7 +1,8 @@
 // Autotuner: for every requested volume side, times each fused volume pass (x with projection,
 // y, z with data consistency) with 8 and 16 pencils per workgroup on this device and keeps the
 // faster choice in the context (used by every later launch of that side, including graph
-// captures made afterwards). Radix plans are fixed per length (fft_dev.hpp); the gridding pass
+// captures made afterwards). An explicit call re-times every side; a reconstruction asking for
+// tuning only times sides not tuned yet. Radix plans are fixed per length (fft_dev.hpp); the gridding pass
 // keeps 16 pencils (it is timed only inside a full reconstruction).
 #include <cstdio>
 #include <sstream>
@@ -10,7 +11,7 @@
 
 namespace ctdm {
 
-std::string autotune(Ctx& c, const int* sizes, int nsizes)
+std::string autotune(Ctx& c, const int* sizes, int nsizes, bool force)
 {
     CTDM_REQUIRE(sizes || nsizes == 0, "autotune: null sizes");
     CTDM_REQUIRE(!c.capturing, "autotune during capture");
@@ -19,7 +20,7 @@ std::string autotune(Ctx& c, const int* sizes, int nsizes)
     for (int i = 0; i < nsizes; ++i) {
         const int V = sizes[i];
         Tune& t = c.tuning(V);
-        if (t.tuned) continue;
+        if (t.tuned && !force) continue;
         for (int k = 0; k < 3; ++k) {
             double bytes = 0;
             const double m8 = time_pass(c, k, V, 8, 10, &bytes);

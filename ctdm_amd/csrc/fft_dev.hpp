@@ -136,6 +136,18 @@ struct LdsSize {
     static constexpr int value = N + N / 16;
 };
 
+// Spacing (float2) of consecutive transforms' LDS areas. When the lanes of a wave walk the transform
+// index fastest (LANES transforms side by side in each 16-lane group: the pencil and column kernels
+// whose global accesses are coalesced across transforms), a spacing that is a multiple of 16 float2
+// (128 B, every LDS bank once) would put all of them on one bank. Offsetting the spacing by 16 / LANES
+// float2 makes (transform, slot) -> bank pair one-to-one for the padded radix-16 slot patterns
+// 17 j + r and j + 17 r. LANES = 1 (the transform's own threads fastest) keeps the plain size.
+template <int N, int LANES>
+struct LdsStride {
+    static constexpr int value =
+        LANES <= 1 ? LdsSize<N>::value : (LdsSize<N>::value + 15) / 16 * 16 + (LANES >= 16 ? 1 : 16 / LANES);
+};
+
 // Copies a per-N twiddle table into LDS at kernel entry (the copy overlaps the first data loads;
 // the first Stockham stage uses no twiddles and its barrier publishes the table).
 template <int N>

@@ -156,7 +156,7 @@ double time_pass(Ctx& c, int kind, int V, int B, int reps, double* bytes);
 void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& opts);
 void recon_from_maps(Ctx& c, const float* unwrapped, const float* logamp, const ctdm_angle* ang, int A, int V,
                      const ctdm_optics& o, int iters, unsigned flags, float2* ri);
-std::string autotune(Ctx& c, const int* sizes, int nsizes);
+std::string autotune(Ctx& c, const int* sizes, int nsizes, bool force);
 
 inline int grid1d(long long n, int block) { return (int)((n + block - 1) / block); }
 

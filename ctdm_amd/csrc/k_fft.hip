@@ -20,7 +20,8 @@ __global__ __launch_bounds__(B* Plan<N>::T) void k_fft_pencils(
     int conj_out)
 {
     constexpr int T = Plan<N>::T;
-    __shared__ float2 lds[B * LdsSize<N>::value];
+    constexpr int LS = LdsStride<N, BFAST ? B : 1>::value;
+    __shared__ float2 lds[B * LS];
     __shared__ float2 twl[N];
     stage_twiddles<N>(twl, tw);
     const int tid = threadIdx.x;
@@ -31,7 +32,7 @@ __global__ __launch_bounds__(B* Plan<N>::T) void k_fft_pencils(
     const long long pencil = tin * B + b;
     const bool valid = pencil < npencils_per_plane;
     float2* base = data + pl * stride_plane + (valid ? pencil : 0) * stride_b;
-    float2* s = lds + b * LdsSize<N>::value;
+    float2* s = lds + b * LS;
     auto gl = [&](int i) -> float2 { return valid ? base[(long long)i * stride_n] : make_float2(0.f, 0.f); };
     auto gs = [&](int i, float2 v) {
         if (!valid) return;

@@ -128,13 +128,14 @@ __global__ __launch_bounds__(B* Plan<M>::T) void k_rytov_cols(const float2* __re
                                                               float2* __restrict__ spec)
 {
     constexpr int T = Plan<M>::T;
-    __shared__ float2 lds[B * LdsSize<M>::value];
+    constexpr int LS = LdsStride<M, B>::value;
+    __shared__ float2 lds[B * LS];
     const int b = threadIdx.x % B, t = threadIdx.x / B;
     const int tiles = M / B;
     const long long a = blockIdx.x / tiles;
     const int kx0 = (blockIdx.x % tiles) * B;
     const float2* col = P + a * M * M + kx0 + b;
-    float2* s = lds + b * LdsSize<M>::value;
+    float2* s = lds + b * LS;
     auto gl = [&](int y) -> float2 { return col[(long long)y * M]; };
     auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
     Plan<M>::template run<-1>(t, s, tw, gl, gs);
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(B* Plan<M>::T) void k_rytov_cols(const float2* __re
     float2* dst = spec + a * M * M;
     for (int e = threadIdx.x; e < B * M; e += B * T) {
         const int bb = e / M, ky = e % M, kx = kx0 + bb;
-        float2 v = lds[bb * LdsSize<M>::value + lds_pad(ky)];
+        float2 v = lds[bb * LS + lds_pad(ky)];
         if ((kx + ky) & 1) v = make_float2(-v.x, -v.y);
         dst[(long long)kx * M + ky] = v;
     }
@@ -407,6 +408,14 @@ __global__ void k_fill(float* __restrict__ p, long long cnt, float v)
     if (i < cnt) p[i] = v;
 }
 
+// Streaming copy, 16 B per lane: the bandwidth yardstick the volume passes are measured against and the
+// known-size kernel that calibrates the memory-traffic counters.
+__global__ __launch_bounds__(256) void k_copy_vol(const float4* __restrict__ s, float4* __restrict__ d, long long n4)
+{
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n4; i += stride) d[i] = s[i];
+}
+
 static void fill(Ctx& c, void* p, long long nfloats, float v)
 {
     hipLaunchKernelGGL(k_fill, dim3(grid1d(nfloats, 256)), dim3(256), 0, c.stream, (float*)p, nfloats, v);
@@ -533,7 +542,8 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
 {
     constexpr int T = Plan<V>::T;
     constexpr int DC_PRE = 4;
-    __shared__ float2 lds[B * LdsSize<V>::value];
+    constexpr int LS = LdsStride<V, B>::value;
+    __shared__ float2 lds[B * LS];
     __shared__ float2 twl[V];
     stage_twiddles<V>(twl, tw);
     // list prefetch: lanes walk the entries of one pencil (contiguous in the list)
@@ -551,14 +561,14 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
         pv[r] = dc.val[e];
     }
     Tile<V, 2, B> tl;
-    float2* s = lds + tl.b * LdsSize<V>::value;
+    float2* s = lds + tl.b * LS;
     float2* pen = W + tl.base;
     auto gl = [&](int z) -> float2 { return pen[(long long)z * tl.stride]; };
     auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
     Plan<V>::template run<-1>(tl.t, s, twl, gl, gs);
     __syncthreads();
     {
-        float2* sp = lds + pb * LdsSize<V>::value;
+        float2* sp = lds + pb * LS;
 #pragma unroll
         for (int r = 0; r < DC_PRE; ++r) {
             const int k = k0 + r * T;
@@ -618,15 +628,16 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__
 {
     constexpr int T = Plan<V>::T;
     constexpr int ZR = V / T;
-    __shared__ float2 lds[B * LdsSize<V>::value];
+    constexpr int LS = LdsStride<V, B>::value;
+    __shared__ float2 lds[B * LS];
     __shared__ float2 twl[V];
-    __shared__ float2 racc[B][V];
-    __shared__ unsigned short hit[B][V];
-    __shared__ short lnz[B][GRID_CHUNK];
-    __shared__ float2 lval[B][GRID_CHUNK];
+    __shared__ float2 racc[B][V + 1];
+    __shared__ unsigned short hit[B][V + 2];
+    __shared__ short lnz[B][GRID_CHUNK + 2];
+    __shared__ float2 lval[B][GRID_CHUNK + 1];
     stage_twiddles<V>(twl, tw);
     Tile<V, 2, B> tl;
-    float2* s = lds + tl.b * LdsSize<V>::value;
+    float2* s = lds + tl.b * LS;
     for (int z = tl.t; z < V; z += T) {
         racc[tl.b][z] = make_float2(0.f, 0.f);
         hit[tl.b][z] = 0;
@@ -677,11 +688,12 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__
 template <int V, int B, int DIR>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_y(float2* __restrict__ W, const float2* __restrict__ tw)
 {
-    __shared__ float2 lds[B * LdsSize<V>::value];
+    constexpr int LS = LdsStride<V, B>::value;
+    __shared__ float2 lds[B * LS];
     __shared__ float2 twl[V];
     stage_twiddles<V>(twl, tw);
     Tile<V, 1, B> tl;
-    float2* s = lds + tl.b * LdsSize<V>::value;
+    float2* s = lds + tl.b * LS;
     float2* pen = W + tl.base;
     auto gl = [&](int y) -> float2 { return pen[(long long)y * tl.stride]; };
     auto gs = [&](int k, float2 v) { pen[(long long)k * tl.stride] = v; };
@@ -723,15 +735,16 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
 {
     constexpr int T = Plan<V>::T;
     constexpr int ZR = V / T;  // z values owned by one thread
-    __shared__ float2 lds[B * LdsSize<V>::value];
-    __shared__ unsigned short hit[B][V];
-    __shared__ short lnz[B][GRID_CHUNK];
-    __shared__ float2 lval[B][GRID_CHUNK];
-    __shared__ int lcnt[B][T];
+    constexpr int LS = LdsStride<V, B>::value;
+    __shared__ float2 lds[B * LS];
+    __shared__ unsigned short hit[B][V + 2];
+    __shared__ short lnz[B][GRID_CHUNK + 2];
+    __shared__ float2 lval[B][GRID_CHUNK + 1];
+    __shared__ int lcnt[B][T + 1];
     __shared__ float2 twl[V];
     stage_twiddles<V>(twl, tw);
     Tile<V, 2, B> tl;
-    float2* s = lds + tl.b * LdsSize<V>::value;
+    float2* s = lds + tl.b * LS;
     for (int z = tl.t; z < V; z += T) {
         s[lds_pad(z)] = make_float2(0.f, 0.f);
         hit[tl.b][z] = 0;
@@ -857,7 +870,7 @@ struct VolOps {
 };
 
 // Runtime choice of B per pass kind (autotuned; 16 by default).
-enum PassKind { PASS_X = 0, PASS_Y = 1, PASS_ZDC = 2, PASS_GRID = 3, NPASS = 4 };
+enum PassKind { PASS_X = 0, PASS_Y = 1, PASS_ZDC = 2, PASS_GRID = 3, NPASS = 4, PASS_COPY = 4 };
 
 template <int V>
 struct Vol {
@@ -1055,8 +1068,14 @@ static double time_pass_v(Ctx& c, int kind, int B, int reps, double* bytes)
         extra = sizeof(int) * (double)VV + n * (sizeof(float2) + sizeof(short));
     }
     *bytes = vb + extra;
+    const long long n4 = (long long)vol / 2;  // float4 = two complex values
+    const int copy_grid = (int)std::min<long long>(grid1d(n4, 256), 8192);
     auto launch = [&](int r) {
         switch (kind) {
+        case PASS_COPY:
+            hipLaunchKernelGGL(k_copy_vol, dim3(copy_grid), dim3(256), 0, c.stream, (const float4*)W, (float4*)nbuf, n4);
+            CTDM_CHECK_LAUNCH("k_copy_vol");
+            break;
         case PASS_X: B == 8 ? VolOps<V, 8>::x(c, 1, W, nbuf, p, 1.f / (float)vol, true)
                             : VolOps<V, 16>::x(c, 1, W, nbuf, p, 1.f / (float)vol, true); break;
         case PASS_Y: B == 8 ? VolOps<V, 8>::y(c, W, (r & 1) ? 1 : -1) : VolOps<V, 16>::y(c, W, (r & 1) ? 1 : -1); break;
@@ -1083,7 +1102,8 @@ double time_pass(Ctx& c, int kind, int V, int B, int reps, double* bytes)
 {
     CTDM_REQUIRE(!c.capturing, "time_pass during capture");
     CTDM_REQUIRE(reps > 0 && (B == 8 || B == 16), "time_pass: bad arguments");
-    CTDM_REQUIRE(kind >= 0 && kind < PASS_GRID, "time_pass: pass must be 0 (x), 1 (y) or 2 (z+dc)");
+    CTDM_REQUIRE(kind >= 0 && kind <= PASS_COPY && kind != PASS_GRID,
+                 "time_pass: pass must be 0 (x), 1 (y), 2 (z+dc) or 4 (plain copy)");
     switch (V) {
     case 32: return time_pass_v<32>(c, kind, B, reps, bytes);
     case 64: return time_pass_v<64>(c, kind, B, reps, bytes);

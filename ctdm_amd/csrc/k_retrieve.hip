@@ -87,15 +87,16 @@ __global__ __launch_bounds__(C* Plan<H>::T) void k_col_band(
     const float2* __restrict__ tw_m, int r0, int q_lo, int q_n, float2* __restrict__ out)
 {
     constexpr int T = Plan<H>::T;
-    static_assert(LdsSize<M>::value <= LdsSize<H>::value, "band transform must fit the column area");
-    __shared__ float2 lds[C * LdsSize<H>::value];
+    constexpr int LS = LdsStride<H, C>::value;
+    static_assert(LdsSize<M>::value <= LS, "band transform must fit the column area");
+    __shared__ float2 lds[C * LS];
     const int cidx = threadIdx.x % C, t = threadIdx.x / C;  // lanes walk columns (coalesced)
     const int tiles = (nin + C - 1) / C;
     const int img = blockIdx.x / tiles;
     const int q = (blockIdx.x % tiles) * C + cidx;
     const bool valid = q < nin && img < nimg;
     const float2* col = in + (long long)img * H * nin + (valid ? q : 0);
-    float2* s = lds + cidx * LdsSize<H>::value;
+    float2* s = lds + cidx * LS;
     auto gl = [&](int i) -> float2 { return valid ? col[(long long)i * nin] : make_float2(0.f, 0.f); };
     auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };
     Plan<H>::template run<-1>(t, s, tw_h, gl, gs);

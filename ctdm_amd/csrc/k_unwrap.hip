@@ -160,14 +160,15 @@ __global__ __launch_bounds__(C* Plan<LN>::T) void k_uw_cols(const float* __restr
                                                             float* __restrict__ out)
 {
     constexpr int TL = Plan<LN>::T;  // threads per column (sized for the longest transform used)
-    __shared__ float2 lds[C * LdsSize<LN>::value];
+    constexpr int LS = LdsStride<LN, C>::value;
+    __shared__ float2 lds[C * LS];
     __shared__ float coef[C][N + 1];
     const int cidx = threadIdx.x % C, t = threadIdx.x / C;
     const int tiles = N / C;
     const long long b = blockIdx.x / tiles;
     const int kx = (blockIdx.x % tiles) * C + cidx;
     const float* col = in + b * N * N + kx;
-    float2* s = lds + cidx * LdsSize<LN>::value;
+    float2* s = lds + cidx * LS;
     float* X = coef[cidx];
     auto gl = [&](int n) -> float2 { return make_float2(col[(long long)mk_src(n, N) * N], 0.f); };
     auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };

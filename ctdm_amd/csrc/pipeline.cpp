@@ -189,7 +189,7 @@ void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& op
     if (opts.autotune) {
         std::vector<int> sizes;
         for (int i = 0; i < ncells; ++i) sizes.push_back(cells[i].V);
-        autotune(c, sizes.data(), (int)sizes.size());
+        autotune(c, sizes.data(), (int)sizes.size(), false);
     }
     if (!opts.graph) {
         for (int i = 0; i < ncells; ++i) enqueue_batched(c, cells[i]);

@@ -214,12 +214,14 @@ ctdm_status ctdm_reconstruct(ctdm_ctx *ctx, ctdm_cell *cells, int ncells,
 
 /* Times every kernel variant for volume/crop side V (FFT radix plan, pencils per workgroup,
  * gridding block) on this device and caches the fastest in the context (no reference
- * counterpart). Synchronises. `report` (HOST, may be NULL) receives a text summary. */
+ * counterpart). Every listed side is re-timed, also one tuned before. Synchronises. `report` (HOST,
+ * may be NULL) receives a text summary. */
 ctdm_status ctdm_autotune(ctdm_ctx *ctx, const int *sizes, int nsizes, char *report,
                           size_t report_len);
 
 /* Average duration [ms] of one launch of a fused volume pass (pass 0: x with projection, 1: y,
- * 2: z with data consistency) at volume side V with `pencils` pencils per workgroup (8 or 16;
+ * 2: z with data consistency; 4: a plain streaming copy of the V^3 complex64 volume, the bandwidth
+ * yardstick) at volume side V with `pencils` pencils per workgroup (8 or 16;
  * <= 0 = the tuned choice), timed with hipEvents on the context stream over `reps` launches on the
  * context's volume buffer (no reference counterpart: the measuring primitive of ctdm_autotune).
  * *bytes receives the algorithmic bytes of one launch. Synchronises. */
