@@ -148,17 +148,26 @@ struct LdsStride {
         LANES <= 1 ? LdsSize<N>::value : (LdsSize<N>::value + 15) / 16 * 16 + (LANES >= 16 ? 1 : 16 / LANES);
 };
 
-// Copies a per-N twiddle table into LDS at kernel entry (the copy overlaps the first data loads;
-// the first Stockham stage uses no twiddles and its barrier publishes the table).
+// Copies a per-N twiddle table into LDS at kernel entry, padded like the data areas (dst holds
+// LdsSize<N>::value float2). The copy overlaps the first data loads; the first Stockham stage uses
+// no twiddles and its barrier publishes the table.
 template <int N>
 __device__ __forceinline__ void stage_twiddles(float2* dst, const float2* __restrict__ src)
 {
-    for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < N; i += blockDim.x) dst[lds_pad(i)] = src[i];
 }
 
-// Twiddle from a per-N table tw[m] = exp(-2*pi*i*m/N) (float2, computed in double on the host).
-template <int DIR>
-__device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m)
+// View of an LDS twiddle table written by stage_twiddles. The padding keeps the lanes of one
+// transform (slot k = lane) on distinct banks for the strided lookups m = r * k of the later stages.
+struct TwPad {
+    const float2* p;
+    __device__ __forceinline__ float2 operator[](int m) const { return p[lds_pad(m)]; }
+};
+
+// Twiddle from a per-N table tw[m] = exp(-2*pi*i*m/N) (float2, computed in double on the host): a
+// global pointer or a TwPad view.
+template <int DIR, class TW>
+__device__ __forceinline__ float2 twiddle(const TW& tw, int m)
 {
     const float2 w = tw[m];
     return DIR > 0 ? make_float2(w.x, -w.y) : w;
@@ -172,9 +181,8 @@ __device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m)
 // arguments: it contains __syncthreads().
 template <int N, int T, int DIR, int L, int R, int... Rest>
 struct Stockham {
-    template <class GLoad, class GStore>
-    __device__ __forceinline__ static void run(int t, float2* s, const float2* __restrict__ tw,
-                                               GLoad& gload, GStore& gstore)
+    template <class TW, class GLoad, class GStore>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GLoad& gload, GStore& gstore)
     {
         constexpr int S = N / R;
         constexpr int P = (S + T - 1) / T;
@@ -223,14 +231,25 @@ struct Stockham {
     }
 };
 
+// Workgroup index remapped so that consecutive logical workgroups run on one XCD. The dispatcher
+// hands workgroup i to XCD i mod 8; with per = n / 8, logical = (i mod 8) * per + i / 8 for the
+// first 8 * per workgroups (a bijection onto [0, 8 * per)), the tail keeps its index. Tiles whose
+// rows share 128-byte lines then meet in the same L2 instead of fetching the line once per XCD.
+__device__ __forceinline__ int xcd_block(int i, int n)
+{
+    const int per = n >> 3;
+    if (i >= per * 8) return i;
+    return (i & 7) * per + (i >> 3);
+}
+
 // Radix plans per transform length. T = threads cooperating on one transform.
 template <int N>
 struct Plan;
 template <>
 struct Plan<32> {
     static constexpr int T = 4;
-    template <int DIR, class GL, class GS>
-    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
         Stockham<32, T, DIR, 1, 8, 4>::run(t, s, tw, gl, gs);
     }
@@ -238,8 +257,8 @@ struct Plan<32> {
 template <>
 struct Plan<64> {
     static constexpr int T = 8;
-    template <int DIR, class GL, class GS>
-    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
         Stockham<64, T, DIR, 1, 8, 8>::run(t, s, tw, gl, gs);
     }
@@ -247,8 +266,8 @@ struct Plan<64> {
 template <>
 struct Plan<128> {
     static constexpr int T = 16;
-    template <int DIR, class GL, class GS>
-    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
         Stockham<128, T, DIR, 1, 8, 16>::run(t, s, tw, gl, gs);
     }
@@ -256,8 +275,8 @@ struct Plan<128> {
 template <>
 struct Plan<192> {
     static constexpr int T = 16;
-    template <int DIR, class GL, class GS>
-    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
         Stockham<192, T, DIR, 1, 3, 4, 16>::run(t, s, tw, gl, gs);
     }
@@ -265,8 +284,8 @@ struct Plan<192> {
 template <>
 struct Plan<256> {
     static constexpr int T = 16;
-    template <int DIR, class GL, class GS>
-    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
         Stockham<256, T, DIR, 1, 16, 16>::run(t, s, tw, gl, gs);
     }
@@ -274,8 +293,8 @@ struct Plan<256> {
 template <>
 struct Plan<384> {
     static constexpr int T = 16;
-    template <int DIR, class GL, class GS>
-    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
         Stockham<384, T, DIR, 1, 3, 8, 16>::run(t, s, tw, gl, gs);
     }
@@ -283,8 +302,8 @@ struct Plan<384> {
 template <>
 struct Plan<512> {
     static constexpr int T = 32;
-    template <int DIR, class GL, class GS>
-    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
         Stockham<512, T, DIR, 1, 16, 32>::run(t, s, tw, gl, gs);
     }
@@ -292,8 +311,8 @@ struct Plan<512> {
 template <>
 struct Plan<1024> {
     static constexpr int T = 64;
-    template <int DIR, class GL, class GS>
-    __device__ __forceinline__ static void run(int t, float2* s, const float2* tw, GL& gl, GS& gs)
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
         Stockham<1024, T, DIR, 1, 16, 16, 4>::run(t, s, tw, gl, gs);
     }

@@ -22,8 +22,9 @@ __global__ __launch_bounds__(B* Plan<N>::T) void k_fft_pencils(
     constexpr int T = Plan<N>::T;
     constexpr int LS = LdsStride<N, BFAST ? B : 1>::value;
     __shared__ float2 lds[B * LS];
-    __shared__ float2 twl[N];
+    __shared__ float2 twl[LdsSize<N>::value];
     stage_twiddles<N>(twl, tw);
+    const TwPad twp{twl};
     const int tid = threadIdx.x;
     const int b = BFAST ? (tid % B) : (tid / T);
     const int t = BFAST ? (tid / B) : (tid % T);
@@ -40,7 +41,7 @@ __global__ __launch_bounds__(B* Plan<N>::T) void k_fft_pencils(
         if (conj_out) v.y = -v.y;
         base[(long long)i * stride_n] = v;
     };
-    Plan<N>::template run<DIR>(t, s, twl, gl, gs);
+    Plan<N>::template run<DIR>(t, s, twp, gl, gs);
 }
 
 template <int N, int B, bool BFAST>

@@ -342,20 +342,27 @@ static Proj make_proj(const EwaldGeom& g)
 }
 
 // calculate_n3D (Recon_kernel.cu:305-342): n = n_m sqrt(1 - f/k^2) with Re n >= n_m, Im n >= 0.
-// The square root is evaluated in the cancellation-free form (b = d / 2a for c >= 0); it equals the
-// reference's (sqrt((|z|+c)/2), sign(d) sqrt((|z|-c)/2)) in exact arithmetic.
+// The square root is evaluated in the cancellation-free form: with h = (|z| + |c|) / 2 the larger
+// component is sqrt(h) and the smaller |d| / (2 sqrt(h)); it equals the reference's
+// (sqrt((|z|+c)/2), sign(d) sqrt((|z|-c)/2)) in exact arithmetic. The hardware square root and
+// reciprocal square root (v_sqrt_f32, v_rsq_f32, about 1 ulp) replace the correctly rounded library
+// sequences: the x pass that applies this per voxel is bound by vector ALU issue, and the error
+// stays four orders below the 1e-5 RI bar.
 __device__ __forceinline__ float2 est_n(float2 f, const Proj& p)
 {
     const float c = 1.f - f.x * p.inv_k2;
     const float d = -f.y * p.inv_k2;
-    const float r = sqrtf(c * c + d * d);
+    const float r = __builtin_amdgcn_sqrtf(c * c + d * d);
+    const float h = 0.5f * (r + fabsf(c));
+    const float big = __builtin_amdgcn_sqrtf(h);
+    const float small = h > 0.f ? 0.5f * fabsf(d) * __builtin_amdgcn_rsqf(h) : 0.f;
     float a, b;
     if (c >= 0.f) {
-        a = sqrtf(0.5f * (r + c));
-        b = (a > 0.f) ? d / (2.f * a) : 0.f;
+        a = big;
+        b = copysignf(small, d);
     } else {
-        b = copysignf(sqrtf(0.5f * (r - c)), d);
-        a = (d != 0.f) ? d / (2.f * b) : 0.f;
+        a = small;
+        b = copysignf(big, d);
     }
     if (d == 0.f) b = 0.f;
     float nr = a * p.nm, ni = b * p.nm;
@@ -466,28 +473,33 @@ struct Tile {
     long long base;  // element offset of this thread's pencil
     long long stride;
     int px, py;      // pencil coordinates (the two fixed axes)
+    // Pencils of B < 16 side by side cover less than a 128-byte line per row; neighbouring tiles then
+    // share lines and are kept on one XCD (xcd_block).
+    static constexpr bool REMAP = BFAST && B * 8 < 128;
+    static __device__ int block() { return REMAP ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x; }
     __device__ Tile()
     {
         const int tid = threadIdx.x;
+        const int blk = block();
         b = BFAST ? tid % B : tid / T;
         t = BFAST ? tid / B : tid % T;
         const long long VV = (long long)V * V;
         if (AXIS == 0) {  // pencil (y, z): rows
-            const long long r = (long long)blockIdx.x * B + b;
+            const long long r = (long long)blk * B + b;
             px = (int)(r % V);
             py = (int)(r / V);
             base = r * V;
             stride = 1;
         } else if (AXIS == 1) {  // pencil (x, z)
             const int tiles = V / B;
-            px = (int)(blockIdx.x % tiles) * B + b;
-            py = (int)(blockIdx.x / tiles);
+            px = (blk % tiles) * B + b;
+            py = blk / tiles;
             base = px + VV * py;
             stride = V;
         } else {  // pencil (x, y)
             const int tiles = V / B;
-            px = (int)(blockIdx.x % tiles) * B + b;
-            py = (int)(blockIdx.x / tiles);
+            px = (blk % tiles) * B + b;
+            py = blk / tiles;
             base = px + (long long)V * py;
             stride = VV;
         }
@@ -502,8 +514,9 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_x(float2* __restrict__ W,
                                                          int shift)
 {
     __shared__ float2 lds[B * LdsSize<V>::value];
-    __shared__ float2 twl[V];
+    __shared__ float2 twl[LdsSize<V>::value];
     stage_twiddles<V>(twl, tw);
+    const TwPad twp{twl};
     Tile<V, 0, B> tl;
     float2* s = lds + tl.b * LdsSize<V>::value;
     float2* row = W + tl.base;
@@ -514,21 +527,21 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_x(float2* __restrict__ W,
     if constexpr (MODE == 0) {
         auto gl = [&](int x) -> float2 { return modify_f(est_n(row[x], p), p); };
         auto gs = [&](int k, float2 v) { row[k] = v; };
-        Plan<V>::template run<-1>(tl.t, s, twl, gl, gs);
+        Plan<V>::template run<-1>(tl.t, s, twp, gl, gs);
     } else if constexpr (MODE == 3) {
         for (int x = tl.t; x < V; x += Plan<V>::T) put_n(x, est_n(row[x], p));
     } else {
         auto gl = [&](int k) -> float2 { return row[k]; };
         if constexpr (MODE == 1) {
             auto gs = [&](int x, float2 v) { s[lds_pad(x)] = modify_f(est_n(cscale(v, scale), p), p); };
-            Plan<V>::template run<+1>(tl.t, s, twl, gl, gs);
+            Plan<V>::template run<+1>(tl.t, s, twp, gl, gs);
             __syncthreads();
             auto gl2 = [&](int x) -> float2 { return s[lds_pad(x)]; };
             auto gs2 = [&](int k, float2 v) { row[k] = v; };
-            Plan<V>::template run<-1>(tl.t, s, twl, gl2, gs2);
+            Plan<V>::template run<-1>(tl.t, s, twp, gl2, gs2);
         } else {
             auto gs = [&](int x, float2 v) { put_n(x, est_n(cscale(v, scale), p)); };
-            Plan<V>::template run<+1>(tl.t, s, twl, gl, gs);
+            Plan<V>::template run<+1>(tl.t, s, twp, gl, gs);
         }
     }
 }
@@ -544,11 +557,13 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
     constexpr int DC_PRE = 4;
     constexpr int LS = LdsStride<V, B>::value;
     __shared__ float2 lds[B * LS];
-    __shared__ float2 twl[V];
+    __shared__ float2 twl[LdsSize<V>::value];
     stage_twiddles<V>(twl, tw);
+    const TwPad twp{twl};
     // list prefetch: lanes walk the entries of one pencil (contiguous in the list)
     const int pb = threadIdx.x / T, k0 = threadIdx.x % T;
-    const long long dpencil = (long long)(blockIdx.x % (V / B)) * B + pb + (long long)V * (blockIdx.x / (V / B));
+    const int blk = Tile<V, 2, B>::block();
+    const long long dpencil = (long long)(blk % (V / B)) * B + pb + (long long)V * (blk / (V / B));
     const int n = dc.cnt[dpencil];
     short pz[DC_PRE];
     float2 pv[DC_PRE];
@@ -565,7 +580,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
     float2* pen = W + tl.base;
     auto gl = [&](int z) -> float2 { return pen[(long long)z * tl.stride]; };
     auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
-    Plan<V>::template run<-1>(tl.t, s, twl, gl, gs);
+    Plan<V>::template run<-1>(tl.t, s, twp, gl, gs);
     __syncthreads();
     {
         float2* sp = lds + pb * LS;
@@ -584,7 +599,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
     __syncthreads();
     auto gl2 = [&](int k) -> float2 { return s[lds_pad(k)]; };
     auto gs2 = [&](int z, float2 v) { pen[(long long)z * tl.stride] = v; };
-    Plan<V>::template run<+1>(tl.t, s, twl, gl2, gs2);
+    Plan<V>::template run<+1>(tl.t, s, twp, gl2, gs2);
 }
 
 // ---------------------------------------------------------------- projected-gradient fit
@@ -630,12 +645,13 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__
     constexpr int ZR = V / T;
     constexpr int LS = LdsStride<V, B>::value;
     __shared__ float2 lds[B * LS];
-    __shared__ float2 twl[V];
+    __shared__ float2 twl[LdsSize<V>::value];
     __shared__ float2 racc[B][V + 1];
     __shared__ unsigned short hit[B][V + 2];
     __shared__ short lnz[B][GRID_CHUNK + 2];
     __shared__ float2 lval[B][GRID_CHUNK + 1];
     stage_twiddles<V>(twl, tw);
+    const TwPad twp{twl};
     Tile<V, 2, B> tl;
     float2* s = lds + tl.b * LS;
     for (int z = tl.t; z < V; z += T) {
@@ -645,7 +661,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__
     float2* pen = W + tl.base;
     auto gl = [&](int z) -> float2 { return pen[(long long)z * tl.stride]; };
     auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
-    Plan<V>::template run<-1>(tl.t, s, twl, gl, gs);
+    Plan<V>::template run<-1>(tl.t, s, twp, gl, gs);
     __syncthreads();
     const long long pencil = tl.base;  // x + V*y
     const int n = ml.cnt[pencil];
@@ -681,7 +697,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__
     __syncthreads();
     auto gl2 = [&](int k) -> float2 { return s[lds_pad(k)]; };
     auto gs2 = [&](int z, float2 v) { pen[(long long)z * tl.stride] = v; };
-    Plan<V>::template run<+1>(tl.t, s, twl, gl2, gs2);
+    Plan<V>::template run<+1>(tl.t, s, twp, gl2, gs2);
 }
 
 // Y pass (plain FFT along y).
@@ -690,14 +706,15 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_y(float2* __restrict__ W,
 {
     constexpr int LS = LdsStride<V, B>::value;
     __shared__ float2 lds[B * LS];
-    __shared__ float2 twl[V];
+    __shared__ float2 twl[LdsSize<V>::value];
     stage_twiddles<V>(twl, tw);
+    const TwPad twp{twl};
     Tile<V, 1, B> tl;
     float2* s = lds + tl.b * LS;
     float2* pen = W + tl.base;
     auto gl = [&](int y) -> float2 { return pen[(long long)y * tl.stride]; };
     auto gs = [&](int k, float2 v) { pen[(long long)k * tl.stride] = v; };
-    Plan<V>::template run<DIR>(tl.t, s, twl, gl, gs);
+    Plan<V>::template run<DIR>(tl.t, s, twp, gl, gs);
 }
 
 // Compact list from a dense measured spectrum (positivity API): one thread per pencil.
@@ -741,8 +758,9 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
     __shared__ short lnz[B][GRID_CHUNK + 2];
     __shared__ float2 lval[B][GRID_CHUNK + 1];
     __shared__ int lcnt[B][T + 1];
-    __shared__ float2 twl[V];
+    __shared__ float2 twl[LdsSize<V>::value];
     stage_twiddles<V>(twl, tw);
+    const TwPad twp{twl};
     Tile<V, 2, B> tl;
     float2* s = lds + tl.b * LS;
     for (int z = tl.t; z < V; z += T) {
@@ -809,7 +827,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
     float2* pen = W + tl.base;
     auto gl = [&](int k) -> float2 { return s[lds_pad(k)]; };
     auto gs = [&](int z, float2 v) { pen[VV * z] = v; };
-    Plan<V>::template run<+1>(tl.t, s, twl, gl, gs);
+    Plan<V>::template run<+1>(tl.t, s, twp, gl, gs);
 }
 
 static DcList dc_alloc(Ctx& c, int V, int cap)

@@ -92,8 +92,9 @@ __global__ __launch_bounds__(C* Plan<H>::T) void k_col_band(
     __shared__ float2 lds[C * LS];
     const int cidx = threadIdx.x % C, t = threadIdx.x / C;  // lanes walk columns (coalesced)
     const int tiles = (nin + C - 1) / C;
-    const int img = blockIdx.x / tiles;
-    const int q = (blockIdx.x % tiles) * C + cidx;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);  // neighbouring column tiles share lines
+    const int img = blk / tiles;
+    const int q = (blk % tiles) * C + cidx;
     const bool valid = q < nin && img < nimg;
     const float2* col = in + (long long)img * H * nin + (valid ? q : 0);
     float2* s = lds + cidx * LS;
