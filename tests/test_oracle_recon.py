@@ -87,3 +87,51 @@ def test_positivity_fixed_point(orc):
     np.testing.assert_allclose(out, n, atol=1e-12)
     shifted = orc.positivity(f, F, np.ones((V, V, V)), 1, LAM, NM, flags=0)
     np.testing.assert_allclose(shifted, np.fft.fftshift(n), atol=1e-12)
+
+
+def _fit_case(orc, V=16, A=5, seed=4):
+    rng = np.random.default_rng(seed)
+    dx = 0.4e-6 * 256 / V
+    ang = _angles(A)
+    spec = rng.standard_normal((A, V, V)) + 1j * rng.standard_normal((A, V, V))
+    n = (NM + rng.uniform(0, 0.03, (V, V, V))) + 1j * rng.uniform(0, 0.003, (V, V, V))
+    f0 = orc.modify_f(n, LAM, NM)
+    return dx, ang, spec, f0
+
+
+def test_fit_step_one_is_the_positivity_loop(orc):
+    """F - 1 * G^T(G F - m) / hits replaces every measured voxel by the hit-averaged measurement, i.e.
+    the data-consistency step of the positivity loop with F_meas = normalize(scatter(spec))."""
+    V = 16
+    dx, ang, spec, f0 = _fit_case(orc, V)
+    F, hits = orc.ewald_scatter(spec, ang, V, LAM, NM, dx)
+    _, Fm = orc.normalize(F, hits, dx)
+    for iters in (1, 3):
+        fit = orc.fit_pgd(f0, spec, ang, LAM, NM, dx, iters, 1.0)
+        pos = orc.positivity(f0, Fm, hits, iters, LAM, NM)
+        np.testing.assert_allclose(fit, pos, rtol=0, atol=1e-10)  # summation order differs
+
+
+def test_fit_step_zero_keeps_the_feasible_start(orc):
+    """With step 0 the k-space update vanishes: each iteration is FFT, inverse FFT and the projection,
+    which leaves a feasible n unchanged."""
+    V = 16
+    dx, ang, spec, f0 = _fit_case(orc, V)
+    n0 = orc.est_n(f0, LAM, NM)
+    out = orc.fit_pgd(f0, spec, ang, LAM, NM, dx, 3, 0.0, flags=2)
+    np.testing.assert_allclose(out, n0, rtol=0, atol=1e-12)
+
+
+def test_fit_half_step_moves_halfway(orc):
+    """One iteration with step 1/2: every measured voxel becomes (F0 + Fm) / 2, with F0 the spectrum of
+    the projected start, so the fit equals one positivity iteration whose measured spectrum is that
+    average."""
+    V = 16
+    dx, ang, spec, f0 = _fit_case(orc, V, seed=5)
+    F, hits = orc.ewald_scatter(spec, ang, V, LAM, NM, dx)
+    _, Fm = orc.normalize(F, hits, dx)
+    F0 = np.fft.fftn(orc.modify_f(orc.est_n(f0, LAM, NM), LAM, NM))
+    half = np.where(hits > 0, 0.5 * (F0 + Fm), F0)
+    fit = orc.fit_pgd(f0, spec, ang, LAM, NM, dx, 1, 0.5)
+    pos = orc.positivity(f0, half, hits, 1, LAM, NM)
+    np.testing.assert_allclose(fit, pos, rtol=0, atol=1e-10)
