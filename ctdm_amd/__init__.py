@@ -31,6 +31,8 @@ UNWRAP_REFCOMPAT = UNWRAP_REFCOMPAT_EIG | UNWRAP_REFCOMPAT_IDCT
 DC_HITMASK = 0x0
 DC_REFCOMPAT = 0x1
 POS_NO_SHIFT = 0x2
+MAPS_MEDIAN = 0x4
+MAPS_AMPLITUDE = 0x8
 EXEC_PLAIN = 0x100
 
 
@@ -101,6 +103,8 @@ SIGNATURES = {
     "ctdm_normalize_measured": ([_vp, _vp, _vp, _vp, C.c_int, C.c_double], C.c_int),
     "ctdm_fft3d": ([_vp, _vp, C.c_int, C.c_int], C.c_int),
     "ctdm_positivity": ([_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.POINTER(Optics), C.c_uint], C.c_int),
+    "ctdm_median3x3": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_int], C.c_int),
+    "ctdm_check_maps": ([_vp, _vp, _vp, C.c_size_t], C.c_int),
     "ctdm_recon_from_maps": ([_vp, _vp, _vp, C.POINTER(Angle), C.c_int, C.c_int, C.POINTER(Optics), C.c_int, C.c_uint, _vp], C.c_int),
     "ctdm_reconstruct": ([_vp, C.POINTER(Cell), C.c_int, C.POINTER(BatchOpts)], C.c_int),
     "ctdm_autotune": ([_vp, C.POINTER(C.c_int), C.c_int, C.c_char_p, C.c_size_t], C.c_int),
@@ -334,11 +338,24 @@ class Context:
                    C.byref(optics), iters, step, _ptr(n), flags)
         return n
 
-    def recon_from_maps(self, unwrapped, logamp, angles, optics: Optics, iters: int, flags: int = 0):
+    def median3x3(self, maps):
+        """3x3 median of each map (batch x H x W float32), borders copied (MedianFilter_gpu)."""
+        m = maps if maps.dim() == 3 else maps[None]
+        out = self._torch.empty_like(m)
+        self._call("ctdm_median3x3", _ptr(m), _ptr(out), m.shape[0], m.shape[1], m.shape[2])
+        return out if maps.dim() == 3 else out[0]
+
+    def check_maps(self, phase=None, amp=None):
+        """checkMaps in place: phase NaN/Inf -> 0; amplitude NaN/Inf -> 1, 0 -> 0.01, else |A|."""
+        n = (phase if phase is not None else amp).numel()
+        self._call("ctdm_check_maps", _ptr(phase), _ptr(amp), n)
+
+    def recon_from_maps(self, unwrapped, map2, angles, optics: Optics, iters: int, flags: int = 0):
+        """BatchRecon: map2 is ln A, or A with MAPS_AMPLITUDE; MAPS_MEDIAN filters both maps first."""
         A, V, _ = unwrapped.shape
         arr = angles_array(angles)
         ri = self._torch.empty((V, V, V), dtype=self._torch.complex64, device=unwrapped.device)
-        self._call("ctdm_recon_from_maps", _ptr(unwrapped), _ptr(logamp), arr, A, V, C.byref(optics), iters,
+        self._call("ctdm_recon_from_maps", _ptr(unwrapped), _ptr(map2), arr, A, V, C.byref(optics), iters,
                    flags, _ptr(ri))
         return ri
 

@@ -245,6 +245,16 @@ ctdm_status ctdm_positivity(ctdm_ctx* ctx, ctdm_complex* f, const ctdm_complex* 
     });
 }
 
+ctdm_status ctdm_median3x3(ctdm_ctx* ctx, const float* in, float* out, int batch, int H, int W)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) { ctdm::median3x3(c, in, out, batch, H, W); });
+}
+
+ctdm_status ctdm_check_maps(ctdm_ctx* ctx, float* phase, float* amp, size_t n)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) { ctdm::check_maps(c, phase, amp, (long long)n); });
+}
+
 ctdm_status ctdm_recon_from_maps(ctdm_ctx* ctx, const float* unwrapped, const float* logamp,
                                  const ctdm_angle* angles, int A, int V, const ctdm_optics* optics, int iters,
                                  unsigned recon_flags, ctdm_complex* ri)

@@ -156,6 +156,10 @@ double time_pass(Ctx& c, int kind, int V, int B, int reps, double* bytes);
 void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& opts);
 void recon_from_maps(Ctx& c, const float* unwrapped, const float* logamp, const ctdm_angle* ang, int A, int V,
                      const ctdm_optics& o, int iters, unsigned flags, float2* ri);
+void median3x3(Ctx& c, const float* in, float* out, int batch, int H, int W);
+void check_maps(Ctx& c, float* phase, float* amp, long long n);
+void prepare_maps(Ctx& c, const float* ph, const float* m2, float* ph_out, float* la_out, int batch, int M,
+                  unsigned flags);
 std::string autotune(Ctx& c, const int* sizes, int nsizes, bool force);
 
 inline int grid1d(long long n, int block) { return (int)((n + block - 1) / block); }

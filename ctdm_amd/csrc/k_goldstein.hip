@@ -14,8 +14,9 @@
 //                     first neighbour of the previous level in the order right, left, below, above.
 //   4. k_gs_finish    one thread per pixel: interior cut pixels next to a reached pixel take k from
 //                     it (same order); unwrapped = phi + 2 pi k, or 0 where unreached.
-// Float operations that decide integers use explicitly rounded intrinsics (no contraction), matching
-// the host specification compiled without contraction.
+// Float operations that decide integers are written as single rounded operations and this file is
+// built with -ffp-contract=off (the _rn intrinsics are plain operations that the compiler would
+// otherwise fuse into FMAs), matching the host specification compiled without contraction.
 #include <climits>
 
 #include "internal.hpp"

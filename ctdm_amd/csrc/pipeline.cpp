@@ -75,6 +75,18 @@ static void run_plain(Ctx& c, ctdm_cell& cl)
                            cl.logamp_opt, n);
         CTDM_CHECK_LAUNCH("k_wrap_log");
     }
+    const float* ph_r = unw;
+    const float* amp_r = amp;
+    if (cl.recon_flags & CTDM_MAPS_MEDIAN) {  // BatchRecon: MedianFilter on both stacks, then checkMaps
+        float* ph2 = (float*)c.buf("pl.phm", sizeof(float) * A * img);
+        float* amp2 = (float*)c.buf("pl.ampm", sizeof(float) * A * img);
+        median3x3(c, unw, ph2, A, V, V);
+        median3x3(c, amp, amp2, A, V, V);
+        check_maps(c, ph2, amp2, (long long)A * img);
+        CTDM_HIP(hipStreamSynchronize(c.stream));
+        ph_r = ph2;
+        amp_r = amp2;
+    }
     const EwaldGeom g = ewald_geom(cl.optics, V);
     const auto rows = angle_rows(g, cl.angles, A);
     float2* F = (float2*)c.buf("pl.F", sizeof(float2) * img * V);
@@ -83,7 +95,7 @@ static void run_plain(Ctx& c, ctdm_cell& cl)
     float2* u_sp = (float2*)c.buf("pl.usp", sizeof(float2) * img);
     zero_volume(c, F, hits, V);
     for (int a = 0; a < A; ++a) {
-        rytov_spectrum_plain_one(c, unw + a * img, amp + a * img, V, u_sp);
+        rytov_spectrum_plain_one(c, ph_r + a * img, amp_r + a * img, V, u_sp);
         scatter_plain_one(c, u_sp, rows[a], V, g, F, hits);
         CTDM_HIP(hipStreamSynchronize(c.stream));
     }
@@ -100,27 +112,48 @@ __global__ void k_exp(const float* __restrict__ la, float* __restrict__ amp, lon
     if (i < n) amp[i] = expf(la[i]);
 }
 
-void recon_from_maps(Ctx& c, const float* unwrapped, const float* logamp, const ctdm_angle* ang, int A, int V,
+void recon_from_maps(Ctx& c, const float* unwrapped, const float* map2, const ctdm_angle* ang, int A, int V,
                      const ctdm_optics& o, int iters, unsigned flags, float2* ri)
 {
-    CTDM_REQUIRE(unwrapped && logamp && ang && ri, "recon_from_maps: null pointer");
+    CTDM_REQUIRE(unwrapped && map2 && ang && ri, "recon_from_maps: null pointer");
     CTDM_REQUIRE(A > 0 && iters >= 0, "recon_from_maps: bad sizes");
     CTDM_REQUIRE(V == 32 || V == 64 || V == 128 || V == 192 || V == 256, "recon_from_maps: volume side has no kernel");
     const size_t img = (size_t)V * V;
-    if (flags & CTDM_EXEC_PLAIN) {  // BatchRecon's own sequence: per angle copy/shift/FFT/scatter, unfused loop
+    const long long n = (long long)A * img;
+    const bool median = flags & CTDM_MAPS_MEDIAN, amplitude = flags & CTDM_MAPS_AMPLITUDE;
+    if (flags & CTDM_EXEC_PLAIN) {  // BatchRecon's own sequence: median, checkMaps, per angle copy/shift/FFT/scatter
         const EwaldGeom g = ewald_geom(o, V);
         const auto rows = angle_rows(g, ang, A);
-        float* amp = (float*)c.buf("rm.amp", sizeof(float) * A * img);
-        hipLaunchKernelGGL(k_exp, dim3(grid1d((long long)A * img, 256)), dim3(256), 0, c.stream, logamp, amp,
-                           (long long)A * img);
-        CTDM_CHECK_LAUNCH("k_exp");
+        const float* amp_in = map2;
+        if (!amplitude) {
+            float* e = (float*)c.buf("rm.exp", sizeof(float) * n);
+            hipLaunchKernelGGL(k_exp, dim3(grid1d(n, 256)), dim3(256), 0, c.stream, map2, e, n);
+            CTDM_CHECK_LAUNCH("k_exp");
+            amp_in = e;
+        }
+        const float* ph = unwrapped;
+        const float* amp = amp_in;
+        if (median || amplitude) {
+            float* ph2 = (float*)c.buf("rm.ph", sizeof(float) * n);
+            float* amp2 = (float*)c.buf("rm.amp", sizeof(float) * n);
+            if (median) {
+                median3x3(c, unwrapped, ph2, A, V, V);
+                median3x3(c, amp_in, amp2, A, V, V);
+            } else {
+                CTDM_HIP(hipMemcpyAsync(ph2, unwrapped, sizeof(float) * n, hipMemcpyDeviceToDevice, c.stream));
+                CTDM_HIP(hipMemcpyAsync(amp2, amp_in, sizeof(float) * n, hipMemcpyDeviceToDevice, c.stream));
+            }
+            check_maps(c, ph2, amp2, n);
+            ph = ph2;
+            amp = amp2;
+        }
         float2* F = (float2*)c.buf("pl.F", sizeof(float2) * img * V);
         float* hits = (float*)c.buf("pl.hits", sizeof(float) * img * V);
         float2* Fm = (float2*)c.buf("pl.Fm", sizeof(float2) * img * V);
         float2* u_sp = (float2*)c.buf("pl.usp", sizeof(float2) * img);
         zero_volume(c, F, hits, V);
         for (int a = 0; a < A; ++a) {
-            rytov_spectrum_plain_one(c, unwrapped + a * img, amp + a * img, V, u_sp);
+            rytov_spectrum_plain_one(c, ph + a * img, amp + a * img, V, u_sp);
             scatter_plain_one(c, u_sp, rows[a], V, g, F, hits);
         }
         normalize_measured(c, F, hits, Fm, V, g.dx);
@@ -128,8 +161,17 @@ void recon_from_maps(Ctx& c, const float* unwrapped, const float* logamp, const 
         positivity_plain(c, F, Fm, hits, ri, V, iters, o, flags);
         return;
     }
+    const float* un = unwrapped;
+    const float* la = map2;
+    if (median || amplitude) {  // median, checkMaps and ln A in one pass
+        float* un2 = (float*)c.buf("rm.ph", sizeof(float) * n);
+        float* la2 = (float*)c.buf("rm.la", sizeof(float) * n);
+        prepare_maps(c, unwrapped, map2, un2, la2, A, V, flags);
+        un = un2;
+        la = la2;
+    }
     float2* spec = (float2*)c.buf("rm.spec", sizeof(float2) * A * img);
-    rytov_spectrum(c, unwrapped, logamp, A, V, spec);
+    rytov_spectrum(c, un, la, A, V, spec);
     volume_batched(c, spec, ang, A, V, o, iters, flags, ri);
 }
 
@@ -145,7 +187,16 @@ static void enqueue_batched(Ctx& c, ctdm_cell& cl)
     float2* spec = (float2*)c.buf("bt.spec." + key, sizeof(float2) * n);
     retrieve_batched(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.crop, wr, la, nullptr);
     unwrap_dct_batched(c, wr, un, A, V, cl.unwrap_flags);
-    rytov_spectrum(c, un, la, A, V, spec);
+    const float* un_r = un;
+    const float* la_r = la;
+    if (cl.recon_flags & CTDM_MAPS_MEDIAN) {  // BatchRecon's median + checkMaps on the maps it receives
+        float* un2 = (float*)c.buf("bt.unm." + key, sizeof(float) * n);
+        float* la2 = (float*)c.buf("bt.lam." + key, sizeof(float) * n);
+        prepare_maps(c, un, la, un2, la2, A, V, CTDM_MAPS_MEDIAN);
+        un_r = un2;
+        la_r = la2;
+    }
+    rytov_spectrum(c, un_r, la_r, A, V, spec);
     volume_batched(c, spec, cl.angles, A, V, cl.optics, cl.iters, cl.recon_flags, (float2*)cl.ri);
 }
 

@@ -79,6 +79,10 @@ typedef struct ctdm_angle { double ang_x, ang_y; } ctdm_angle;
 #define CTDM_DC_HITMASK     0x0u   /* data consistency where the hit count is > 0 (default) */
 #define CTDM_DC_REFCOMPAT   0x1u   /* reference test: F2.re != 0 && F2.im != 0 (Recon_kernel.cu:406) */
 #define CTDM_POS_NO_SHIFT   0x2u   /* skip the final 3-D octant shift (HilbertTransform.cu:1974) */
+#define CTDM_MAPS_MEDIAN    0x4u   /* 3x3 median of the phase and amplitude maps before gridding
+                                      (MedianFilter, Recon_kernel.cu:7-27; see ctdm_median3x3) */
+#define CTDM_MAPS_AMPLITUDE 0x8u   /* ctdm_recon_from_maps: the second map holds the amplitude A (BatchRecon's
+                                      AmpStack), sanitised by checkMaps and logged on the device */
 #define CTDM_EXEC_PLAIN     0x100u /* run the reference-structured, unfused kernel sequence */
 
 /* ---------------------------------------------------------------- context */
@@ -171,12 +175,27 @@ ctdm_status ctdm_positivity(ctdm_ctx *ctx, ctdm_complex *f, const ctdm_complex *
                             const float *hits, ctdm_complex *n, int V, int iters,
                             const ctdm_optics *optics, unsigned flags);
 
-/* Replaces BatchRecon (HilbertTransform.cu:1825-2047) from its own inputs: A unwrapped-phase and
- * log-amplitude maps (V x V each, device) and angles (HOST) -> Rytov spectra, Ewald gridding with hit
+/* Replaces MedianFilter_gpu (Recon_kernel.cu:65-119) as MedianFilter (:7-27) applies it: 3x3 median of
+ * each H x W map of `in` into `out` (device, must not overlap); first and last rows and columns are
+ * copied. The reference filters in place while neighbours are being overwritten (its result depends
+ * on thread order); this filter reads only the unfiltered input. The nine values are gathered and
+ * selected as the reference does (column-major 3x3 gather, fifth pick of its selection sort), so NaN
+ * inputs resolve the same way. */
+ctdm_status ctdm_median3x3(ctdm_ctx *ctx, const float *in, float *out, int batch, int H, int W);
+
+/* Replaces checkMaps (Recon_kernel.cu:121-142), in place on n values (device; either may be NULL):
+ * phase NaN/Inf -> 0; amplitude NaN/Inf -> 1, 0 -> 0.01, otherwise |A|. */
+ctdm_status ctdm_check_maps(ctdm_ctx *ctx, float *phase, float *amp, size_t n);
+
+/* Replaces BatchRecon (HilbertTransform.cu:1825-2047) from its own inputs: A unwrapped-phase maps and
+ * A second maps (V x V each, device) and angles (HOST) -> Rytov spectra, Ewald gridding with hit
  * averaging, inverse 3-D DFT and `iters` positivity iterations; writes the octant-shifted complex
- * RI to `ri` (V^3, device). The reference takes the amplitude and logs it inside (cuCopyArray,
- * Recon_kernel.cu:165); here the log-amplitude comes in. recon_flags: CTDM_DC_* |
- * CTDM_POS_NO_SHIFT | CTDM_EXEC_PLAIN (reference-structured kernel sequence). */
+ * RI to `ri` (V^3, device). The second map is the log-amplitude, or with CTDM_MAPS_AMPLITUDE the
+ * amplitude that the reference sanitises and logs itself (checkMaps, cuCopyArray,
+ * Recon_kernel.cu:121-169). CTDM_MAPS_MEDIAN runs the reference's 3x3 median on both maps first
+ * (BatchRecon, HilbertTransform.cu:1878). recon_flags: CTDM_DC_* | CTDM_POS_NO_SHIFT | CTDM_MAPS_* |
+ * CTDM_EXEC_PLAIN (reference-structured kernel sequence). BatchRecon itself corresponds to
+ * CTDM_MAPS_MEDIAN | CTDM_MAPS_AMPLITUDE. */
 ctdm_status ctdm_recon_from_maps(ctdm_ctx *ctx, const float *unwrapped, const float *logamp,
                                  const ctdm_angle *angles, int A, int V, const ctdm_optics *optics,
                                  int iters, unsigned recon_flags, ctdm_complex *ri);
@@ -192,7 +211,7 @@ typedef struct ctdm_cell {
     int V;                      /* volume side = crop side M */
     int iters;                  /* positivity iterations (reference IterTime + 1) */
     unsigned unwrap_flags;      /* CTDM_UNWRAP_* */
-    unsigned recon_flags;       /* CTDM_DC_* | CTDM_POS_NO_SHIFT */
+    unsigned recon_flags;       /* CTDM_DC_* | CTDM_POS_NO_SHIFT | CTDM_MAPS_MEDIAN */
     ctdm_optics optics;
     ctdm_crop crop;             /* crop.M == V; crop.radius < 0: locate from bg[0] (reference frame 1) */
     ctdm_complex *ri;           /* V^3 complex RI output (device) */

@@ -83,17 +83,30 @@ inline void cudaGoldsteinUnwrap2D(const Context& c, const float* cu_phi, float* 
 }
 
 // BatchRecon(float *PhaseStack, float *AmpStack, float *sampleAngleRadX, float *sampleAngleRadY,
-// int ReconFrame) (HilbertTransform.cu:1825): ReconFrame unwrapped-phase and log-amplitude maps of
-// V x V (device), angles on the host, result = octant-shifted complex RI volume (device, V^3) that
-// the reference wrote to a .rimap file (HilbertTransform.cu:1974-2041).
-inline void BatchRecon(const Context& c, const float* PhaseStack, const float* LogAmpStack,
-                       const float* sampleAngleRadX, const float* sampleAngleRadY, int ReconFrame, int V,
-                       const ctdm_optics& optics, int iters, ctdm_complex* ri, unsigned recon_flags = 0)
+// int ReconFrame) (HilbertTransform.cu:1825): ReconFrame unwrapped-phase and amplitude maps of V x V
+// (device), angles on the host, result = octant-shifted complex RI volume (device, V^3) that the
+// reference wrote to a .rimap file (HilbertTransform.cu:1974-2041). As in the reference the maps are
+// 3x3-median filtered and sanitised (MedianFilter, checkMaps) and the amplitude is logged on the device;
+// pass recon_flags without CTDM_MAPS_MEDIAN to skip the filter, or without CTDM_MAPS_AMPLITUDE when the
+// second stack already holds ln A.
+inline void BatchRecon(const Context& c, const float* PhaseStack, const float* AmpStack, const float* sampleAngleRadX,
+                       const float* sampleAngleRadY, int ReconFrame, int V, const ctdm_optics& optics, int iters,
+                       ctdm_complex* ri, unsigned recon_flags = CTDM_MAPS_MEDIAN | CTDM_MAPS_AMPLITUDE)
 {
     std::vector<ctdm_angle> ang(ReconFrame);
     for (int i = 0; i < ReconFrame; ++i) ang[i] = ctdm_angle{sampleAngleRadX[i], sampleAngleRadY[i]};
-    c.check(ctdm_recon_from_maps(c.get(), PhaseStack, LogAmpStack, ang.data(), ReconFrame, V, &optics, iters,
+    c.check(ctdm_recon_from_maps(c.get(), PhaseStack, AmpStack, ang.data(), ReconFrame, V, &optics, iters,
                                  recon_flags, ri));
+}
+
+// MedianFilter(PhaseData, AmpData, Phase_img, Amp_img, nx, ny, nz) (Recon_kernel.cu:7-27) on device
+// stacks: 3x3 median of both stacks into the outputs, then checkMaps on them.
+inline void MedianFilter(const Context& c, float* PhaseData, float* AmpData, const float* Phase_img,
+                         const float* Amp_img, int nx, int ny, int nz)
+{
+    c.check(ctdm_median3x3(c.get(), Phase_img, PhaseData, nz, ny, nx));
+    c.check(ctdm_median3x3(c.get(), Amp_img, AmpData, nz, ny, nx));
+    c.check(ctdm_check_maps(c.get(), PhaseData, AmpData, (size_t)nx * ny * nz));
 }
 
 }  // namespace ctdm_ref

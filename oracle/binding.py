@@ -67,6 +67,10 @@ _EXTRA: dict = {
     "orc_goldstein": [_fp, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, _ip, _fp],
     "orc_fit_pgd": [_dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                     C.c_double, _dp, C.c_uint],
+    "orc_median3x3": [_fp, _fp, C.c_int, C.c_int, C.c_int],
+    "orc_check_maps": [C.c_void_p, C.c_void_p, C.c_long],
+    "orc_recon_from_maps": [_fp, _fp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                            C.c_uint, _dp],
     "orc_reconstruct_cell": [_fp, _fp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                              C.c_double, C.c_double, C.c_double, C.c_int, C.c_uint, C.c_uint, _dp,
                              _dp, _dp, _dp],
@@ -281,3 +285,30 @@ def fit_pgd(f0, spec, angles, wavelength, n_med, dx, iters, step, flags=0):
     lib().orc_fit_pgd(_cp(f), _cp(sp), _p(a, _dp), len(a), f.shape[0], wavelength, n_med, dx, iters, step,
                       _cp(n), flags)
     return n
+
+
+def median3x3(maps):
+    m = np.ascontiguousarray(maps, np.float32)
+    m3 = m.reshape((-1,) + m.shape[-2:])
+    out = np.empty_like(m3)
+    lib().orc_median3x3(_p(m3, _fp), _p(out, _fp), m3.shape[0], m3.shape[1], m3.shape[2])
+    return out.reshape(m.shape)
+
+
+def check_maps(phase=None, amp=None):
+    """Returns sanitised copies (phase, amp) as float32."""
+    ph = None if phase is None else np.ascontiguousarray(phase, np.float32).copy()
+    am = None if amp is None else np.ascontiguousarray(amp, np.float32).copy()
+    n = (ph if ph is not None else am).size
+    lib().orc_check_maps(None if ph is None else ph.ctypes.data, None if am is None else am.ctypes.data, n)
+    return ph, am
+
+
+def recon_from_maps(unwrapped, map2, angles, wavelength, n_med, dx, iters, flags=0):
+    u = np.ascontiguousarray(unwrapped, np.float32)
+    m = np.ascontiguousarray(map2, np.float32)
+    a = _ang(angles)
+    A, V = u.shape[0], u.shape[1]
+    ri = np.zeros((V, V, V), np.complex128)
+    lib().orc_recon_from_maps(_p(u, _fp), _p(m, _fp), _p(a, _dp), A, V, wavelength, n_med, dx, iters, flags, _cp(ri))
+    return ri

@@ -19,6 +19,12 @@
 //   orc_goldstein           hand-enumerated residue / cut / fill cases; integer invariants
 //   orc_ewald_* / recon     single-plane-wave closed form; scatter/gather adjointness; the
 //                           est_n / modify_F identity on feasible n; idempotent positivity
+//   orc_fit_pgd             step 1 == positivity loop; step 0 keeps a feasible start; step 1/2
+//   orc_median3x3           numpy median of every 3x3 window on finite data, borders unchanged;
+//                           hand-enumerated NaN windows for the selection order
+//   orc_check_maps          hand cases of every branch
+//   orc_recon_from_maps     equals the cell reconstruction fed with the same maps; the median
+//                           commutes with the logarithm (amplitude vs ln A inputs)
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -721,6 +727,111 @@ void orc_positivity(double* f, const double* Fm, const double* hits, double* n, 
 // crop = (row, col, radius); angles = (ang_x, ang_y) pairs; unwrap_flags as orc_unwrap_dct;
 // recon_flags as orc_positivity. Optional per-angle outputs (V x V each): wrapped (-pi, pi],
 // unwrapped, logamp.
+}  // extern "C"
+
+namespace {
+
+// MedianFilter_gpu (Recon_kernel.cu:65-119) read out of place: the nine values are gathered with x
+// outer and y inner, and the result is the fifth pick of the reference's selection sort (strict '<',
+// the first minimum kept). Border rows and columns are copied (the reference skips them).
+template <class T>
+void median3x3_t(const T* in, T* out, int B, int H, int W)
+{
+    const long n = (long)H * W;
+    for (int b = 0; b < B; ++b) {
+        const T* m = in + b * n;
+        T* o = out + b * n;
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                if (x == 0 || x == W - 1 || y == 0 || y == H - 1) {
+                    o[x + (long)W * y] = m[x + (long)W * y];
+                    continue;
+                }
+                T v[9];
+                int it = 0;
+                for (int r = x - 1; r <= x + 1; ++r)
+                    for (int c = y - 1; c <= y + 1; ++c) v[it++] = m[r + (long)W * c];
+                for (int i = 0; i < 5; ++i) {
+                    int mn = i;
+                    for (int l = i + 1; l < 9; ++l)
+                        if (v[l] < v[mn]) mn = l;
+                    std::swap(v[i], v[mn]);
+                }
+                o[x + (long)W * y] = v[4];
+            }
+    }
+}
+
+// checkMaps (Recon_kernel.cu:121-142)
+template <class T>
+T check_phase_t(T p) { return std::isfinite(p) ? p : T(0); }
+template <class T>
+T check_amp_t(T a)
+{
+    if (!std::isfinite(a)) return T(1);
+    if (a == T(0)) return T(0.01f);
+    return std::fabs(a);
+}
+
+// From unwrapped phase + ln A maps to the RI volume: cuCopyArray + 2-D FFT (orc_rytov_spectrum),
+// fillEdwardSphere, Initial_F_F2, inverse 3-D FFT and the positivity loop (HilbertTransform.cu:1900-1975).
+void recon_tail(const double* unw, const double* la, const double* angles, int A, int V, double wavelength,
+                double n_med, double dx, int iters, unsigned recon_flags, double* ri)
+{
+    const size_t img = (size_t)V * V, vol = img * V;
+    std::vector<double> spec(2 * A * img);
+    orc_rytov_spectrum(unw, la, A, V, spec.data());
+    std::vector<double> F(2 * vol, 0.0), hits(vol, 0.0), Fm(2 * vol);
+    orc_ewald_scatter(spec.data(), angles, A, V, wavelength, n_med, dx, F.data(), hits.data());
+    orc_normalize(F.data(), hits.data(), Fm.data(), V, dx);
+    orc_fft3d(F.data(), V, +1);
+    orc_positivity(F.data(), Fm.data(), hits.data(), ri, V, iters, wavelength, n_med, recon_flags);
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_median3x3(const float* in, float* out, int B, int H, int W) { median3x3_t(in, out, B, H, W); }
+
+void orc_check_maps(float* phase, float* amp, long n)
+{
+    for (long i = 0; i < n; ++i) {
+        if (phase) phase[i] = check_phase_t(phase[i]);
+        if (amp) amp[i] = check_amp_t(amp[i]);
+    }
+}
+
+// BatchRecon from its map stacks (float, as they arrive on the device). flags 4: 3x3 median of both
+// stacks first (MedianFilter, HilbertTransform.cu:1878); flags 8: the second stack is the amplitude A
+// (checkMaps and the logarithm of cuCopyArray applied here), otherwise it is ln A.
+void orc_recon_from_maps(const float* unw, const float* map2, const double* angles, int A, int V,
+                         double wavelength, double n_med, double dx, int iters, unsigned flags, double* ri)
+{
+    const size_t n = (size_t)A * V * V;
+    const bool median = flags & 4u, amplitude = flags & 8u;
+    std::vector<float> p(unw, unw + n), m(map2, map2 + n);
+    if (median) {
+        std::vector<float> pm(n), mm(n);
+        median3x3_t(p.data(), pm.data(), A, V, V);
+        median3x3_t(m.data(), mm.data(), A, V, V);
+        p.swap(pm);
+        m.swap(mm);
+    }
+    std::vector<double> ph(n), la(n);
+    for (size_t i = 0; i < n; ++i) {
+        ph[i] = (median || amplitude) ? check_phase_t((double)p[i]) : (double)p[i];
+        const double v = m[i];
+        if (amplitude)
+            la[i] = std::log(check_amp_t(v));
+        else if (median)
+            la[i] = (std::isnan(v) || v == INFINITY) ? 0.0 : (v == -INFINITY ? std::log((double)0.01f) : v);
+        else
+            la[i] = v;
+    }
+    recon_tail(ph.data(), la.data(), angles, A, V, wavelength, n_med, dx, iters, flags & 3u, ri);
+}
+
 void orc_reconstruct_cell(const float* holo, const float* bg, const double* angles, int A, int H, int V,
                           int row, int col, int radius, double wavelength, double n_med, double dx, int iters,
                           unsigned unwrap_flags, unsigned recon_flags, double* ri, double* wrapped_out,
@@ -739,14 +850,19 @@ void orc_reconstruct_cell(const float* holo, const float* bg, const double* angl
     orc_unwrap_dct(phase.data(), unw.data(), A, V, unwrap_flags);
     if (unwrapped_out) std::memcpy(unwrapped_out, unw.data(), sizeof(double) * A * img);
     if (logamp_out) std::memcpy(logamp_out, la.data(), sizeof(double) * A * img);
-    std::vector<double> spec(2 * A * img);
-    orc_rytov_spectrum(unw.data(), la.data(), A, V, spec.data());
-    const size_t vol = img * V;
-    std::vector<double> F(2 * vol, 0.0), hits(vol, 0.0), Fm(2 * vol);
-    orc_ewald_scatter(spec.data(), angles, A, V, wavelength, n_med, dx, F.data(), hits.data());
-    orc_normalize(F.data(), hits.data(), Fm.data(), V, dx);
-    orc_fft3d(F.data(), V, +1);
-    orc_positivity(F.data(), Fm.data(), hits.data(), ri, V, iters, wavelength, n_med, recon_flags);
+    if (recon_flags & 4u) {  // BatchRecon's median + checkMaps on the maps it receives
+        std::vector<double> um(A * img), lm(A * img);
+        median3x3_t(unw.data(), um.data(), A, V, V);
+        median3x3_t(la.data(), lm.data(), A, V, V);
+        for (size_t i = 0; i < um.size(); ++i) {
+            um[i] = check_phase_t(um[i]);
+            const double v = lm[i];
+            lm[i] = (std::isnan(v) || v == INFINITY) ? 0.0 : (v == -INFINITY ? std::log((double)0.01f) : v);
+        }
+        unw.swap(um);
+        la.swap(lm);
+    }
+    recon_tail(unw.data(), la.data(), angles, A, V, wavelength, n_med, dx, iters, recon_flags & 3u, ri);
 }
 
 }  // extern "C"

@@ -135,3 +135,59 @@ def test_fit_half_step_moves_halfway(orc):
     fit = orc.fit_pgd(f0, spec, ang, LAM, NM, dx, 1, 0.5)
     pos = orc.positivity(f0, half, hits, 1, LAM, NM)
     np.testing.assert_allclose(fit, pos, rtol=0, atol=1e-10)
+
+
+def _windows_median(m):
+    """Median of every interior 3x3 window by numpy (odd count: the exact middle element)."""
+    H, W = m.shape
+    out = m.copy()
+    st = np.stack([m[1 + dy:H - 1 + dy, 1 + dx:W - 1 + dx] for dy in (-1, 0, 1) for dx in (-1, 0, 1)])
+    out[1:-1, 1:-1] = np.median(st, axis=0)
+    return out
+
+
+def test_median_matches_numpy_windows(orc):
+    rng = np.random.default_rng(6)
+    maps = rng.standard_normal((3, 17, 23)).astype(np.float32)
+    maps[1, 5:9, 5:9] = 0.25  # ties
+    got = orc.median3x3(maps)
+    for b in range(3):
+        np.testing.assert_array_equal(got[b], _windows_median(maps[b]))
+
+
+def test_median_nan_windows_follow_selection_order(orc):
+    """The reference's selection sort never moves a NaN it meets at the current slot and never picks a
+    NaN as a minimum, so the fifth pick depends on where the NaN sits in the column-major gather."""
+    w = np.arange(9, dtype=np.float32).reshape(3, 3).T.copy()  # gather order k = 3x + y holds value k
+    a = w.copy()
+    a[0, 0] = np.nan  # k = 0: slot 0 keeps the NaN, slots 1..4 take 1..4 -> 4
+    assert orc.median3x3(a)[1, 1] == 4.0
+    b = w.copy()
+    b[1, 1] = np.nan  # k = 4 (centre): slots 0..3 take 0..3, slot 4 keeps the NaN -> NaN
+    assert np.isnan(orc.median3x3(b)[1, 1])
+    c = w.copy()
+    c[2, 2] = np.nan  # k = 8: never reached by a minimum -> the ordinary median of 0..7, i.e. 4
+    assert orc.median3x3(c)[1, 1] == 4.0
+
+
+def test_check_maps_branches(orc):
+    ph, am = orc.check_maps(np.array([np.nan, np.inf, -np.inf, 1.5, -2.0]),
+                            np.array([np.nan, np.inf, 0.0, -3.0, 2.0]))
+    np.testing.assert_array_equal(ph, np.array([0, 0, 0, 1.5, -2.0], np.float32))
+    np.testing.assert_array_equal(am, np.array([1, 1, 0.01, 3.0, 2.0], np.float32))
+
+
+def test_recon_from_maps_amplitude_and_log_agree(orc):
+    """Median commutes with the logarithm: A maps with the amplitude flag give the RI of ln A maps;
+    without the median the maps reconstruct exactly as the cell path's tail."""
+    V, A = 16, 4
+    rng = np.random.default_rng(7)
+    ph = (0.3 * rng.standard_normal((A, V, V))).astype(np.float32)
+    la = (0.05 * rng.standard_normal((A, V, V))).astype(np.float32)
+    ang = _angles(A)
+    dx = 0.4e-6 * 256 / V
+    r_log = orc.recon_from_maps(ph, la, ang, LAM, NM, dx, 2, flags=4)
+    r_amp = orc.recon_from_maps(ph, np.exp(la.astype(np.float64)).astype(np.float32), ang, LAM, NM, dx, 2, flags=12)
+    np.testing.assert_allclose(r_amp, r_log, rtol=0, atol=1e-6)
+    r_med = orc.recon_from_maps(orc.median3x3(ph), orc.median3x3(la), ang, LAM, NM, dx, 2, flags=0)
+    np.testing.assert_allclose(r_med, r_log, rtol=0, atol=1e-12)
