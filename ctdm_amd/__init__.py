@@ -109,6 +109,7 @@ SIGNATURES = {
     "ctdm_reconstruct": ([_vp, C.POINTER(Cell), C.c_int, C.POINTER(BatchOpts)], C.c_int),
     "ctdm_autotune": ([_vp, C.POINTER(C.c_int), C.c_int, C.c_char_p, C.c_size_t], C.c_int),
     "ctdm_time_pass": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+    "ctdm_time_loop": ([_vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "ctdm_fit_pgd": ([_vp, _vp, _vp, C.POINTER(Angle), C.c_int, C.c_int, C.POINTER(Optics), C.c_int, C.c_double, _vp, C.c_uint], C.c_int),
     "ctdm_synth_default_params": ([C.POINTER(SynthParams)], None),
     "ctdm_synth_cell": ([C.c_uint64, C.c_int, C.POINTER(SynthParams), _vp, _vp, C.POINTER(Angle)], C.c_int),
@@ -386,6 +387,12 @@ class Context:
         ms, nb = C.c_double(), C.c_double()
         self._call("ctdm_time_pass", pass_kind, V, pencils, reps, C.byref(ms), C.byref(nb))
         return ms.value, nb.value
+
+    def time_loop(self, V: int, iters: int = 10):
+        """Per-pass average ms and algorithmic bytes ([x, y, z+dc]) measured inside the loop sequence."""
+        ms, nb = (C.c_double * 3)(), (C.c_double * 3)()
+        self._call("ctdm_time_loop", V, iters, ms, nb)
+        return list(ms), list(nb)
 
     def autotune(self, sizes) -> str:
         s = (C.c_int * len(sizes))(*sizes)

@@ -296,6 +296,11 @@ ctdm_status ctdm_time_pass(ctdm_ctx* ctx, int pass, int V, int pencils, int reps
     });
 }
 
+ctdm_status ctdm_time_loop(ctdm_ctx* ctx, int V, int iters, double* avg_ms, double* bytes)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) { ctdm::time_loop(c, V, iters, avg_ms, bytes); });
+}
+
 ctdm_status ctdm_fit_pgd(ctdm_ctx* ctx, ctdm_complex* f, const ctdm_complex* spec,
                          const ctdm_angle* angles, int A, int V, const ctdm_optics* optics, int iters,
                          double step, ctdm_complex* n, unsigned flags)

@@ -72,6 +72,7 @@ struct GraphEntry {
 };
 
 struct Ctx {
+    hipEvent_t tev[2] = {nullptr, nullptr};  // when set: recorded at start and end of the next volume pass
     int device = 0;
     hipStream_t stream = nullptr;
     std::string last_error;
@@ -151,6 +152,7 @@ void fit_pgd(Ctx& c, float2* f, const float2* spec, const ctdm_angle* ang, int A
 
 // Average ms of one launch of volume pass `kind` (0 x, 1 y, 2 z+dc) with B pencils per workgroup.
 double time_pass(Ctx& c, int kind, int V, int B, int reps, double* bytes);
+void time_loop(Ctx& c, int V, int iters, double ms[3], double bytes[3]);
 
 // Pipeline (pipeline.cpp)
 void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& opts);

@@ -22,6 +22,8 @@
 #include "fft_dev.hpp"
 #include "internal.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <cmath>
 
 namespace ctdm {
@@ -564,7 +566,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
     const int pb = threadIdx.x / T, k0 = threadIdx.x % T;
     const int blk = Tile<V, 2, B>::block();
     const long long dpencil = (long long)(blk % (V / B)) * B + pb + (long long)V * (blk / (V / B));
-    const int n = dc.cnt[dpencil];
+    const int n = min(dc.cnt[dpencil], dc.cap);  // a count never exceeds the list capacity
     short pz[DC_PRE];
     float2 pv[DC_PRE];
 #pragma unroll
@@ -664,7 +666,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__
     Plan<V>::template run<-1>(tl.t, s, twp, gl, gs);
     __syncthreads();
     const long long pencil = tl.base;  // x + V*y
-    const int n = ml.cnt[pencil];
+    const int n = min(ml.cnt[pencil], ml.cap);
     const int z0 = tl.t * ZR;
     for (int e0 = 0; e0 < n; e0 += GRID_CHUNK) {
         const int ne = min(GRID_CHUNK, n - e0);
@@ -844,6 +846,19 @@ static DcList dc_alloc(Ctx& c, int V, int cap)
 }
 
 // Launchers of the fused passes for one pencils-per-workgroup choice B.
+// Launches a volume pass. When the context holds a pair of timing events they are attached to this
+// dispatch (recorded as it starts and as it ends, no packets in between) and consumed.
+template <class K, class... Args>
+static void vlaunch(Ctx& c, K kernel, dim3 g, dim3 b, Args... args)
+{
+    if (c.tev[0]) {
+        hipExtLaunchKernelGGL(kernel, g, b, 0, c.stream, c.tev[0], c.tev[1], 0, args...);
+        c.tev[0] = c.tev[1] = nullptr;
+    } else {
+        hipLaunchKernelGGL(kernel, g, b, 0, c.stream, args...);
+    }
+}
+
 template <int V, int B>
 struct VolOps {
     static constexpr int NT = B * Plan<V>::T;
@@ -852,25 +867,28 @@ struct VolOps {
     static void x(Ctx& c, int mode, float2* W, float2* out, const Proj& p, float scale, bool shift)
     {
         const float2* tw = c.tw(V);
+        const int sh = shift;
         switch (mode) {
-        case 0: hipLaunchKernelGGL((k_vol_x<V, B, 0>), grid(), dim3(NT), 0, c.stream, W, out, tw, p, scale, (int)shift); break;
-        case 1: hipLaunchKernelGGL((k_vol_x<V, B, 1>), grid(), dim3(NT), 0, c.stream, W, out, tw, p, scale, (int)shift); break;
-        case 2: hipLaunchKernelGGL((k_vol_x<V, B, 2>), grid(), dim3(NT), 0, c.stream, W, out, tw, p, scale, (int)shift); break;
-        default: hipLaunchKernelGGL((k_vol_x<V, B, 3>), grid(), dim3(NT), 0, c.stream, W, out, tw, p, scale, (int)shift); break;
+        case 0: vlaunch(c, &k_vol_x<V, B, 0>, grid(), dim3(NT), W, out, tw, p, scale, sh); break;
+        case 1: vlaunch(c, &k_vol_x<V, B, 1>, grid(), dim3(NT), W, out, tw, p, scale, sh); break;
+        case 2: vlaunch(c, &k_vol_x<V, B, 2>, grid(), dim3(NT), W, out, tw, p, scale, sh); break;
+        default: vlaunch(c, &k_vol_x<V, B, 3>, grid(), dim3(NT), W, out, tw, p, scale, sh); break;
         }
         CTDM_CHECK_LAUNCH("k_vol_x");
     }
     static void y(Ctx& c, float2* W, int dir)
     {
+        const float2* tw = c.tw(V);
         if (dir < 0)
-            hipLaunchKernelGGL((k_vol_y<V, B, -1>), grid(), dim3(NT), 0, c.stream, W, c.tw(V));
+            vlaunch(c, &k_vol_y<V, B, -1>, grid(), dim3(NT), W, tw);
         else
-            hipLaunchKernelGGL((k_vol_y<V, B, +1>), grid(), dim3(NT), 0, c.stream, W, c.tw(V));
+            vlaunch(c, &k_vol_y<V, B, +1>, grid(), dim3(NT), W, tw);
         CTDM_CHECK_LAUNCH("k_vol_y");
     }
     static void zdc(Ctx& c, float2* W, const DcList& dc)
     {
-        hipLaunchKernelGGL((k_vol_zdc<V, B>), grid(), dim3(NT), 0, c.stream, W, c.tw(V), dc);
+        const float2* tw = c.tw(V);
+        vlaunch(c, &k_vol_zdc<V, B>, grid(), dim3(NT), W, tw, dc);
         CTDM_CHECK_LAUNCH("k_vol_zdc");
     }
     static void zpgd(Ctx& c, float2* W, const MeasList& ml, float tau)
@@ -1114,6 +1132,96 @@ static double time_pass_v(Ctx& c, int kind, int B, int reps, double* bytes)
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     return ms / reps;
+}
+
+// The positivity loop's own sequence (x, forward y, z + data consistency, inverse y) run `iters` times
+// on the context's volume, each launch carrying its own start/end events (hipExtLaunchKernel): per-pass
+// average durations in the context they run in (each pass reads what the previous one wrote), with the
+// tuned choices.
+// Stand-in data-consistency list for timing before any reconstruction at this V: `per` entries per
+// pencil spread evenly over z (a 60-angle cell lands 20-30 samples on a typical pencil).
+__global__ void k_dc_synth(DcList dc, int V, int per)
+{
+    const long long VV = (long long)V * V;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= VV) return;
+    for (int k = 0; k < per; ++k) {
+        dc.idx[p * dc.cap + k] = (short)((k * V) / per);
+        dc.val[p * dc.cap + k] = make_float2(1e-3f, -1e-3f);
+    }
+    dc.cnt[p] = per;
+}
+
+template <int V>
+static void time_loop_v(Ctx& c, int iters, double ms[3], double bytes[3])
+{
+    const size_t vol = (size_t)V * V * V, VV = (size_t)V * V;
+    const std::string wname = "vol.W";
+    const bool have = c.dc_V == V && c.scratch.count(wname) && c.scratch[wname].bytes >= sizeof(float2) * vol;
+    const int per = std::min(24, V / 2);
+    const int cap = have ? std::max(1, c.dc_cap) : per;
+    float2* W = (float2*)c.buf(wname, sizeof(float2) * vol);
+    const DcList dc = dc_alloc(c, V, cap);
+    if (!have) {
+        fill(c, W, 2 * (long long)vol, 1.f);
+        hipLaunchKernelGGL(k_dc_synth, dim3(grid1d((long long)VV, 256)), dim3(256), 0, c.stream, dc, V, per);
+        CTDM_CHECK_LAUNCH("k_dc_synth");
+    }
+    float2* nbuf = (float2*)c.buf("tp.n", sizeof(float2) * vol);
+    const Proj p = make_proj(ewald_geom(ctdm_optics{532e-9, 1.337, 0.4e-6}, V));
+    std::vector<int> h(VV);
+    CTDM_HIP(hipMemcpy(h.data(), dc.cnt, sizeof(int) * VV, hipMemcpyDeviceToHost));
+    double nent = 0;
+    for (int v : h) nent += v;
+    const double vb = 2.0 * sizeof(float2) * vol;
+    bytes[0] = vb;
+    bytes[1] = vb;
+    bytes[2] = vb + sizeof(int) * (double)VV + nent * (sizeof(float2) + sizeof(short));
+    const int nk = 4 * iters;
+    std::vector<hipEvent_t> ev(2 * nk);
+    for (auto& e : ev) CTDM_HIP(hipEventCreate(&e));
+    Vol<V>::x(c, 1, W, nbuf, p, 1.f / (float)vol, true);  // warm-up of the sequence
+    auto arm = [&](int k) {
+        c.tev[0] = ev[2 * k];
+        c.tev[1] = ev[2 * k + 1];
+    };
+    for (int it = 0; it < iters; ++it) {
+        arm(4 * it);
+        Vol<V>::x(c, 1, W, nbuf, p, 1.f / (float)vol, true);
+        arm(4 * it + 1);
+        Vol<V>::y(c, W, -1);
+        arm(4 * it + 2);
+        Vol<V>::zdc(c, W, dc);
+        arm(4 * it + 3);
+        Vol<V>::y(c, W, +1);
+    }
+    c.tev[0] = c.tev[1] = nullptr;
+    CTDM_HIP(hipEventSynchronize(ev[2 * nk - 1]));
+    double acc[3] = {0, 0, 0};
+    for (int k = 0; k < nk; ++k) {
+        float t = 0;
+        CTDM_HIP(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
+        const int kind = k % 4 == 0 ? 0 : (k % 4 == 2 ? 2 : 1);
+        acc[kind] += t;
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    ms[0] = acc[0] / iters;
+    ms[1] = acc[1] / (2.0 * iters);
+    ms[2] = acc[2] / iters;
+}
+
+void time_loop(Ctx& c, int V, int iters, double ms[3], double bytes[3])
+{
+    CTDM_REQUIRE(!c.capturing, "time_loop during capture");
+    CTDM_REQUIRE(iters > 0 && ms && bytes, "time_loop: bad arguments");
+    switch (V) {
+    case 32: time_loop_v<32>(c, iters, ms, bytes); break;
+    case 64: time_loop_v<64>(c, iters, ms, bytes); break;
+    case 128: time_loop_v<128>(c, iters, ms, bytes); break;
+    case 192: time_loop_v<192>(c, iters, ms, bytes); break;
+    case 256: time_loop_v<256>(c, iters, ms, bytes); break;
+    default: fail(CTDM_EUNSUPPORTED, "time_loop: volume side has no kernel");
+    }
 }
 
 double time_pass(Ctx& c, int kind, int V, int B, int reps, double* bytes)

@@ -247,6 +247,15 @@ ctdm_status ctdm_autotune(ctdm_ctx *ctx, const int *sizes, int nsizes, char *rep
 ctdm_status ctdm_time_pass(ctdm_ctx *ctx, int pass, int V, int pencils, int reps, double *avg_ms,
                            double *bytes);
 
+/* Runs `iters` iterations of the fused positivity loop's pass sequence (x with projection, forward y,
+ * z with data consistency, inverse y) on the context's volume of side V, each launch carrying its own
+ * start and end events (hipExtLaunchKernel), and returns the average duration [ms] of each pass in that
+ * sequence
+ * (avg_ms[0] x, [1] y, [2] z + data consistency; host arrays of 3) and its algorithmic bytes per
+ * launch (bytes[3]). Uses the data-consistency list of the last reconstruction at this V, when there is
+ * one. No reference counterpart: the per-kernel measurement of the loop as it runs. Synchronises. */
+ctdm_status ctdm_time_loop(ctdm_ctx *ctx, int V, int iters, double *avg_ms, double *bytes);
+
 /* ---------------------------------------------------------------- differentiable fit */
 /* Projected-gradient fit of measured Rytov spectra (no reference counterpart; an alternative to the
  * data-consistency loop of ctdm_positivity). Minimises
