@@ -13,8 +13,10 @@ pytestmark = pytest.mark.gpu
 
 
 def _check(ctx, orc, w):
+    wd = torch.from_numpy(w).cuda()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res, cut, k, un = ctx.goldstein(torch.from_numpy(w).cuda())
+    res, cut, k, un = ctx.goldstein(wd)
     torch.cuda.synchronize()
     tg = time.perf_counter() - t0
     t0 = time.perf_counter()
@@ -56,3 +58,10 @@ def test_goldstein_config3_bit_exact(ctx, orc):
     dens = float((rr != 0).mean())
     print(f"config3: GPU {tg * 1e3:.1f} ms, oracle {tc:.2f} s, residue density {dens:.3f}")
     assert dens > 0.01
+
+
+def test_goldstein_large_map_global_path(ctx, orc):
+    """1024^2 maps exceed the LDS-resident kernels' budget and take the global-memory variants."""
+    w = ctdm_amd.synth_wrapped(5, 2, 1024, 1024)
+    tg, tc, rr = _check(ctx, orc, w)
+    print(f"2 x 1024^2: GPU {tg * 1e3:.1f} ms, oracle {tc:.2f} s")
