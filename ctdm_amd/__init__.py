@@ -33,6 +33,7 @@ DC_REFCOMPAT = 0x1
 POS_NO_SHIFT = 0x2
 MAPS_MEDIAN = 0x4
 MAPS_AMPLITUDE = 0x8
+RETRIEVE_ROWS = 0x10
 EXEC_PLAIN = 0x100
 
 
@@ -94,6 +95,8 @@ SIGNATURES = {
     "ctdm_version": ([], C.c_char_p),
     "ctdm_find_sideband": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Crop)], C.c_int),
     "ctdm_retrieve": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(Crop), _vp, _vp, _vp], C.c_int),
+    "ctdm_find_sideband_rows": ([_vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "ctdm_retrieve_rows": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "ctdm_unwrap_dct": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_uint], C.c_int),
     "ctdm_unwrap_dct_adjoint": ([_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_uint], C.c_int),
     "ctdm_goldstein": ([_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp], C.c_int),
@@ -249,6 +252,23 @@ class Context:
         crop = Crop()
         self._call("ctdm_find_sideband", _ptr(bg), H, W, M, row_start, C.byref(crop))
         return crop
+
+    def find_sideband_rows(self, bg) -> int:
+        H, W = bg.shape[-2:]
+        c = C.c_int()
+        self._call("ctdm_find_sideband_rows", _ptr(bg), H, W, C.byref(c))
+        return c.value
+
+    def retrieve_rows(self, holo, bg, center: int):
+        """Algorithm B maps, (batch, W/4, H/4) indexed [n][y'] (the reference's transposed layout)."""
+        torch = self._torch
+        if holo.dim() == 2:
+            holo, bg = holo[None], bg[None]
+        B, H, W = holo.shape
+        wrapped = torch.empty((B, W // 4, H // 4), dtype=torch.float32, device=holo.device)
+        logamp = torch.empty_like(wrapped)
+        self._call("ctdm_retrieve_rows", _ptr(holo), _ptr(bg), B, H, W, center, _ptr(wrapped), _ptr(logamp))
+        return wrapped, logamp
 
     def retrieve(self, holo, bg, crop: Crop, field: bool = False):
         torch = self._torch

@@ -160,6 +160,22 @@ ctdm_status ctdm_retrieve(ctdm_ctx* ctx, const float* holo, const float* bg, int
     });
 }
 
+ctdm_status ctdm_find_sideband_rows(ctdm_ctx* ctx, const float* bg, int H, int W, int* center)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        CTDM_REQUIRE(center, "find_sideband_rows: null output");
+        *center = ctdm::find_sideband_rows(c, bg, H, W);
+    });
+}
+
+ctdm_status ctdm_retrieve_rows(ctdm_ctx* ctx, const float* holo, const float* bg, int batch, int H, int W, int center,
+                               float* wrapped, float* logamp)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        ctdm::retrieve_rows(c, holo, bg, batch, H, W, center, wrapped, logamp);
+    });
+}
+
 ctdm_status ctdm_unwrap_dct(ctdm_ctx* ctx, const float* wrapped, float* unwrapped, int batch, int M,
                             unsigned flags)
 {

@@ -4,7 +4,8 @@
 // and keeps the faster choice in the context (used by every later launch of that side, including graph
 // captures made afterwards). An explicit call re-times every side; a reconstruction asking for
 // tuning only times sides not tuned yet. Radix plans are fixed per length (fft_dev.hpp); the gridding pass
-// keeps 16 pencils (it is timed only inside a full reconstruction).
+// keeps 8 pencils (its LDS staging of every angle's samples limits it to two 16-pencil workgroups per CU;
+// it is timed only inside a full reconstruction).
 #include <cstdio>
 #include <sstream>
 

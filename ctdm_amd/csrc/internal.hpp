@@ -56,7 +56,7 @@ struct Ctx;
 
 // Per-size kernel choices picked by the autotuner (defaults are the untuned picks).
 struct Tune {
-    int pencils[4] = {16, 16, 16, 16};  // pencils per workgroup of the x, y, z+dc and gridding passes
+    int pencils[4] = {16, 16, 16, 8};  // pencils per workgroup of the x, y, z+dc and gridding passes
     bool tuned = false;
     double ms[4][2] = {};               // measured ms per launch for B = 8, 16 (after tuning)
 };
@@ -95,6 +95,9 @@ struct Ctx {
 // ---------------------------------------------------------------- module launchers
 // Retrieval (k_retrieve.hip)
 void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, ctdm_crop* out);
+int find_sideband_rows(Ctx& c, const float* bg, int H, int W);
+void retrieve_rows(Ctx& c, const float* holo, const float* bg, int batch, int H, int W, int center, float* wrapped,
+                   float* logamp);
 void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int H, int W,
                       const ctdm_crop& crop, float* wrapped, float* logamp, float2* field);
 // Reference-structured retrieval of one hologram pair (full 2-D FFTs, shift, crop, inverse,

@@ -19,6 +19,7 @@
 #include "internal.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace ctdm {
 
@@ -316,6 +317,227 @@ void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, 
     out->col = (int)(idx % W);
     out->radius = std::abs(out->row - H / 2) / 3;
     out->M = M;
+}
+
+// ---------------------------------------------------------------- row-wise retrieval (Algorithm B)
+// extractQPI_AlgorithmB (HilbertTransform.cu:1587-1605) + sequence1DFFT (:1608-1665) + calcWrapPhase:
+// each hologram is resampled to H/4 rows by bilinear_interpolation_kernel (:3658-3718; x positions
+// i (W-1)/W, y positions j (H-1)/(H/4)), every resampled row is transformed along x, the W/4 bins
+// around the sideband column are kept, and an inverse W/4-point transform along x gives the field.
+// The reference stores the field transposed (copySharedMem :1704-1726), so map index y' + (H/4) n.
+//
+// Fused here per workgroup: G resampled rows of one pair (SP and BG), resampling in the load, the
+// real row transform as a W/2-point complex transform + split, the band cut, the inverse W/4 transform
+// and the phase / log-amplitude combination. With M = W/4 and signed offsets d in [-M/2, M/2), the
+// inverse transform's input after the reference's crop and half swap is X[(center + W/2 + d) mod W]
+// at position d mod M.
+
+// bilinear_interpolation_kernel's value at output (i, row j) of an H x W image, the reference's
+// expression in float.
+__device__ __forceinline__ float resample_b(const float* __restrict__ img, int H, int W, int i, int iy, float b)
+{
+    const float xr = (float)(W - 1) / (float)W;
+    const int ix = (int)(xr * (float)i);
+    const float a = xr * (float)i - (float)ix;
+    const float* r0 = img + (long long)iy * W;
+    const float d00 = r0[ix];
+    const float d10 = ix + 1 < W ? r0[ix + 1] : 0.f;
+    const float d01 = iy + 1 < H ? r0[W + ix] : 0.f;
+    const float d11 = (ix + 1 < W && iy + 1 < H) ? r0[W + ix + 1] : 0.f;
+    const float t1 = a * d10 + (-d00 * a + d00);
+    const float t2 = a * d11 + (-d01 * a + d01);
+    return b * t2 + (-t1 * b + t1);
+}
+
+// Bin k of the real-input W-point DFT from Z = DFT_{W/2}(x[2m] + i x[2m+1]) held in LDS (padded).
+template <int W>
+__device__ __forceinline__ float2 real_bin(const float2* s, int k, const float2* __restrict__ tw_full)
+{
+    constexpr int NH = W / 2;
+    const int ka = k % NH;
+    const int kb = (NH - ka) % NH;
+    const float2 za = s[lds_pad(ka)], zb = s[lds_pad(kb)];
+    const float2 e = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
+    const float2 o = make_float2(0.5f * (za.y + zb.y), -0.5f * (za.x - zb.x));
+    return cadd(e, cmul(tw_full[k], o));
+}
+
+template <int W, int G>
+__global__ __launch_bounds__(2 * G * Plan<W / 2>::T) void k_rows_b(
+    const float* __restrict__ holo, const float* __restrict__ bg, int H, int center,
+    const float2* __restrict__ tw_half, const float2* __restrict__ tw_full, const float2* __restrict__ tw_m,
+    float* __restrict__ wrapped, float* __restrict__ logamp)
+{
+    constexpr int NH = W / 2, T = Plan<NH>::T, M = W / 4;
+    constexpr int LS = LdsSize<NH>::value;
+    __shared__ float2 lds[2 * G * LS];
+    const int N2 = H / 4;
+    const int tr = threadIdx.x / T, t = threadIdx.x % T;  // transform: SP rows then BG rows
+    const int which = tr / G, r = tr % G;
+    const int tiles = N2 / G;
+    const int pair = blockIdx.x / tiles;
+    const int yq = (blockIdx.x % tiles) * G + r;  // resampled row y'
+    const float* img = (which ? bg : holo) + (long long)pair * H * W;
+    const float yr = (float)(H - 1) / (float)N2;
+    const int iy = (int)(yr * (float)yq);
+    const float bfr = yr * (float)yq - (float)iy;
+    float2* s = lds + tr * LS;
+    auto gl = [&](int m) -> float2 {
+        return make_float2(resample_b(img, H, W, 2 * m, iy, bfr), resample_b(img, H, W, 2 * m + 1, iy, bfr));
+    };
+    auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
+    Plan<NH>::template run<-1>(t, s, tw_half, gl, gs);
+    __syncthreads();
+    // the band: M bins into registers, then back into this transform's LDS area
+    constexpr int PB = (M + T - 1) / T;
+    float2 band[PB];
+#pragma unroll
+    for (int p = 0; p < PB; ++p) {
+        const int x = t + p * T;
+        if (x < M) {
+            const int d = x < M / 2 ? x : x - M;
+            int k = (center + W / 2 + d) % W;
+            if (k < 0) k += W;
+            band[p] = real_bin<W>(s, k, tw_full);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PB; ++p) {
+        const int x = t + p * T;
+        if (x < M) s[lds_pad(x)] = band[p];
+    }
+    __syncthreads();
+    auto gl2 = [&](int i) -> float2 { return s[lds_pad(i)]; };
+    auto gs2 = [&](int n, float2 v) { s[lds_pad(n)] = v; };
+    Plan<M>::template run<+1>(t, s, tw_m, gl2, gs2);
+    __syncthreads();
+    // combine: lanes walk y' fastest (contiguous in the transposed map)
+    const int y0 = (blockIdx.x % tiles) * G;
+    for (int e = threadIdx.x; e < G * M; e += 2 * G * T) {
+        const int rr = e % G, n = e / G;
+        const float2 sp = lds[rr * LS + lds_pad(n)];
+        const float2 bv = lds[(G + rr) * LS + lds_pad(n)];
+        const float2 z = cmulc(sp, bv);
+        float ph = atan2f(z.y, z.x);
+        float la = 0.5f * (logf(sp.x * sp.x + sp.y * sp.y) - logf(bv.x * bv.x + bv.y * bv.y));
+        if (!isfinite(ph)) ph = 0.f;
+        if (!isfinite(la)) la = 0.f;
+        const long long o = (long long)pair * M * N2 + (long long)(y0 + rr) + (long long)N2 * n;
+        wrapped[o] = ph;
+        logamp[o] = la;
+    }
+}
+
+// HistogramFT (HilbertTransform.cu:1728-1748) input: |X| of every shifted column of every resampled row
+// of one background, mag[y'][s].
+template <int W, int G>
+__global__ __launch_bounds__(G * Plan<W / 2>::T) void k_rows_mag(const float* __restrict__ img, int H,
+                                                                 const float2* __restrict__ tw_half,
+                                                                 const float2* __restrict__ tw_full,
+                                                                 float* __restrict__ mag)
+{
+    constexpr int NH = W / 2, T = Plan<NH>::T, LS = LdsSize<NH>::value;
+    __shared__ float2 lds[G * LS];
+    const int N2 = H / 4;
+    const int tr = threadIdx.x / T, t = threadIdx.x % T;
+    const int yq = blockIdx.x * G + tr;
+    const bool valid = yq < N2;
+    const float yr = (float)(H - 1) / (float)N2;
+    const int iy = valid ? (int)(yr * (float)yq) : 0;
+    const float bfr = valid ? yr * (float)yq - (float)iy : 0.f;
+    float2* s = lds + tr * LS;
+    auto gl = [&](int m) -> float2 {
+        return make_float2(resample_b(img, H, W, 2 * m, iy, bfr), resample_b(img, H, W, 2 * m + 1, iy, bfr));
+    };
+    auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
+    Plan<NH>::template run<-1>(t, s, tw_half, gl, gs);
+    __syncthreads();
+    if (!valid) return;
+    for (int sc = t; sc < W; sc += T) {
+        const float2 v = real_bin<W>(s, (sc + W / 2) % W, tw_full);
+        mag[(long long)yq * W + sc] = sqrtf(v.x * v.x + v.y * v.y);
+    }
+}
+
+// Column sums over the resampled rows in row order, then the first maximum over [lo, W) as a 64-bit key
+// (value bits, inverted column), as thrust::max_element scans.
+__global__ void k_rows_hist_argmax(const float* __restrict__ mag, int N2, int W, int lo,
+                                   unsigned long long* __restrict__ best)
+{
+    const int sc = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long key = 0;
+    if (sc < W) {
+        float sum = 0.f;
+        for (int j = 0; j < N2; ++j) sum += mag[(long long)j * W + sc];
+        key = ((unsigned long long)__float_as_uint(sum) << 32) | (0xffffffffull - (unsigned long long)sc);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(key, off);
+        key = o > key ? o : key;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(best, key);
+}
+
+static void check_rows_geometry(int H, int W)
+{
+    CTDM_REQUIRE(H == W, "row-wise retrieval: square holograms required");
+    CTDM_REQUIRE(W == 256 || W == 512 || W == 1024, "row-wise retrieval: holograms of 256, 512 or 1024");
+}
+
+int find_sideband_rows(Ctx& c, const float* bg, int H, int W)
+{
+    CTDM_REQUIRE(bg, "find_sideband_rows: null pointer");
+    check_rows_geometry(H, W);
+    const int N2 = H / 4;
+    float* mag = (float*)c.buf("sbr.mag", sizeof(float) * (size_t)N2 * W);
+    unsigned long long* best = (unsigned long long*)c.buf("sbr.best", sizeof(unsigned long long));
+    auto launch = [&](auto wtag) {
+        constexpr int Wc = decltype(wtag)::value;
+        constexpr int G = 256 / Plan<Wc / 2>::T;
+        hipLaunchKernelGGL((k_rows_mag<Wc, G>), dim3((N2 + G - 1) / G), dim3(G * Plan<Wc / 2>::T), 0, c.stream, bg,
+                           H, c.tw(Wc / 2), c.tw(Wc), mag);
+    };
+    switch (W) {
+    case 256: launch(std::integral_constant<int, 256>{}); break;
+    case 512: launch(std::integral_constant<int, 512>{}); break;
+    default: launch(std::integral_constant<int, 1024>{}); break;
+    }
+    CTDM_CHECK_LAUNCH("k_rows_mag");
+    const int lo = (int)(W * 0.6);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c.stream, best);
+    hipLaunchKernelGGL(k_rows_hist_argmax, dim3((W - lo + 255) / 256), dim3(256), 0, c.stream, mag, N2, W, lo, best);
+    CTDM_CHECK_LAUNCH("k_rows_hist_argmax");
+    unsigned long long h = 0;
+    CTDM_HIP(hipMemcpyAsync(&h, best, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    CTDM_HIP(hipStreamSynchronize(c.stream));
+    int idx = (int)(0xffffffffull - (h & 0xffffffffull));
+    if (W - idx < (W / 4) / 2) idx = W - idx;  // HilbertTransform.cu:1641
+    return idx;
+}
+
+void retrieve_rows(Ctx& c, const float* holo, const float* bg, int batch, int H, int W, int center, float* wrapped,
+                   float* logamp)
+{
+    CTDM_REQUIRE(holo && bg && wrapped && logamp, "retrieve_rows: null pointer");
+    CTDM_REQUIRE(batch >= 0, "retrieve_rows: negative batch");
+    check_rows_geometry(H, W);
+    CTDM_REQUIRE(center - W / 8 >= 0 && center + W / 8 <= W,
+                 "retrieve_rows: the W/4 band around the sideband column leaves the spectrum");
+    if (batch == 0) return;
+    auto launch = [&](auto wtag) {
+        constexpr int Wc = decltype(wtag)::value;
+        constexpr int G = 128 / Plan<Wc / 2>::T;
+        const int N2 = H / 4;
+        hipLaunchKernelGGL((k_rows_b<Wc, G>), dim3(batch * (N2 / G)), dim3(2 * G * Plan<Wc / 2>::T), 0, c.stream,
+                           holo, bg, H, center, c.tw(Wc / 2), c.tw(Wc), c.tw(Wc / 4), wrapped, logamp);
+    };
+    switch (W) {
+    case 256: launch(std::integral_constant<int, 256>{}); break;
+    case 512: launch(std::integral_constant<int, 512>{}); break;
+    default: launch(std::integral_constant<int, 1024>{}); break;
+    }
+    CTDM_CHECK_LAUNCH("k_rows_b");
 }
 
 // ---------------------------------------------------------------- plain (reference-structured) path

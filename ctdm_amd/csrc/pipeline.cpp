@@ -28,12 +28,18 @@ static void validate_cell(const ctdm_cell& cl)
                  "cell: volume side has no kernel");
     CTDM_REQUIRE(cl.iters >= 0, "cell: negative iteration count");
     CTDM_REQUIRE(cl.crop.radius < 0 || cl.crop.M == cl.V, "cell: crop side must equal the volume side");
+    if (cl.recon_flags & CTDM_RETRIEVE_ROWS)
+        CTDM_REQUIRE(cl.H == cl.W && cl.V * 4 == cl.W, "cell: row-wise retrieval needs square holograms and V = W / 4");
     CTDM_REQUIRE(cl.optics.wavelength > 0 && cl.optics.n_med > 0 && cl.optics.dx > 0, "cell: bad optics");
 }
 
 static ctdm_crop cell_crop(Ctx& c, const ctdm_cell& cl)
 {
     if (cl.crop.radius >= 0) return cl.crop;
+    if (cl.recon_flags & CTDM_RETRIEVE_ROWS) {  // sideband column located on frame 1 (HilbertTransform.cu:1634)
+        const int center = find_sideband_rows(c, cl.bg, cl.H, cl.W);
+        return ctdm_crop{0, center, cl.W / 8, cl.V};
+    }
     ctdm_crop cr{};
     find_sideband(c, cl.bg, cl.H, cl.W, cl.V, 0, &cr);  // frame 1 of the reference (HilbertTransform.cu:1572)
     return cr;
@@ -54,6 +60,13 @@ __global__ void k_wrap_log(const float* __restrict__ phase, const float* __restr
     }
 }
 
+// amp = exp(la), elementwise (la and amp may be the same array).
+__global__ void k_exp(const float* la, float* amp, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) amp[i] = expf(la[i]);
+}
+
 static void run_plain(Ctx& c, ctdm_cell& cl)
 {
     const int A = cl.A, H = cl.H, W = cl.W, V = cl.V;
@@ -62,9 +75,18 @@ static void run_plain(Ctx& c, ctdm_cell& cl)
     float* phase = (float*)c.buf("pl.phase", sizeof(float) * A * img);
     float* amp = (float*)c.buf("pl.amp", sizeof(float) * A * img);
     float* unw = cl.unwrapped_opt ? cl.unwrapped_opt : (float*)c.buf("pl.unw", sizeof(float) * A * img);
+    const bool by_rows = cl.recon_flags & CTDM_RETRIEVE_ROWS;
     for (int a = 0; a < A; ++a) {
-        retrieve_plain_one(c, cl.holo + (size_t)a * H * W, cl.bg + (size_t)a * H * W, H, W, cl.crop,
-                           phase + a * img, amp + a * img);
+        if (by_rows) {  // one angle per call; the map is ln A, turned into A for the reference's copy kernel
+            retrieve_rows(c, cl.holo + (size_t)a * H * W, cl.bg + (size_t)a * H * W, 1, H, W, cl.crop.col,
+                          phase + a * img, amp + a * img);
+            hipLaunchKernelGGL(k_exp, dim3(grid1d((long long)img, 256)), dim3(256), 0, c.stream, amp + a * img,
+                               amp + a * img, (long long)img);
+            CTDM_CHECK_LAUNCH("k_exp");
+        } else {
+            retrieve_plain_one(c, cl.holo + (size_t)a * H * W, cl.bg + (size_t)a * H * W, H, W, cl.crop,
+                               phase + a * img, amp + a * img);
+        }
         CTDM_HIP(hipStreamSynchronize(c.stream));
         unwrap_dct_plain_one(c, phase + a * img, unw + a * img, V, cl.unwrap_flags);
         CTDM_HIP(hipStreamSynchronize(c.stream));
@@ -104,12 +126,6 @@ static void run_plain(Ctx& c, ctdm_cell& cl)
     CTDM_HIP(hipStreamSynchronize(c.stream));
     positivity_plain(c, F, Fm, hits, (float2*)cl.ri, V, cl.iters, cl.optics, cl.recon_flags);
     CTDM_HIP(hipStreamSynchronize(c.stream));
-}
-
-__global__ void k_exp(const float* __restrict__ la, float* __restrict__ amp, long long n)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) amp[i] = expf(la[i]);
 }
 
 void recon_from_maps(Ctx& c, const float* unwrapped, const float* map2, const ctdm_angle* ang, int A, int V,
@@ -185,7 +201,10 @@ static void enqueue_batched(Ctx& c, ctdm_cell& cl)
     float* la = cl.logamp_opt ? cl.logamp_opt : (float*)c.buf("bt.la." + key, sizeof(float) * n);
     float* un = cl.unwrapped_opt ? cl.unwrapped_opt : (float*)c.buf("bt.un." + key, sizeof(float) * n);
     float2* spec = (float2*)c.buf("bt.spec." + key, sizeof(float2) * n);
-    retrieve_batched(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.crop, wr, la, nullptr);
+    if (cl.recon_flags & CTDM_RETRIEVE_ROWS)
+        retrieve_rows(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.crop.col, wr, la);
+    else
+        retrieve_batched(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.crop, wr, la, nullptr);
     unwrap_dct_batched(c, wr, un, A, V, cl.unwrap_flags);
     const float* un_r = un;
     const float* la_r = la;

@@ -83,6 +83,9 @@ typedef struct ctdm_angle { double ang_x, ang_y; } ctdm_angle;
                                       (MedianFilter, Recon_kernel.cu:7-27; see ctdm_median3x3) */
 #define CTDM_MAPS_AMPLITUDE 0x8u   /* ctdm_recon_from_maps: the second map holds the amplitude A (BatchRecon's
                                       AmpStack), sanitised by checkMaps and logged on the device */
+#define CTDM_RETRIEVE_ROWS  0x10u  /* ctdm_reconstruct: row-wise retrieval (extractQPI_AlgorithmB, the
+                                      reference's default) instead of the 2-D crop; crop.col holds the
+                                      sideband column (crop.radius < 0: located from bg[0]); V = W / 4 */
 #define CTDM_EXEC_PLAIN     0x100u /* run the reference-structured, unfused kernel sequence */
 
 /* ---------------------------------------------------------------- context */
@@ -111,6 +114,24 @@ ctdm_status ctdm_find_sideband(ctdm_ctx *ctx, const float *bg, int H, int W, int
 ctdm_status ctdm_retrieve(ctdm_ctx *ctx, const float *holo, const float *bg, int batch, int H, int W,
                           const ctdm_crop *crop, float *wrapped, float *logamp,
                           ctdm_complex *field_opt);
+
+/* Replaces the sideband search of extractQPI_AlgorithmB (sequence1DFFT frame 1,
+ * HilbertTransform.cu:1634-1643): every row of the background, resampled to H/4 rows as
+ * bilinear_interpolation_kernel does (:3658-3718), is transformed along x; HistogramFT (:1728-1748) sums
+ * |X| over the rows per fftshifted column; the first maximum over columns [0.6 W, W) is the sideband
+ * column, mirrored to W - column when it lies within W/8 of the end. Writes the column to HOST *center
+ * (synchronises). Square 256, 512 or 1024 holograms. */
+ctdm_status ctdm_find_sideband_rows(ctdm_ctx *ctx, const float *bg, int H, int W, int *center);
+
+/* Replaces extractQPI_AlgorithmB (HilbertTransform.cu:1587-1605) + sequence1DFFT (:1608-1665) +
+ * calcWrapPhase (:3618-3639), the reference's default retrieval (QPI_Method 0), for `batch` hologram
+ * pairs: bilinear resampling to H/4 rows, forward transform of each row, the W/4 fftshifted columns
+ * [center - W/8, center + W/8) kept and swapped by halves (CropFTdomain, shiftArray), inverse W/4-point
+ * transform. Outputs per item, (W/4) x (H/4) maps in the reference's transposed layout (copySharedMem
+ * :1704-1726): value (n, y') at n * (H/4) + y', with y' the resampled row and n the filtered column;
+ * wrapped = arg(SP) - arg(BG) in (-pi, pi], logamp = ln(|SP| / |BG|), non-finite -> 0. */
+ctdm_status ctdm_retrieve_rows(ctdm_ctx *ctx, const float *holo, const float *bg, int batch, int H, int W,
+                               int center, float *wrapped, float *logamp);
 
 /* ---------------------------------------------------------------- unwrapping (L3) */
 /* Replaces DCT_UWLS_Unwrapped (DCT_Unwrapping.cu:10-59; RHS option from FastUnwrapping.cu:5-117):
@@ -211,7 +232,7 @@ typedef struct ctdm_cell {
     int V;                      /* volume side = crop side M */
     int iters;                  /* positivity iterations (reference IterTime + 1) */
     unsigned unwrap_flags;      /* CTDM_UNWRAP_* */
-    unsigned recon_flags;       /* CTDM_DC_* | CTDM_POS_NO_SHIFT | CTDM_MAPS_MEDIAN */
+    unsigned recon_flags;       /* CTDM_DC_* | CTDM_POS_NO_SHIFT | CTDM_MAPS_MEDIAN | CTDM_RETRIEVE_ROWS */
     ctdm_optics optics;
     ctdm_crop crop;             /* crop.M == V; crop.radius < 0: locate from bg[0] (reference frame 1) */
     ctdm_complex *ri;           /* V^3 complex RI output (device) */

@@ -57,6 +57,17 @@ inline void extractQPI_AlgorithmA(const Context& c, const float* SP, const float
     c.check(ctdm_retrieve(c.get(), SP, BG, 1, Ny, Nx, &crop, wrapped, logamp, nullptr));
 }
 
+// extractQPI_AlgorithmB(SP, BG, cuSP_FFT, cuBG_FFT, Nx, Ny, frame) (HilbertTransform.cu:1587) followed by
+// calcWrapPhase: the reference's default retrieval. On frame 1 the sideband column is located from BG
+// and kept in `center` for the following frames (sequence1DFFT, HilbertTransform.cu:1634). Outputs are
+// (Nx/4) x (Ny/4) maps in the reference's transposed layout.
+inline void extractQPI_AlgorithmB(const Context& c, const float* SP, const float* BG, int Nx, int Ny, int frame,
+                                  int& center, float* wrapped, float* logamp)
+{
+    if (frame == 1 || center < 0) c.check(ctdm_find_sideband_rows(c.get(), BG, Ny, Nx, &center));
+    c.check(ctdm_retrieve_rows(c.get(), SP, BG, 1, Ny, Nx, center, wrapped, logamp));
+}
+
 // DCT_UWLS_Unwrapped(float *ImgDst, float *ImgSrc, int sizeX, int sizeY) (DCT_Unwrapping.cu:10).
 // flags = CTDM_UNWRAP_REFCOMPAT reproduces the reference's eigenvalues and IDCT exactly.
 inline void DCT_UWLS_Unwrapped(const Context& c, float* ImgDst, const float* ImgSrc, int sizeX, int sizeY,

@@ -67,6 +67,8 @@ _EXTRA: dict = {
     "orc_goldstein": [_fp, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, _ip, _fp],
     "orc_fit_pgd": [_dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                     C.c_double, _dp, C.c_uint],
+    "orc_find_sideband_rows": [_fp, C.c_int, C.c_int],
+    "orc_retrieve_rows": [_fp, _fp, C.c_int, C.c_int, C.c_int, _dp, _dp],
     "orc_median3x3": [_fp, _fp, C.c_int, C.c_int, C.c_int],
     "orc_check_maps": [C.c_void_p, C.c_void_p, C.c_long],
     "orc_recon_from_maps": [_fp, _fp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
@@ -312,3 +314,23 @@ def recon_from_maps(unwrapped, map2, angles, wavelength, n_med, dx, iters, flags
     ri = np.zeros((V, V, V), np.complex128)
     lib().orc_recon_from_maps(_p(u, _fp), _p(m, _fp), _p(a, _dp), A, V, wavelength, n_med, dx, iters, flags, _cp(ri))
     return ri
+
+
+def find_sideband_rows(bg):
+    b = np.ascontiguousarray(bg, np.float32)
+    H, W = b.shape
+    L = lib()
+    L.orc_find_sideband_rows.restype = C.c_int
+    return int(L.orc_find_sideband_rows(_p(b, _fp), H, W))
+
+
+def retrieve_rows(holo, bg, center):
+    """Algorithm B maps, shape (W/4, H/4) indexed [n][y'] (the reference's transposed layout)."""
+    h = np.ascontiguousarray(holo, np.float32)
+    b = np.ascontiguousarray(bg, np.float32)
+    H, W = h.shape
+    M, N2 = W // 4, H // 4
+    w = np.zeros((M, N2))
+    la = np.zeros((M, N2))
+    lib().orc_retrieve_rows(_p(h, _fp), _p(b, _fp), H, W, center, _p(w, _dp), _p(la, _dp))
+    return {"wrapped": w, "logamp": la}

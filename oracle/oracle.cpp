@@ -255,6 +255,113 @@ void orc_find_sideband(const float* bg, int H, int W, int row_start, int* row, i
     *radius = std::abs(r - H / 2) / 3;
 }
 
+// bilinear_interpolation_kernel (HilbertTransform.cu:3658-3718) for one image, W x H -> W x H/4, in
+// float as the reference evaluates it: x positions i (W-1)/W, y positions j (H-1)/(H/4).
+static void resample_rows_b(const float* in, int H, int W, std::vector<float>& out)
+{
+    const int N2 = H / 4;
+    out.assign((size_t)W * N2, 0.f);
+    const float xr = (float)(W - 1) / (float)W, yr = (float)(H - 1) / (float)N2;
+    for (int j = 0; j < N2; ++j)
+        for (int i = 0; i < W; ++i) {
+            const int ix = (int)(xr * (float)i);
+            const float a = xr * (float)i - (float)ix;
+            const int iy = (int)(yr * (float)j);
+            const float b = yr * (float)j - (float)iy;
+            const float d00 = in[(size_t)iy * W + ix];
+            const float d10 = ix + 1 < W ? in[(size_t)iy * W + ix + 1] : 0.f;
+            const float d01 = iy + 1 < H ? in[(size_t)(iy + 1) * W + ix] : 0.f;
+            const float d11 = (ix + 1 < W && iy + 1 < H) ? in[(size_t)(iy + 1) * W + ix + 1] : 0.f;
+            const float r1 = a * d10 + (-d00 * a + d00);
+            const float r2 = a * d11 + (-d01 * a + d01);
+            out[(size_t)j * W + i] = b * r2 + (-r1 * b + r1);
+        }
+}
+
+// Resampled rows -> forward W-point DFT of each row -> shiftArray (swap halves) (sequence1DFFT,
+// HilbertTransform.cu:1608-1632).
+static std::vector<cd> rows_spectrum_b(const float* img, int H, int W)
+{
+    std::vector<float> r;
+    resample_rows_b(img, H, W, r);
+    const int N2 = H / 4;
+    std::vector<cd> F((size_t)W * N2);
+    for (size_t i = 0; i < F.size(); ++i) F[i] = cd(r[i], 0.0);
+    for (int j = 0; j < N2; ++j) {
+        cd* row = F.data() + (size_t)j * W;
+        fft1(row, W, 1, -1);
+        for (int i = 0; i < W / 2; ++i) std::swap(row[i], row[i + W / 2]);
+    }
+    return F;
+}
+
+extern "C" {
+
+// Algorithm B sideband column (sequence1DFFT frame 1, HilbertTransform.cu:1634-1643): HistogramFT
+// sums |F| of the shifted row spectra of the background over all rows (:1728-1748); the first maximum
+// over columns [0.6 W, W) wins (thrust::max_element); a maximum closer than W/8 to the end is mirrored
+// to W - index.
+int orc_find_sideband_rows(const float* bg, int H, int W)
+{
+    const std::vector<cd> F = rows_spectrum_b(bg, H, W);
+    const int N2 = H / 4;
+    int best_i = -1;
+    double best = -1.0;
+    for (int i = (int)(W * 0.6); i < W; ++i) {
+        double sum = 0;
+        for (int j = 0; j < N2; ++j) sum += std::abs(F[i + (size_t)W * j]);
+        if (sum > best) {
+            best = sum;
+            best_i = i;
+        }
+    }
+    if (W - best_i < (W / 4) / 2) best_i = W - best_i;
+    return best_i;
+}
+
+// Algorithm B retrieval (extractQPI_AlgorithmB, HilbertTransform.cu:1587-1605; sequence1DFFT
+// :1608-1665; CropFTdomain :1811-1823; copySharedMem :1704-1726) + calcWrapPhase (:3618-3639): per
+// resampled row the W/4 shifted-spectrum columns [center - W/8, center + W/8) are cut out, swapped by
+// halves, inverse-transformed (W/4 points) and scaled by 1/(W/4) with conjugation; the map is stored
+// transposed, out[y' + (H/4) n]. wrapped = arg SP - arg BG in (-pi, pi], logamp = ln(|SP| / |BG|).
+// Square holograms (H == W).
+void orc_retrieve_rows(const float* holo, const float* bg, int H, int W, int center, double* wrapped,
+                       double* logamp)
+{
+    const int M = W / 4, N2 = H / 4;
+    const std::vector<cd> S = rows_spectrum_b(holo, H, W), B = rows_spectrum_b(bg, H, W);
+    std::vector<cd> gs((size_t)M * N2), gb((size_t)M * N2);
+    for (int j = 0; j < N2; ++j) {
+        cd* a = gs.data() + (size_t)j * M;
+        cd* b = gb.data() + (size_t)j * M;
+        for (int x = 0; x < M; ++x) {
+            const int src = x - W / 8 + center;
+            a[x] = S[src + (size_t)W * j];
+            b[x] = B[src + (size_t)W * j];
+        }
+        for (int x = 0; x < M / 2; ++x) {
+            std::swap(a[x], a[x + M / 2]);
+            std::swap(b[x], b[x + M / 2]);
+        }
+        fft1(a, M, 1, +1);
+        fft1(b, M, 1, +1);
+    }
+    for (int j = 0; j < N2; ++j)
+        for (int n = 0; n < M; ++n) {
+            const cd sp = std::conj(gs[n + (size_t)M * j]) / (double)M;
+            const cd bv = std::conj(gb[n + (size_t)M * j]) / (double)M;
+            double p = std::atan2(bv.imag(), bv.real()) - std::atan2(sp.imag(), sp.real());
+            if (!std::isfinite(p)) p = 0;
+            double l = std::log(std::abs(sp) / std::abs(bv));
+            if (!std::isfinite(l)) l = 0;
+            const size_t o = (size_t)j + (size_t)N2 * n;
+            if (wrapped) wrapped[o] = wrap_pi(p);
+            if (logamp) logamp[o] = l;
+        }
+}
+
+}  // extern "C"
+
 // extractQPI_AlgorithmA (HilbertTransform.cu:1546-1585) + calcWrapPhase (:3618-3639):
 // FFT2 -> cuFFT2Dshift -> get1stOrder_new (:3566-3586, M-generalised) -> cuFFT2Dshift (M) ->
 // inverse FFT2 -> scaleFFT2D (x 1/M^2 and conjugate, FFT.cu:853-868) ->
@@ -841,8 +948,13 @@ void orc_reconstruct_cell(const float* holo, const float* bg, const double* angl
     std::vector<double> phase(A * img), la(A * img), unw(A * img);
     for (int a = 0; a < A; ++a) {
         std::vector<double> pr(img), am(img), lg(img), wr(img);
-        orc_retrieve(holo + a * hh, bg + a * hh, H, H, row, col, radius, V, pr.data(), am.data(), wr.data(),
-                     lg.data(), nullptr);
+        if (recon_flags & 16u) {  // row-wise retrieval (Algorithm B), sideband column in col
+            orc_retrieve_rows(holo + a * hh, bg + a * hh, H, H, col, wr.data(), lg.data());
+            pr = wr;
+        } else {
+            orc_retrieve(holo + a * hh, bg + a * hh, H, H, row, col, radius, V, pr.data(), am.data(), wr.data(),
+                         lg.data(), nullptr);
+        }
         std::memcpy(phase.data() + a * img, pr.data(), sizeof(double) * img);
         std::memcpy(la.data() + a * img, lg.data(), sizeof(double) * img);
         if (wrapped_out) std::memcpy(wrapped_out + a * img, wr.data(), sizeof(double) * img);

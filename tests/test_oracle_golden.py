@@ -169,3 +169,46 @@ def test_retrieve_recovers_smooth_phase_and_amplitude(orc):
     # the reference's own wrapped output lies in (-2 pi, 2 pi) and equals ours mod 2 pi
     d = r["phase_ref"] - r["wrapped"]
     np.testing.assert_allclose(np.round(d / (2 * np.pi)) * 2 * np.pi, d, atol=1e-9)
+
+
+def _rows_holograms(N, k0, phase_amp=2.0, sigma=None, amp_mod=0.0):
+    """Carrier along x at bin k0 (k0 / N cycles per pixel); SP carries a smooth phase and amplitude."""
+    y, x = np.mgrid[0:N, 0:N].astype(np.float64)
+    sigma = sigma or N / 6
+    phi = phase_amp * np.exp(-((x - 0.45 * N) ** 2 + (y - 0.55 * N) ** 2) / (2 * sigma ** 2))
+    a = 1.0 + amp_mod * np.exp(-((x - 0.5 * N) ** 2 + (y - 0.5 * N) ** 2) / (2 * sigma ** 2))
+    sp = a * a + 1 + 2 * a * np.cos(2 * np.pi * k0 * x / N + phi)
+    bg = 2 + 2 * np.cos(2 * np.pi * k0 * x / N)
+    return sp.astype(np.float32), bg.astype(np.float32), phi, a
+
+
+def test_rows_sideband_column_is_the_carrier(orc):
+    """HistogramFT peak: carrier bin k0 sits at shifted column k0 + N/2."""
+    for N, k0 in [(256, 72), (512, 140)]:
+        _, bg, _, _ = _rows_holograms(N, k0)
+        assert orc.find_sideband_rows(bg) == k0 + N // 2
+
+
+def test_rows_retrieval_identical_pair_is_zero(orc):
+    sp, bg, _, _ = _rows_holograms(256, 72)
+    r = orc.retrieve_rows(bg, bg, 72 + 128)
+    assert np.abs(r["wrapped"]).max() == 0 and np.abs(r["logamp"]).max() == 0
+
+
+def test_rows_retrieval_recovers_phase_and_amplitude(orc):
+    """Demodulation by the cropped band: output (n, y') samples the object at x = 4 n (N-1)/N, y = y' (N-1)/(N/4),
+    and the resampling's carrier distortion cancels between SP and BG."""
+    N, k0 = 256, 72
+    sp, bg, phi, a = _rows_holograms(N, k0, amp_mod=0.2)
+    r = orc.retrieve_rows(sp, bg, k0 + N // 2)
+    M = N // 4
+    n, yq = np.meshgrid(np.arange(M), np.arange(M), indexing="ij")
+    xs = 4 * n * (N - 1) / N
+    ys = yq * (N - 1) / (N / 4)
+    y, x = np.mgrid[0:N, 0:N].astype(np.float64)
+    from scipy.ndimage import map_coordinates
+    phi_s = map_coordinates(phi, [ys, xs], order=1)
+    la_s = np.log(map_coordinates(a, [ys, xs], order=1))
+    inner = (slice(4, M - 4), slice(4, M - 4))  # away from the periodic wrap of the row transform
+    assert np.abs(r["wrapped"][inner] - phi_s[inner]).max() < 0.03
+    assert np.abs(r["logamp"][inner] - la_s[inner]).max() < 0.03
