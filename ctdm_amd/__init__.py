@@ -97,6 +97,8 @@ SIGNATURES = {
     "ctdm_retrieve": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(Crop), _vp, _vp, _vp], C.c_int),
     "ctdm_find_sideband_rows": ([_vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "ctdm_retrieve_rows": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp], C.c_int),
+    "ctdm_angle_from_image": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.POINTER(C.c_double)], C.c_int),
     "ctdm_unwrap_dct": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_uint], C.c_int),
     "ctdm_unwrap_dct_adjoint": ([_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_uint], C.c_int),
     "ctdm_goldstein": ([_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp], C.c_int),
@@ -114,6 +116,11 @@ SIGNATURES = {
     "ctdm_time_pass": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "ctdm_time_loop": ([_vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "ctdm_fit_pgd": ([_vp, _vp, _vp, C.POINTER(Angle), C.c_int, C.c_int, C.POINTER(Optics), C.c_int, C.c_double, _vp, C.c_uint], C.c_int),
+    "ctdm_phimap_write": ([C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _vp], C.c_int),
+    "ctdm_phimap_read": ([C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                          C.POINTER(C.c_int), C.POINTER(C.c_int), _vp, _vp, C.c_size_t], C.c_int),
+    "ctdm_rimap_write": ([C.c_char_p, _vp, C.c_size_t, C.c_int], C.c_int),
+    "ctdm_rimap_read": ([C.c_char_p, _vp, C.c_size_t, C.c_int], C.c_int),
     "ctdm_synth_default_params": ([C.POINTER(SynthParams)], None),
     "ctdm_synth_cell": ([C.c_uint64, C.c_int, C.POINTER(SynthParams), _vp, _vp, C.POINTER(Angle)], C.c_int),
     "ctdm_synth_hologram": ([C.c_uint64, C.c_int, _vp, _vp], C.c_int),
@@ -173,6 +180,56 @@ def synth_params(**kw) -> SynthParams:
     return p
 
 
+def phimap_write(path, phase, amp, ang_x: float = 0.0, ang_y: float = 0.0, status: bool = True):
+    """Writes one frame in the reference's .phimap format (host arrays, sizeY x sizeX)."""
+    ph = np.ascontiguousarray(phase, np.float32)
+    am = np.ascontiguousarray(amp, np.float32)
+    sy, sx = ph.shape
+    _check_free(lib().ctdm_phimap_write(str(path).encode(), int(bool(status)), ang_x, ang_y, sx, sy,
+                                        ph.ctypes.data, am.ctypes.data))
+
+
+def phimap_read(path):
+    """Returns dict(status, ang_x, ang_y, phase, amp) of a .phimap file."""
+    st, sx, sy = C.c_int(), C.c_int(), C.c_int()
+    ax, ay = C.c_double(), C.c_double()
+    L = lib()
+    _check_free(L.ctdm_phimap_read(str(path).encode(), C.byref(st), C.byref(ax), C.byref(ay), C.byref(sx),
+                                   C.byref(sy), None, None, 0))
+    ph = np.empty((sy.value, sx.value), np.float32)
+    am = np.empty_like(ph)
+    _check_free(L.ctdm_phimap_read(str(path).encode(), None, None, None, None, None, ph.ctypes.data,
+                                   am.ctypes.data, ph.size))
+    return dict(status=bool(st.value), ang_x=ax.value, ang_y=ay.value, phase=ph, amp=am)
+
+
+def load_phimap_stack(paths):
+    """The reference's reload path (LoadDateFromDir + RefreshStack, HilbertTransform.cu:1453-1530): frames
+    whose status is 0 are dropped; returns (phase stack, amplitude stack, angles A x 2). The reference
+    reads angle y from the x field (:1479); here each angle keeps its own field."""
+    ph, am, ang = [], [], []
+    for p in paths:
+        d = phimap_read(p)
+        if not d["status"]:
+            continue
+        ph.append(d["phase"])
+        am.append(d["amp"])
+        ang.append((d["ang_x"], d["ang_y"]))
+    return np.stack(ph), np.stack(am), np.array(ang)
+
+
+def rimap_write(path, ri, part: int = 0):
+    """Writes a volume as BatchRecon exports it: part 0 complex64, 1 real, 2 imaginary."""
+    v = np.ascontiguousarray(ri, np.complex64)
+    _check_free(lib().ctdm_rimap_write(str(path).encode(), v.ctypes.data, v.size, part))
+
+
+def rimap_read(path, shape, part: int = 0):
+    v = np.zeros(shape, np.complex64)
+    _check_free(lib().ctdm_rimap_read(str(path).encode(), v.ctypes.data, v.size, part))
+    return v
+
+
 def synth_hologram(seed: int, H: int = 1024):
     holo = np.empty((H, H), np.float32)
     bg = np.empty((H, H), np.float32)
@@ -196,9 +253,9 @@ def synth_wrapped(seed: int, batch: int, H: int, W: int):
     return out
 
 
-def _check_free(status):
+def _check_free(status, what: str = "host call rejected its arguments"):
     if status != OK:
-        raise CtdmError(status, "synthetic generator rejected its arguments")
+        raise CtdmError(status, what)
 
 
 def optics_for(V: int, H: int = 1024, pixel: float = 0.1e-6, wavelength: float = 532e-9,
@@ -252,6 +309,14 @@ class Context:
         crop = Crop()
         self._call("ctdm_find_sideband", _ptr(bg), H, W, M, row_start, C.byref(crop))
         return crop
+
+    def angle_from_image(self, img, pixel_um: float, mag: float, wavelength_nm: float, n_med: float):
+        """AngCal_GPU: illumination angle [rad] of each angle-calibration image (batch x H x W)."""
+        im = img if img.dim() == 3 else img[None]
+        B, H, W = im.shape
+        out = (C.c_double * B)()
+        self._call("ctdm_angle_from_image", _ptr(im), B, H, W, pixel_um, mag, wavelength_nm, n_med, out)
+        return list(out) if img.dim() == 3 else out[0]
 
     def find_sideband_rows(self, bg) -> int:
         H, W = bg.shape[-2:]

@@ -176,6 +176,15 @@ ctdm_status ctdm_retrieve_rows(ctdm_ctx* ctx, const float* holo, const float* bg
     });
 }
 
+ctdm_status ctdm_angle_from_image(ctdm_ctx* ctx, const float* img, int batch, int H, int W, double pixel_um,
+                                  double mag, double wavelength_nm, double n_med, double* angles)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        CTDM_REQUIRE(batch >= 0, "angle_from_image: negative batch");
+        if (batch > 0) ctdm::angle_from_image(c, img, batch, H, W, pixel_um, mag, wavelength_nm, n_med, angles);
+    });
+}
+
 ctdm_status ctdm_unwrap_dct(ctdm_ctx* ctx, const float* wrapped, float* unwrapped, int batch, int M,
                             unsigned flags)
 {

@@ -96,6 +96,8 @@ struct Ctx {
 // Retrieval (k_retrieve.hip)
 void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, ctdm_crop* out);
 int find_sideband_rows(Ctx& c, const float* bg, int H, int W);
+void angle_from_image(Ctx& c, const float* img, int batch, int H, int W, double pixel_um, double mag,
+                      double wavelength_nm, double n_med, double* angles);
 void retrieve_rows(Ctx& c, const float* holo, const float* bg, int batch, int H, int W, int center, float* wrapped,
                    float* logamp);
 void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int H, int W,

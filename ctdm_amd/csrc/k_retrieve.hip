@@ -319,6 +319,67 @@ void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, 
     out->M = M;
 }
 
+// ---------------------------------------------------------------- illumination angle (AngCal_GPU)
+// Order-preserving 32-bit image of a float (negative values below positive ones).
+__device__ __forceinline__ unsigned long long ordered_bits(float v)
+{
+    const unsigned u = __float_as_uint(v);
+    return (u & 0x80000000u) ? (unsigned long long)(~u) : (unsigned long long)(u | 0x80000000u);
+}
+
+// First maximum of log10|F| (non-finite -> 0, cufftComplex2Real HilbertTransform.cu:2984-2996) over the
+// flat fftshifted indices [lo, hi), as thrust::max_element scans (AngCal_GPU :2958-2960).
+__global__ void k_argmax_log10(const float2* __restrict__ spec, int H, int W, long long lo, long long hi,
+                               unsigned long long* __restrict__ best)
+{
+    unsigned long long local = 0;
+    for (long long g = lo + (long long)blockIdx.x * blockDim.x + threadIdx.x; g < hi;
+         g += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(g / W), i = (int)(g % W);
+        const float2 v = spec[(long long)((j + H / 2) % H) * W + (i + W / 2) % W];
+        float d = log10f(sqrtf(v.x * v.x + v.y * v.y));
+        if (!isfinite(d)) d = 0.f;
+        const unsigned long long key = (ordered_bits(d) << 32) | (0xffffffffull - (unsigned long long)g);
+        local = key > local ? key : local;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(local, off);
+        local = o > local ? o : local;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(best, local);
+}
+
+void angle_from_image(Ctx& c, const float* img, int batch, int H, int W, double pixel_um, double mag,
+                      double wavelength_nm, double n_med, double* angles)
+{
+    CTDM_REQUIRE(img && angles, "angle_from_image: null pointer");
+    CTDM_REQUIRE(H == W && (H == 256 || H == 512 || H == 1024), "angle_from_image: square 256, 512 or 1024 images");
+    CTDM_REQUIRE(pixel_um > 0 && mag > 0 && wavelength_nm > 0 && n_med > 0, "angle_from_image: bad optics");
+    const long long N = (long long)H * W;
+    float2* spec = (float2*)c.buf("ang.spec", sizeof(float2) * (size_t)N);
+    unsigned long long* best = (unsigned long long*)c.buf("ang.best", sizeof(unsigned long long) * (size_t)batch);
+    const long long lo = (long long)(N * 0.53), hi = (long long)(N * 0.79);
+    std::vector<unsigned long long> h(batch);
+    for (int b = 0; b < batch; ++b) {
+        const float* im = img + (size_t)b * N;
+        row_real(c, im, 1, im, 1, H, W, 0, W, spec);
+        fft_cols(c, spec, H, W, W, 1, 0, -1, 1.f, false);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c.stream, best + b);
+        hipLaunchKernelGGL(k_argmax_log10, dim3(256), dim3(256), 0, c.stream, spec, H, W, lo, hi, best + b);
+        CTDM_CHECK_LAUNCH("k_argmax_log10");
+    }
+    CTDM_HIP(hipMemcpyAsync(h.data(), best, sizeof(unsigned long long) * batch, hipMemcpyDeviceToHost, c.stream));
+    CTDM_HIP(hipStreamSynchronize(c.stream));
+    const double fm = n_med / (wavelength_nm * 1e-9);
+    for (int b = 0; b < batch; ++b) {
+        const long long idx = (long long)(0xffffffffull - (h[b] & 0xffffffffull));
+        const int posX = (int)(idx % W);
+        const double fx = (double)(W / 2 - 1 - posX) / ((double)W * pixel_um * 1e-6);  // (511 - posX) / (1024 dx)
+        const double st = mag * fx / fm;
+        angles[b] = st < 0 ? std::fabs(std::asin(st)) : -std::fabs(std::asin(st));
+    }
+}
+
 // ---------------------------------------------------------------- row-wise retrieval (Algorithm B)
 // extractQPI_AlgorithmB (HilbertTransform.cu:1587-1605) + sequence1DFFT (:1608-1665) + calcWrapPhase:
 // each hologram is resampled to H/4 rows by bilinear_interpolation_kernel (:3658-3718; x positions

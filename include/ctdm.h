@@ -133,6 +133,16 @@ ctdm_status ctdm_find_sideband_rows(ctdm_ctx *ctx, const float *bg, int H, int W
 ctdm_status ctdm_retrieve_rows(ctdm_ctx *ctx, const float *holo, const float *bg, int batch, int H, int W,
                                int center, float *wrapped, float *logamp);
 
+/* Replaces AngCal_GPU (HilbertTransform.cu:2920-2982), the illumination angle of an angle-calibration
+ * image: 2-D DFT of each of `batch` H x W images, fftshift, log10|F| (non-finite -> 0), first maximum over
+ * the flat shifted indices [0.53 HW, 0.79 HW); with posX its column,
+ * fx = (W/2 - 1 - posX) / (W * pixel_um * 1e-6), sin(theta) = mag * fx / (n_med / (wavelength_nm * 1e-9)),
+ * angle = |asin| with the opposite sign of sin(theta) (the reference's convention; W = 1024 gives its
+ * literal 511 and 1024). Writes `batch` angles [rad] to HOST memory (synchronises). Square 256, 512 or 1024
+ * images. */
+ctdm_status ctdm_angle_from_image(ctdm_ctx *ctx, const float *img, int batch, int H, int W, double pixel_um,
+                                  double mag, double wavelength_nm, double n_med, double *angles);
+
 /* ---------------------------------------------------------------- unwrapping (L3) */
 /* Replaces DCT_UWLS_Unwrapped (DCT_Unwrapping.cu:10-59; RHS option from FastUnwrapping.cu:5-117):
  * least-squares unwrapping with Neumann boundaries via an orthonormal DCT-II Poisson solve, for
@@ -276,6 +286,23 @@ ctdm_status ctdm_time_pass(ctdm_ctx *ctx, int pass, int V, int pencils, int reps
  * launch (bytes[3]). Uses the data-consistency list of the last reconstruction at this V, when there is
  * one. No reference counterpart: the per-kernel measurement of the loop as it runs. Synchronises. */
 ctdm_status ctdm_time_loop(ctdm_ctx *ctx, int V, int iters, double *avg_ms, double *bytes);
+
+/* ---------------------------------------------------------------- on-disk formats (host only) */
+/* The reference's per-frame .phimap file (outputImg / openImage, HilbertTransform.cu:2437-2477), written
+ * as the reference writes it: uint8 status, float64 angle x, float64 angle y, int32 sizeX, int32 sizeY,
+ * then sizeX*sizeY (phase, amplitude) float32 pairs (microImg), little endian, no padding. All pointers
+ * HOST. status 0 marks a frame the reference's quality check rejected (RefreshStack drops it). */
+ctdm_status ctdm_phimap_write(const char *path, int status, double ang_x, double ang_y, int sizeX, int sizeY,
+                              const float *phase, const float *amp);
+/* Reads a .phimap. Header outputs may be NULL; with phase = amp = NULL only the header is read,
+ * otherwise `capacity` (elements of each array) must hold sizeX*sizeY. */
+ctdm_status ctdm_phimap_read(const char *path, int *status, double *ang_x, double *ang_y, int *sizeX,
+                             int *sizeY, float *phase, float *amp, size_t capacity);
+/* The reconstructed volume as BatchRecon exports it (HilbertTransform.cu:2013-2041, exportComplex /
+ * exportRAW_F :3457-3487): part 0 = raw complex64, 1 = real floats, 2 = imaginary floats. HOST arrays
+ * of n elements; reading part 1 or 2 fills only that component. */
+ctdm_status ctdm_rimap_write(const char *path, const ctdm_complex *ri, size_t n, int part);
+ctdm_status ctdm_rimap_read(const char *path, ctdm_complex *ri, size_t n, int part);
 
 /* ---------------------------------------------------------------- differentiable fit */
 /* Projected-gradient fit of measured Rytov spectra (no reference counterpart; an alternative to the

@@ -68,6 +68,16 @@ inline void extractQPI_AlgorithmB(const Context& c, const float* SP, const float
     c.check(ctdm_retrieve_rows(c.get(), SP, BG, 1, Ny, Nx, center, wrapped, logamp));
 }
 
+// AngCal_GPU(float *src, int sizeX, int sizeY, int frame, int totalFrame) (HilbertTransform.cu:2920) with
+// the optics the reference reads from globals (dx = camera pixel [um], Mag, wavelength [nm], Nmed).
+inline float AngCal_GPU(const Context& c, const float* src, int sizeX, int sizeY, double dx_um, double mag,
+                        double wavelength_nm, double n_med)
+{
+    double a = 0;
+    c.check(ctdm_angle_from_image(c.get(), src, 1, sizeY, sizeX, dx_um, mag, wavelength_nm, n_med, &a));
+    return (float)a;
+}
+
 // DCT_UWLS_Unwrapped(float *ImgDst, float *ImgSrc, int sizeX, int sizeY) (DCT_Unwrapping.cu:10).
 // flags = CTDM_UNWRAP_REFCOMPAT reproduces the reference's eigenvalues and IDCT exactly.
 inline void DCT_UWLS_Unwrapped(const Context& c, float* ImgDst, const float* ImgSrc, int sizeX, int sizeY,

@@ -334,3 +334,15 @@ def retrieve_rows(holo, bg, center):
     la = np.zeros((M, N2))
     lib().orc_retrieve_rows(_p(h, _fp), _p(b, _fp), H, W, center, _p(w, _dp), _p(la, _dp))
     return {"wrapped": w, "logamp": la}
+
+
+def angle_from_image(img, pixel_um, mag, wavelength_nm, n_med):
+    """Returns (angle [rad], shifted column of the peak)."""
+    a = np.ascontiguousarray(img, np.float32)
+    H, W = a.shape
+    L = lib()
+    L.orc_angle_from_image.argtypes = [_fp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _ip]
+    L.orc_angle_from_image.restype = C.c_double
+    pos = np.zeros(1, np.int32)
+    ang = L.orc_angle_from_image(_p(a, _fp), H, W, pixel_um, mag, wavelength_nm, n_med, _p(pos, _ip))
+    return float(ang), int(pos[0])

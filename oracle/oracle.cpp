@@ -362,6 +362,39 @@ void orc_retrieve_rows(const float* holo, const float* bg, int H, int W, int cen
 
 }  // extern "C"
 
+extern "C" {
+
+// AngCal_GPU (HilbertTransform.cu:2920-2982): 2-D DFT, fftshift, log10|F| (non-finite -> 0), first
+// maximum over flat shifted indices [0.53 HW, 0.79 HW); fx = (W/2 - 1 - posX) / (W dx), sin = mag fx / fm,
+// angle = -sign(sin) |asin(sin)|. pixel_um = dx [um]. Returns the column posX through *pos_x.
+double orc_angle_from_image(const float* img, int H, int W, double pixel_um, double mag, double wavelength_nm,
+                            double n_med, int* pos_x)
+{
+    std::vector<cd> F((size_t)H * W);
+    for (size_t i = 0; i < F.size(); ++i) F[i] = cd(img[i], 0.0);
+    fft2(F.data(), W, H, -1);
+    shift2(F.data(), W, H);
+    const long N = (long)H * W;
+    const long lo = (long)(N * 0.53), hi = (long)(N * 0.79);
+    long best_i = lo;
+    double best = -1e300;
+    for (long g = lo; g < hi; ++g) {
+        double d = std::log10(std::abs(F[g]));
+        if (!std::isfinite(d)) d = 0;
+        if (d > best) {
+            best = d;
+            best_i = g;
+        }
+    }
+    const int posX = (int)(best_i % W);
+    if (pos_x) *pos_x = posX;
+    const double fx = (double)(W / 2 - 1 - posX) / ((double)W * pixel_um * 1e-6);
+    const double st = mag * fx / (n_med / (wavelength_nm * 1e-9));
+    return st < 0 ? std::fabs(std::asin(st)) : -std::fabs(std::asin(st));
+}
+
+}  // extern "C"
+
 // extractQPI_AlgorithmA (HilbertTransform.cu:1546-1585) + calcWrapPhase (:3618-3639):
 // FFT2 -> cuFFT2Dshift -> get1stOrder_new (:3566-3586, M-generalised) -> cuFFT2Dshift (M) ->
 // inverse FFT2 -> scaleFFT2D (x 1/M^2 and conjugate, FFT.cu:853-868) ->

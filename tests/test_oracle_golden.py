@@ -212,3 +212,16 @@ def test_rows_retrieval_recovers_phase_and_amplitude(orc):
     inner = (slice(4, M - 4), slice(4, M - 4))  # away from the periodic wrap of the row transform
     assert np.abs(r["wrapped"][inner] - phi_s[inner]).max() < 0.03
     assert np.abs(r["logamp"][inner] - la_s[inner]).max() < 0.03
+
+
+def test_angle_from_image_closed_form(orc):
+    """A cosine fringe at bins (kx, ky), ky > 0: the shifted peak in the searched band is (kx + N/2, ky + N/2),
+    so fx = (N/2 - 1 - posX) / (N dx) = -(kx + 1) / (N dx) and the angle follows from asin."""
+    N, kx, ky = 256, 37, 30
+    y, x = np.mgrid[0:N, 0:N].astype(np.float64)
+    img = (1 + np.cos(2 * np.pi * (kx * x + ky * y) / N)).astype(np.float32)
+    dx, mag, lam, nm = 5.5, 40.0, 532.0, 1.337
+    ang, pos = orc.angle_from_image(img, dx, mag, lam, nm)
+    assert pos == kx + N // 2
+    st = mag * (-(kx + 1) / (N * dx * 1e-6)) / (nm / (lam * 1e-9))
+    assert abs(ang - abs(np.arcsin(st))) < 1e-12 and ang > 0
