@@ -300,12 +300,12 @@ struct Plan<384> {
     }
 };
 template <>
-struct Plan<512> {
-    static constexpr int T = 32;
+struct Plan<512> {  // three radix-8 stages keep all 64 threads busy (16 x 32 leaves half idle in its second stage)
+    static constexpr int T = 64;
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
-        Stockham<512, T, DIR, 1, 16, 32>::run(t, s, tw, gl, gs);
+        Stockham<512, T, DIR, 1, 8, 8, 8>::run(t, s, tw, gl, gs);
     }
 };
 template <>

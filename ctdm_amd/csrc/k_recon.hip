@@ -382,6 +382,24 @@ __device__ __forceinline__ float2 modify_f(float2 n, const Proj& p)
     return make_float2(-p.k2 * re, -p.k2 * im);
 }
 
+// modify_f(est_n(scale * f)) in one step, as the fused x pass applies it: with (a, b) = sqrt(1 - f/k^2) and
+// the positivity clamps a >= 1, b >= 0 (Re n >= n_m, Im n >= 0 without the n_m factor),
+// f' = -k^2 ((a - 1)(a + 1) - b^2) - i k^2 2ab, the same value as the two operators in exact arithmetic.
+__device__ __forceinline__ float2 project(float2 f, const Proj& p, float scale)
+{
+    const float ik = scale * p.inv_k2;
+    const float c = 1.f - f.x * ik;
+    const float d = -f.y * ik;
+    const float r = __builtin_amdgcn_sqrtf(c * c + d * d);
+    const float h = 0.5f * (r + fabsf(c));
+    const float big = __builtin_amdgcn_sqrtf(h);
+    const float small = h > 0.f ? 0.5f * fabsf(d) * __builtin_amdgcn_rsqf(h) : 0.f;
+    float a = c >= 0.f ? big : small;
+    float b = d > 0.f ? (c >= 0.f ? small : big) : 0.f;  // b < 0 (d < 0) and b = 0 (d = 0) both clamp to 0
+    a = fmaxf(a, 1.f);
+    return make_float2(-p.k2 * ((a - 1.f) * (a + 1.f) - b * b), -p.k2 * (2.f * a * b));
+}
+
 __global__ void k_est_n(const float2* __restrict__ f, float2* __restrict__ n, long long cnt, Proj p)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -527,7 +545,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_x(float2* __restrict__ W,
     const long long orow = ((long long)((tl.px + h) % V) + (long long)V * ((tl.py + h) % V)) * V;
     auto put_n = [&](int x, float2 v) { out[orow + (x + h) % V] = v; };
     if constexpr (MODE == 0) {
-        auto gl = [&](int x) -> float2 { return modify_f(est_n(row[x], p), p); };
+        auto gl = [&](int x) -> float2 { return project(row[x], p, 1.f); };
         auto gs = [&](int k, float2 v) { row[k] = v; };
         Plan<V>::template run<-1>(tl.t, s, twp, gl, gs);
     } else if constexpr (MODE == 3) {
@@ -535,7 +553,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_x(float2* __restrict__ W,
     } else {
         auto gl = [&](int k) -> float2 { return row[k]; };
         if constexpr (MODE == 1) {
-            auto gs = [&](int x, float2 v) { s[lds_pad(x)] = modify_f(est_n(cscale(v, scale), p), p); };
+            auto gs = [&](int x, float2 v) { s[lds_pad(x)] = project(v, p, scale); };
             Plan<V>::template run<+1>(tl.t, s, twp, gl, gs);
             __syncthreads();
             auto gl2 = [&](int x) -> float2 { return s[lds_pad(x)]; };
