@@ -312,7 +312,8 @@ void normalize_measured(Ctx& c, float2* F, const float* hits, float2* Fm, int V,
 // ---------------------------------------------------------------- 3-D FFT
 static void check_vol(int V)
 {
-    CTDM_REQUIRE(V == 32 || V == 64 || V == 128 || V == 192 || V == 256 || V == 512, "volume side has no kernel");
+    if (!(V == 32 || V == 64 || V == 128 || V == 192 || V == 256 || V == 512))
+        fail(CTDM_EUNSUPPORTED, "volume side " + std::to_string(V) + " has no kernel");
 }
 
 // x, y, z passes; `scale` folded into the z pass.
