@@ -4,11 +4,13 @@
 //
 //   1. k_gs_residues  one thread per pixel: the reference's wrapped differences with its float /
 //                     double promotions, summed in float, thresholded at +-6.
-//   2. k_gs_cuts      one wavefront per map: residues in raster order (ballot over packed charge
-//                     words), each box radius's perimeter evaluated by 64 lanes at once in the
-//                     reference's scan order, the first hit (ballot) decides the cut. The reference
-//                     runs this search concurrently for all residues (racy); the raster order makes
-//                     it deterministic. Charges live in LDS, 2 bits per pixel.
+//   2. branch cuts, deterministic raster order (the reference runs its search for all residues
+//      concurrently, which is racy; the raster order makes it reproducible). Three phases:
+//      k_gs_cand lists every residue's first hits on the initial charges in parallel, k_gs_commit (one
+//      wavefront per map, charges in LDS at 2 bits per pixel) walks the residues in order and takes the
+//      first listed hit still present, and k_gs_draw draws the chosen cuts in parallel. Maps whose
+//      charges do not fit LDS use k_gs_cuts: the sequential search with each box radius's perimeter
+//      evaluated by 64 lanes at once and the charge words in global memory.
 //   3. k_gs_bfs       one workgroup per map: level-synchronous breadth-first flood fill over
 //                     non-cut pixels; after a level is discovered each new pixel takes k from its
 //                     first neighbour of the previous level in the order right, left, below, above.
@@ -66,21 +68,13 @@ __global__ void k_gs_residues(const float* __restrict__ phi, int batch, int H, i
 // Packed charges: 2 bits per pixel, 16 pixels per word; 0 = none, 1 = +1, 3 = -1.
 __device__ __forceinline__ int unpack(unsigned w, int p) { return (int)((w >> ((p & 15) * 2)) & 3u); }
 
-template <bool LDSW>
+// Charge words in global memory, changed by one wavefront only (relaxed atomics keep its reads current).
 struct Words {
     unsigned* w;
-    __device__ unsigned get(int i) const
-    {
-        if (LDSW) return w[i];
-        return __hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
+    __device__ unsigned get(int i) const { return __hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
     __device__ void clear(int p) const  // caller: one lane
     {
-        const unsigned m = ~(3u << ((p & 15) * 2));
-        if (LDSW)
-            w[p >> 4] &= m;
-        else
-            __hip_atomic_fetch_and(w + (p >> 4), m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_and(w + (p >> 4), ~(3u << ((p & 15) * 2)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     __device__ int charge(int p) const
     {
@@ -108,17 +102,15 @@ __device__ void gs_place_cut(uint8_t* cut, int W, int r1, int c1, int r2, int c2
     }
 }
 
-template <bool LDSW>
 __global__ __launch_bounds__(64) void k_gs_cuts(const int8_t* __restrict__ res, int H, int W,
                                                 unsigned* __restrict__ gwords, uint8_t* __restrict__ cuts)
 {
-    extern __shared__ unsigned lwords[];
     const int lane = threadIdx.x;
     const long long n = (long long)H * W;
     const int nw = (int)((n + 15) / 16);
     const int8_t* r = res + blockIdx.x * n;
     uint8_t* cut = cuts + blockIdx.x * n;
-    Words<LDSW> ch{LDSW ? lwords : gwords + (long long)blockIdx.x * nw};
+    Words ch{gwords + (long long)blockIdx.x * nw};
     for (int w = lane; w < nw; w += 64) {
         unsigned v = 0;
         for (int k = 0; k < 16; ++k) {
@@ -209,67 +201,134 @@ __global__ __launch_bounds__(64) void k_gs_cuts(const int8_t* __restrict__ res, 
 // and needs no workgroup barrier (one wavefront: its LDS operations complete in issue order, and every
 // branch below is uniform across the wave). The winner of a perimeter scan is read with readlane, and
 // charges are cleared with LDS atomics whose result nothing waits for.
-__device__ __forceinline__ void cut_bit(unsigned* cw, long long p) { atomicOr(cw + (p >> 5), 1u << (p & 31)); }
+// ---------------------------------------------------------------- 2b. branch cuts in three phases
+// The same result as the raster-order search, with most of the work in parallel. Charges only ever
+// disappear, and a border point is always a hit, so the first hit of a residue's scan at the time it is
+// processed is the first point of its scan that was a hit initially and is still present. Hence:
+//   1. every residue (in parallel) lists the first GS_K hits of its scan on the initial charges, stopping
+//      at a border point (always valid) or when the box radius runs out;
+//   2. one wavefront per map walks the residues in raster order and takes the first listed hit that is
+//      still present; only when all GS_K listed charges are gone does it scan again (a full rescan from
+//      radius 1 finds the same point as continuing the scan: the earlier points are all non-hits now);
+//   3. the chosen cuts are drawn in parallel (cut pixels are only ever set to 1).
+constexpr int GS_K = 4;
+constexpr int GS_META_SELF = 8;        // residue on the border: balanced by a one-pixel cut
+constexpr int GS_META_TERM = 16;       // the list ends with a border point
+constexpr int GS_META_EXHAUSTED = 32;  // the scan reached the largest box without GS_K hits or a border
 
-__device__ void gs_place_cut_lds(unsigned* cw, int W, int r1, int c1, int r2, int c2, int lane)
+// Residues per block of 1024 pixels.
+__global__ __launch_bounds__(1024) void k_gs_count(const int8_t* __restrict__ res, long long n, int nblk,
+                                                   int* __restrict__ blkcnt)
 {
-    if (lane == 0) {
-        cut_bit(cw, r1 + (long long)W * c1);
-        cut_bit(cw, r2 + (long long)W * c2);
-    }
-    const float dR = (float)(r2 - r1), dC = (float)(c2 - c1);
-    const float rad = __fsqrt_rn(__fadd_rn(__fmul_rn(dR, dR), __fmul_rn(dC, dC)));
-    const float ct = __fdiv_rn(dR, rad), st = __fdiv_rn(dC, rad);
-    const int n = (int)rintf(rad);
-    const int rlo = min(r1, r2), rhi = max(r1, r2), clo = min(c1, c2), chi = max(c1, c2);
-    for (int ii = 1 + lane; ii <= n; ii += 64) {
-        const int rf = r1 + (int)rintf(__fmul_rn((float)ii, ct));
-        const int cf = c1 + (int)rintf(__fmul_rn((float)ii, st));
-        if (rf >= rlo && rf <= rhi && cf >= clo && cf <= chi) cut_bit(cw, rf + (long long)W * cf);
+    __shared__ int wc[16];
+    const long long map = blockIdx.x / nblk, blk = blockIdx.x % nblk;
+    const long long p = blk * 1024 + threadIdx.x;
+    const bool q = p < n && res[map * n + p] != 0;
+    const unsigned long long m = __ballot(q);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < 16; ++w) t += wc[w];
+        blkcnt[blockIdx.x] = t;
     }
 }
 
-// Perimeter points of box radii 1, 2 and 3 in the search's scan order (first column top to bottom,
-// inner columns top then bottom, last column top to bottom), as (dx, dy): lanes 0-7 ring 1, 8-23
-// ring 2, 24-47 ring 3. Evaluating the three rings in one ballot keeps the order of the sequential
-// search: the lowest lane with a hit is the first hit of the smallest radius.
-struct Rings3 {
-    int8_t dx[48], dy[48];
-    constexpr Rings3() : dx(), dy()
-    {
-        int k = 0;
-        for (int r = 1; r <= 3; ++r)
-            for (int x = -r; x <= r; ++x) {
-                if (x == -r || x == r) {
-                    for (int y = -r; y <= r; ++y) {
-                        dx[k] = (int8_t)x;
-                        dy[k] = (int8_t)y;
-                        ++k;
+// Exclusive scan of the block counts of one map (one thread per map; a few hundred blocks).
+__global__ void k_gs_scan(int* __restrict__ blkcnt, int nblk, int batch, int* __restrict__ nres)
+{
+    const int map = blockIdx.x * blockDim.x + threadIdx.x;
+    if (map >= batch) return;
+    int acc = 0;
+    for (int b = 0; b < nblk; ++b) {
+        const int v = blkcnt[(long long)map * nblk + b];
+        blkcnt[(long long)map * nblk + b] = acc;
+        acc += v;
+    }
+    nres[map] = acc;
+}
+
+// Phase 1: one thread per pixel; residues write their record at their raster rank. A listed hit is
+// encoded as kind << 30 | target pixel (kind 1: column border, cut along the row; 2: row border, cut
+// along the column; 3: opposite charge), kind 0 with the residue's own pixel for a border residue.
+__global__ __launch_bounds__(1024) void k_gs_cand(const int8_t* __restrict__ res, int H, int W, int nblk,
+                                                  const int* __restrict__ blkoff, int2* __restrict__ rec_pm,
+                                                  int4* __restrict__ rec_c)
+{
+    __shared__ int wc[16];
+    const long long n = (long long)H * W;
+    const long long map = blockIdx.x / nblk, blk = blockIdx.x % nblk;
+    const long long p = blk * 1024 + threadIdx.x;
+    const int8_t* r = res + map * n;
+    const int q = p < n ? r[p] : 0;
+    const unsigned long long m = __ballot(q != 0);
+    const int lane = threadIdx.x & 63;
+    if (lane == 0) wc[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (q == 0) return;
+    int rank = blkoff[blockIdx.x] + __popcll(m & ((1ull << lane) - 1ull));
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) rank += wc[w];
+    const int i = (int)(p % W), j = (int)(p / W);
+    int cand[GS_K] = {0, 0, 0, 0};
+    int cnt = 0, meta = 0;
+    if (i <= 0 || i >= W - 1 || j <= 0 || j >= H - 1) {
+        meta = GS_META_SELF;
+    } else {
+        const int R = min(901, W / 2);
+        bool stop = false;
+        int rr = 1;
+        for (; rr < R && !stop; ++rr) {
+            const int m1 = max(i - rr, 0), m2 = min(i + rr, W - 1);
+            const int n1 = max(j - rr, 0), n2 = min(j + rr, H - 1);
+            for (int mm = m1; mm <= m2 && !stop; ++mm) {
+                const int ndel = (mm == m1 || mm == m2) ? 1 : (n2 - n1);
+                for (int nn = n1; nn <= n2 && !stop; nn += ndel) {
+                    unsigned c = 0;
+                    bool hit = true;
+                    if (mm <= 0 || mm >= W - 1) {
+                        c = (1u << 30) | (unsigned)(mm + W * j);
+                        meta |= GS_META_TERM;
+                    } else if (nn <= 0 || nn >= H - 1) {
+                        c = (2u << 30) | (unsigned)(i + W * nn);
+                        meta |= GS_META_TERM;
+                    } else if (q * r[mm + (long long)W * nn] == -1) {
+                        c = (3u << 30) | (unsigned)(mm + W * nn);
+                    } else {
+                        hit = false;
                     }
-                } else {
-                    dx[k] = (int8_t)x;
-                    dy[k] = (int8_t)-r;
-                    ++k;
-                    dx[k] = (int8_t)x;
-                    dy[k] = (int8_t)r;
-                    ++k;
+                    if (hit) {
+                        cand[cnt++] = (int)c;
+                        stop = (meta & GS_META_TERM) || cnt == GS_K;
+                    }
                 }
             }
+        }
+        if (!stop) meta |= GS_META_EXHAUSTED;
     }
-};
-__constant__ constexpr Rings3 kRings3{};
+    const long long o = map * n + rank;
+    rec_pm[o] = make_int2((int)p, meta | cnt);
+    rec_c[o] = make_int4(cand[0], cand[1], cand[2], cand[3]);
+}
 
-__global__ __launch_bounds__(64) void k_gs_cuts_lds(const int8_t* __restrict__ res, int H, int W,
-                                                    uint8_t* __restrict__ cuts)
+__device__ __forceinline__ int sel4(const int4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
+
+// Phase 2: one wavefront per map, residues in raster order. Charges in LDS, 2 bits per pixel. Residues
+// are taken 64 at a time, one per lane: each lane reads, once, whether its residue and its listed hits are
+// still present; the residues are then committed in order from those flags, and after each commit every
+// lane clears the flags of the pixels that commit removed (a few vector compares), so the loop itself
+// waits for no memory. The LDS copy receives the same clears for later batches and rescans.
+__device__ __forceinline__ int charge_of(const unsigned* chw, int pix) { return (int)((chw[pix >> 4] >> ((pix & 15) * 2)) & 3u); }
+
+__global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res, int H, int W,
+                                                  const int* __restrict__ nres, const int2* __restrict__ rec_pm,
+                                                  const int4* __restrict__ rec_c, int2* __restrict__ pairs,
+                                                  int* __restrict__ npairs)
 {
-    extern __shared__ unsigned sm[];
+    extern __shared__ unsigned chw[];
     const int lane = threadIdx.x;
     const long long n = (long long)H * W;
-    const int nw = (int)((n + 15) / 16), nc = (int)((n + 31) / 32);
-    unsigned* chw = sm;       // charges, 2 bits per pixel
-    unsigned* cw = sm + nw;   // cut mask, 1 bit per pixel
+    const int nw = (int)((n + 15) / 16);
     const int8_t* r = res + blockIdx.x * n;
-    uint8_t* cut = cuts + blockIdx.x * n;
     for (int w = lane; w < nw; w += 64) {
         unsigned v = 0;
         for (int k = 0; k < 16; ++k) {
@@ -279,89 +338,71 @@ __global__ __launch_bounds__(64) void k_gs_cuts_lds(const int8_t* __restrict__ r
         }
         chw[w] = v;
     }
-    for (int w = lane; w < nc; w += 64) cw[w] = 0u;
     __syncthreads();
+    const int total = nres[blockIdx.x];
+    const int2* pm = rec_pm + blockIdx.x * n;
+    const int4* rc = rec_c + blockIdx.x * n;
+    int2* out = pairs + blockIdx.x * n;
     const int R = min(901, W / 2);
-    for (int base = 0; base < nw; base += 64) {
-        const int myw = base + lane;
-        unsigned long long live = __ballot(myw < nw && chw[myw] != 0u);
-        while (live) {
-            const int l = __ffsll((long long)live) - 1;
-            live &= live - 1;
-            const int wi = base + l;
-            // Only this wave changes the charges, and only through the two clears in take(), so the
-            // word is read once and kept current in a register.
-            // readfirstlane: the value is the same in every lane; saying so keeps the residue loop scalar
-            unsigned wcur = (unsigned)__builtin_amdgcn_readfirstlane((int)chw[wi]);
-            const int j0 = (wi * 16) / W, i0 = wi * 16 - j0 * W;  // the word's first pixel
-            while (wcur != 0u) {
-                wcur = (unsigned)__builtin_amdgcn_readfirstlane((int)wcur);
-                const int bit = (__ffs((int)wcur) - 1) >> 1;
-                const int c2 = (int)((wcur >> (2 * bit)) & 3u);
-                wcur &= ~(3u << (2 * bit));
-                const int p = wi * 16 + bit;
-                const int q = c2 == 1 ? 1 : -1;
-                int i = i0 + bit, j = j0;
-                while (i >= W) {
-                    i -= W;
-                    ++j;
+    int emitted = 0;
+    int2 nxt_pm = lane < total ? pm[lane] : make_int2(0, 0);
+    int4 nxt_c = lane < total ? rc[lane] : make_int4(0, 0, 0, 0);
+    for (int base = 0; base < total; base += 64) {
+        const int2 mypm = nxt_pm;
+        const int4 myc = nxt_c;
+        const int nidx = base + 64 + lane;  // the next batch's records load while this one commits
+        nxt_pm = nidx < total ? pm[nidx] : make_int2(0, 0);
+        nxt_c = nidx < total ? rc[nidx] : make_int4(0, 0, 0, 0);
+        const int cnt_batch = min(64, total - base);
+        // presence flags: bit 0 the residue, bit 1 + k its k-th listed hit (border hits never go away)
+        const int cn = mypm.y & 7;
+        int valid = 0;
+        int cpix[GS_K];
+        if (lane < cnt_batch) {
+            if (charge_of(chw, mypm.x)) valid |= 1;
+#pragma unroll
+            for (int k = 0; k < GS_K; ++k) {
+                const unsigned c = (unsigned)sel4(myc, k);
+                cpix[k] = (int)(c & 0x3fffffffu);
+                if (k < cn) {
+                    if ((c >> 30) != 3u || charge_of(chw, cpix[k])) valid |= 2 << k;
                 }
-                if (i <= 0 || i >= W - 1 || j <= 0 || j >= H - 1) {
-                    if (lane == 0) {
-                        cut_bit(cw, p);
-                        atomicAnd(chw + wi, ~(3u << (2 * bit)));
-                    }
-                    continue;
-                }
-                auto take = [&](int kind, int mm, int nn) {  // wave-uniform winner: cut and balance
-                    if (kind == 1)
-                        gs_place_cut_lds(cw, W, i, j, mm, j, lane);
-                    else if (kind == 2)
-                        gs_place_cut_lds(cw, W, i, j, i, nn, lane);
-                    else
-                        gs_place_cut_lds(cw, W, i, j, mm, nn, lane);
-                    if (lane == 0) atomicAnd(chw + wi, ~(3u << (2 * bit)));
-                    if (kind == 3) {
-                        const int pp = mm + W * nn;
-                        if (lane == 0) atomicAnd(chw + (pp >> 4), ~(3u << ((pp & 15) * 2)));
-                        if ((pp >> 4) == wi) wcur &= ~(3u << ((pp & 15) * 2));
-                    }
-                };
-                // kind of one perimeter point: 1 column border, 2 row border, 3 opposite charge
-                auto classify = [&](int mm, int nn) -> int {
-                    if (mm <= 0 || mm >= W - 1) return 1;
-                    if (nn <= 0 || nn >= H - 1) return 2;
-                    const int pp = mm + W * nn;
-                    const int cc = unpack(chw[pp >> 4], pp);
-                    return q * (cc == 1 ? 1 : (cc == 3 ? -1 : 0)) == -1 ? 3 : 0;
-                };
-                bool done = false;
-                int rstart = 1;
-                if (R > 3 && i >= 3 && i <= W - 4 && j >= 3 && j <= H - 4) {  // radii 1-3 unclamped
-                    int kind = 0, mm = 0, nn = 0;
-                    if (lane < 48) {
-                        mm = i + kRings3.dx[lane];
-                        nn = j + kRings3.dy[lane];
-                        kind = classify(mm, nn);
-                    }
-                    const unsigned long long hit = __ballot(kind != 0);
-                    if (hit) {
-                        const int src = __ffsll((long long)hit) - 1;  // uniform: read the winner's lane
-                        take(__builtin_amdgcn_readlane(kind, src), __builtin_amdgcn_readlane(mm, src),
-                             __builtin_amdgcn_readlane(nn, src));
-                        done = true;
-                    }
-                    rstart = 4;
-                }
-                for (int rr = rstart; rr < R && !done; ++rr) {
+                if ((c >> 30) != 3u) cpix[k] = -1;  // not a charge: no clear can touch it
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < GS_K; ++k) cpix[k] = -1;
+        }
+        for (int l = 0; l < cnt_batch; ++l) {
+            const int v = __builtin_amdgcn_readlane(valid, l);
+            if (!(v & 1)) continue;  // balanced earlier as another residue's partner
+            const int p = __builtin_amdgcn_readlane(mypm.x, l);
+            const int meta = __builtin_amdgcn_readlane(mypm.y, l);
+            int chosen = 0;  // encoded choice; kinds 2 and 3 set the sign bit, so `found` tracks it
+            bool found = false;
+            if (meta & GS_META_SELF) {
+                chosen = p;  // kind 0
+                found = true;
+            } else if (v & 0x1e) {
+                const int k = __ffs(v >> 1) - 1;
+                chosen = __builtin_amdgcn_readlane(sel4(myc, k), l);
+                found = true;
+            } else if (!(meta & (GS_META_TERM | GS_META_EXHAUSTED))) {
+                // every listed charge is gone: scan again on the current charges
+                const int cp2 = charge_of(chw, p);
+                const int q = cp2 == 1 ? 1 : -1;
+                const int i = p % W, j = p / W;
+                for (int rr = 1; rr < R && !found; ++rr) {
                     const int m1 = max(i - rr, 0), m2 = min(i + rr, W - 1);
                     const int n1 = max(j - rr, 0), n2 = min(j + rr, H - 1);
                     const int nrow = n2 - n1 + 1, ncol = m2 - m1 + 1;
                     const int P = 2 * nrow + 2 * (ncol - 2);
-                    for (int q0 = 0; q0 < P && !done; q0 += 64) {
+                    for (int q0 = 0; q0 < P && !found; q0 += 64) {
                         const int qi = q0 + lane;
-                        int kind = 0, mm = 0, nn = 0;
+                        int c = 0;
+                        bool hit_here = false;
                         if (qi < P) {
+                            int mm, nn;
                             if (qi < nrow) {
                                 mm = m1;
                                 nn = n1 + qi;
@@ -373,22 +414,75 @@ __global__ __launch_bounds__(64) void k_gs_cuts_lds(const int8_t* __restrict__ r
                                 mm = m2;
                                 nn = n1 + (qi - nrow - 2 * (ncol - 2));
                             }
-                            kind = classify(mm, nn);
+                            if (mm <= 0 || mm >= W - 1) {
+                                c = (int)((1u << 30) | (unsigned)(mm + W * j));
+                                hit_here = true;
+                            } else if (nn <= 0 || nn >= H - 1) {
+                                c = (int)((2u << 30) | (unsigned)(i + W * nn));
+                                hit_here = true;
+                            } else {
+                                const int pp = mm + W * nn;
+                                const int cc = charge_of(chw, pp);
+                                if (q * (cc == 1 ? 1 : (cc == 3 ? -1 : 0)) == -1) {
+                                    c = (int)((3u << 30) | (unsigned)pp);
+                                    hit_here = true;
+                                }
+                            }
                         }
-                        const unsigned long long hit = __ballot(kind != 0);
+                        const unsigned long long hit = __ballot(hit_here);
                         if (hit) {
-                            const int src = __ffsll((long long)hit) - 1;
-                            take(__builtin_amdgcn_readlane(kind, src), __builtin_amdgcn_readlane(mm, src),
-                                 __builtin_amdgcn_readlane(nn, src));
-                            done = true;
+                            chosen = __builtin_amdgcn_readlane(c, __ffsll((long long)hit) - 1);
+                            found = true;
                         }
                     }
                 }
             }
+            if (!found) continue;  // nothing within reach: the residue stays unbalanced
+            const int pix = ((unsigned)chosen >> 30) == 3u ? (chosen & 0x3fffffff) : -1;
+            if (lane == 0) {
+                out[emitted] = make_int2(p, chosen);
+                atomicAnd(chw + (p >> 4), ~(3u << ((p & 15) * 2)));
+                if (pix >= 0) atomicAnd(chw + (pix >> 4), ~(3u << ((pix & 15) * 2)));
+            }
+            ++emitted;
+            // the batch's flags follow the two clears
+            if (mypm.x == p || (pix >= 0 && mypm.x == pix)) valid &= ~1;
+#pragma unroll
+            for (int k = 0; k < GS_K; ++k)
+                if (cpix[k] >= 0 && (cpix[k] == p || cpix[k] == pix)) valid &= ~(2 << k);
         }
     }
-    __syncthreads();
-    for (long long p = lane; p < n; p += 64) cut[p] = (uint8_t)((cw[p >> 5] >> (p & 31)) & 1u);
+    if (lane == 0) npairs[blockIdx.x] = emitted;
+}
+
+// Phase 3: one thread per chosen cut; cut pixels are only set, so the order does not matter.
+__global__ void k_gs_draw(const int2* __restrict__ pairs, const int* __restrict__ npairs, int H, int W,
+                          uint8_t* __restrict__ cuts)
+{
+    const long long n = (long long)H * W;
+    const long long map = blockIdx.y;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npairs[map]) return;
+    const int2 pr = pairs[map * n + k];
+    uint8_t* cut = cuts + map * n;
+    const int p = pr.x, kind = (int)((unsigned)pr.y >> 30), t = pr.y & 0x3fffffff;
+    if (kind == 0) {
+        cut[p] = 1;
+        return;
+    }
+    const int r1 = p % W, c1 = p / W, r2 = t % W, c2 = t / W;
+    cut[p] = 1;
+    cut[t] = 1;
+    const float dR = (float)(r2 - r1), dC = (float)(c2 - c1);
+    const float rad = __fsqrt_rn(__fadd_rn(__fmul_rn(dR, dR), __fmul_rn(dC, dC)));
+    const float ct = __fdiv_rn(dR, rad), st = __fdiv_rn(dC, rad);
+    const int nseg = (int)rintf(rad);
+    const int rlo = min(r1, r2), rhi = max(r1, r2), clo = min(c1, c2), chi = max(c1, c2);
+    for (int ii = 1; ii <= nseg; ++ii) {
+        const int rf = r1 + (int)rintf(__fmul_rn((float)ii, ct));
+        const int cf = c1 + (int)rintf(__fmul_rn((float)ii, st));
+        if (rf >= rlo && rf <= rhi && cf >= clo && cf <= chi) cut[rf + (long long)W * cf] = 1;
+    }
 }
 
 // ---------------------------------------------------------------- 3. flood fill (BFS levels)
@@ -635,18 +729,28 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
     hipLaunchKernelGGL(k_gs_residues, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, batch, H, W, r);
     const long long nw = (n + 15) / 16, nb = (n + 31) / 32;
     const size_t lds_max = (size_t)(c.max_lds > 0 ? c.max_lds : 64 * 1024);
-    const size_t cuts_lds = sizeof(unsigned) * (size_t)(nw + nb);
-    if (cuts_lds <= lds_max) {
-        hipLaunchKernelGGL(k_gs_cuts_lds, dim3(batch), dim3(64), cuts_lds, c.stream, r, H, W, ct);
-    } else {
+    const size_t commit_lds = sizeof(unsigned) * (size_t)nw;
+    if (commit_lds <= lds_max) {  // three phases: candidates in parallel, raster-order commit, drawing
+        const int nblk = (int)((n + 1023) / 1024);
+        int* blk = (int*)c.buf("gs.blk", sizeof(int) * (size_t)nblk * batch);
+        int* nres = (int*)c.buf("gs.nres", sizeof(int) * (size_t)batch);
+        int* npairs = (int*)c.buf("gs.npairs", sizeof(int) * (size_t)batch);
+        int2* rec_pm = (int2*)c.buf("gs.recpm", sizeof(int2) * (size_t)tot);
+        int4* rec_c = (int4*)c.buf("gs.recc", sizeof(int4) * (size_t)tot);
+        int2* pairs = (int2*)c.buf("gs.pairs", sizeof(int2) * (size_t)tot);
+        hipLaunchKernelGGL(k_gs_count, dim3((unsigned)(nblk * batch)), dim3(1024), 0, c.stream, r, n, nblk, blk);
+        hipLaunchKernelGGL(k_gs_scan, dim3((batch + 63) / 64), dim3(64), 0, c.stream, blk, nblk, batch, nres);
+        hipLaunchKernelGGL(k_gs_cand, dim3((unsigned)(nblk * batch)), dim3(1024), 0, c.stream, r, H, W, nblk, blk,
+                           rec_pm, rec_c);
+        hipLaunchKernelGGL(k_gs_commit, dim3(batch), dim3(64), commit_lds, c.stream, r, H, W, nres, rec_pm, rec_c,
+                           pairs, npairs);
         hipLaunchKernelGGL(k_gs_fill_u8, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, ct, tot);
-        const size_t lds = sizeof(unsigned) * (size_t)nw;
-        if (lds <= lds_max) {
-            hipLaunchKernelGGL((k_gs_cuts<true>), dim3(batch), dim3(64), lds, c.stream, r, H, W, nullptr, ct);
-        } else {
-            unsigned* gw = (unsigned*)c.buf("gs.words", sizeof(unsigned) * (size_t)nw * batch);
-            hipLaunchKernelGGL((k_gs_cuts<false>), dim3(batch), dim3(64), 0, c.stream, r, H, W, gw, ct);
-        }
+        hipLaunchKernelGGL(k_gs_draw, dim3((unsigned)((n + 255) / 256), (unsigned)batch), dim3(256), 0, c.stream,
+                           pairs, npairs, H, W, ct);
+    } else {  // charges beyond LDS: sequential search over charge words in global memory
+        hipLaunchKernelGGL(k_gs_fill_u8, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, ct, tot);
+        unsigned* gw = (unsigned*)c.buf("gs.words", sizeof(unsigned) * (size_t)nw * batch);
+        hipLaunchKernelGGL(k_gs_cuts, dim3(batch), dim3(64), 0, c.stream, r, H, W, gw, ct);
     }
     CTDM_CHECK_LAUNCH("k_gs_cuts");
     const size_t bfs_lds = sizeof(unsigned) * (size_t)(4 * nb);
