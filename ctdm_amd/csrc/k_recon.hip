@@ -88,6 +88,28 @@ __device__ __forceinline__ int signed_freq(int n, int V) { return n < V / 2 ? n 
 // LDS chunk of per-pencil samples staged at a time by the gridding and fit kernels.
 constexpr int GRID_CHUNK = 64;
 
+// Entries of a staged chunk that fall in a thread's z range: bit k is set when entry k < n has its z
+// index in [z0, z0 + ZR). The chunk's z indices (row: GRID_CHUNK shorts, 16-byte aligned) are read 16
+// bytes at a time and tested in registers, so a thread only touches LDS for its own entries.
+template <int ZR>
+__device__ __forceinline__ unsigned long long chunk_mask(const short* row, int n, int z0)
+{
+    unsigned long long m = 0;
+    const uint4* r4 = reinterpret_cast<const uint4*>(row);
+#pragma unroll
+    for (int q = 0; q < GRID_CHUNK / 8; ++q) {
+        const uint4 w4 = r4[q];
+        const unsigned wd[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            const int k = 8 * q + h;
+            const int z = (short)(wd[h >> 1] >> (16 * (h & 1)));
+            if (k < n && (unsigned)(z - z0) < (unsigned)ZR) m |= 1ull << k;
+        }
+    }
+    return m;
+}
+
 // Device copy of the per-angle rows (fx0, fy0, fz0).
 static const double* upload_rows(Ctx& c, const std::string& name, const std::vector<AngleRow>& rows)
 {
@@ -669,7 +691,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__
     __shared__ float2 twl[LdsSize<V>::value];
     __shared__ float2 racc[B][V + 1];
     __shared__ unsigned short hit[B][V + 2];
-    __shared__ short lnz[B][GRID_CHUNK + 2];
+    __shared__ __align__(16) short lnz[B][GRID_CHUNK + 8];  // rows 16-byte aligned for chunk_mask
     __shared__ float2 lval[B][GRID_CHUNK + 1];
     stage_twiddles<V>(twl, tw);
     const TwPad twp{twl};
@@ -694,14 +716,15 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__
             lval[tl.b][k] = ml.val[pencil * ml.cap + e0 + k];
         }
         __syncthreads();
-        for (int k = 0; k < ne; ++k) {
+        unsigned long long mine = chunk_mask<ZR>(&lnz[tl.b][0], ne, z0);  // this thread's entries, in order
+        while (mine) {
+            const int k = __ffsll((long long)mine) - 1;
+            mine &= mine - 1;
             const int z = lnz[tl.b][k];
-            if (z >= z0 && z < z0 + ZR) {
-                const float2 F = s[lds_pad(z)], m = lval[tl.b][k];
-                racc[tl.b][z].x += F.x - m.x;
-                racc[tl.b][z].y += F.y - m.y;
-                hit[tl.b][z] += 1;
-            }
+            const float2 F = s[lds_pad(z)], m = lval[tl.b][k];
+            racc[tl.b][z].x += F.x - m.x;
+            racc[tl.b][z].y += F.y - m.y;
+            hit[tl.b][z] += 1;
         }
         __syncthreads();
     }
@@ -776,7 +799,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
     constexpr int LS = LdsStride<V, B>::value;
     __shared__ float2 lds[B * LS];
     __shared__ unsigned short hit[B][V + 2];
-    __shared__ short lnz[B][GRID_CHUNK + 2];
+    __shared__ __align__(16) short lnz[B][GRID_CHUNK + 8];  // rows 16-byte aligned for the wide reads
     __shared__ float2 lval[B][GRID_CHUNK + 1];
     __shared__ int lcnt[B][T + 1];
     __shared__ float2 twl[LdsSize<V>::value];
@@ -809,15 +832,17 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
             lval[tl.b][k] = val;
         }
         __syncthreads();
-        for (int k = 0; k < na; ++k) {
+        // this thread's entries, in angle order
+        unsigned long long mine = chunk_mask<ZR>(&lnz[tl.b][0], na, z0);
+        while (mine) {
+            const int k = __ffsll((long long)mine) - 1;
+            mine &= mine - 1;
             const int nz = lnz[tl.b][k];
-            if (nz >= z0 && nz < z0 + ZR) {
-                float2& acc = s[lds_pad(nz)];
-                const float2 v = lval[tl.b][k];
-                acc.x += v.x;
-                acc.y += v.y;
-                hit[tl.b][nz] += 1;
-            }
+            float2& acc = s[lds_pad(nz)];
+            const float2 v = lval[tl.b][k];
+            acc.x += v.x;
+            acc.y += v.y;
+            hit[tl.b][nz] += 1;
         }
         __syncthreads();
     }

@@ -49,6 +49,9 @@ __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
     constexpr int NH = W / 2;
     constexpr int T = Plan<NH>::T;
     __shared__ float2 lds[G * LdsSize<NH>::value];
+    __shared__ float2 twl[LdsSize<NH>::value];
+    stage_twiddles<NH>(twl, tw_half);
+    const TwPad twp{twl};
     const int g = threadIdx.x / T, t = threadIdx.x % T;
     const long long row = (long long)blockIdx.x * G + g;  // over nimg*H rows
     const long long nrows = (long long)nimg * H;
@@ -60,7 +63,7 @@ __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
     float2* s = lds + g * LdsSize<NH>::value;
     auto gl = [&](int i) -> float2 { return valid ? src2[i] : make_float2(0.f, 0.f); };
     auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };
-    Plan<NH>::template run<-1>(t, s, tw_half, gl, gs);
+    Plan<NH>::template run<-1>(t, s, twp, gl, gs);
     __syncthreads();
     if (!valid) return;
     float2* dst = out + row * nout;
