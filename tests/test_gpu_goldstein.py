@@ -65,3 +65,37 @@ def test_goldstein_large_map_global_path(ctx, orc):
     w = ctdm_amd.synth_wrapped(5, 2, 1024, 1024)
     tg, tc, rr = _check(ctx, orc, w)
     print(f"2 x 1024^2: GPU {tg * 1e3:.1f} ms, oracle {tc:.2f} s")
+
+
+def test_goldstein_no_residues(ctx, orc):
+    """A smooth ramp has no residues: no cuts, every pixel reached, unwrapped equal to the ramp."""
+    H, W = 48, 70
+    y, x = np.mgrid[0:H, 0:W].astype(np.float64)
+    phi = 0.3 * x + 0.2 * y
+    w = np.angle(np.exp(1j * phi)).astype(np.float32)[None]
+    _, _, rr = _check(ctx, orc, w)
+    assert (rr != 0).sum() == 0
+
+
+def test_goldstein_maximum_residue_density(ctx, orc):
+    """Uniform random phase (residue density about 1/3): most listed hits are consumed before a residue
+    is reached, so the commit's rescans run constantly."""
+    rng = np.random.default_rng(11)
+    w = rng.uniform(-np.pi, np.pi, size=(4, 96, 120)).astype(np.float32)
+    _, _, rr = _check(ctx, orc, w)
+    assert (rr != 0).mean() > 0.2
+
+
+def test_goldstein_thin_and_narrow_maps(ctx, orc):
+    """Maps at the border limits: 3 rows or 3 columns, and a 9-wide tall map whose box radius
+    (min(901, W/2) = 4) never reaches a border from the middle column, so an isolated vortex there
+    exhausts its search and stays unbalanced."""
+    rng = np.random.default_rng(12)
+    for shape in [(2, 3, 40), (2, 40, 3), (1, 3, 3)]:
+        _check(ctx, orc, rng.uniform(-np.pi, np.pi, size=shape).astype(np.float32))
+    H, W = 200, 9
+    y, x = np.mgrid[0:H, 0:W].astype(np.float64)
+    phi = np.arctan2(y - 100.5, x - 4.5)
+    w = np.angle(np.exp(1j * phi)).astype(np.float32)[None]
+    _, _, rr = _check(ctx, orc, w)
+    assert (rr != 0).sum() >= 1
