@@ -373,9 +373,13 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
 #pragma unroll
             for (int k = 0; k < GS_K; ++k) cpix[k] = -1;
         }
-        for (int l = 0; l < cnt_batch; ++l) {
+        // lanes whose residue is still present, in raster order; residues balanced as partners of an
+        // earlier commit drop out of the mask without an iteration
+        unsigned long long live = __ballot(lane < cnt_batch && (valid & 1));
+        while (live) {
+            const int l = __ffsll((long long)live) - 1;
+            live &= l == 63 ? 0ull : (~0ull << (l + 1));
             const int v = __builtin_amdgcn_readlane(valid, l);
-            if (!(v & 1)) continue;  // balanced earlier as another residue's partner
             const int p = __builtin_amdgcn_readlane(mypm.x, l);
             const int meta = __builtin_amdgcn_readlane(mypm.y, l);
             int chosen = 0;  // encoded choice; kinds 2 and 3 set the sign bit, so `found` tracks it
@@ -450,6 +454,7 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
 #pragma unroll
             for (int k = 0; k < GS_K; ++k)
                 if (cpix[k] >= 0 && (cpix[k] == p || cpix[k] == pix)) valid &= ~(2 << k);
+            live &= __ballot(valid & 1);
         }
     }
     if (lane == 0) npairs[blockIdx.x] = emitted;
