@@ -49,6 +49,22 @@ void* Ctx::buf(const std::string& name, size_t bytes)
     return b.ptr;
 }
 
+unsigned long long* Ctx::host_words(int n)
+{
+    CTDM_REQUIRE(n > 0 && n <= 64, "host_words: at most 64 words");
+    if (!host_words_) CTDM_HIP(hipHostMalloc((void**)&host_words_, 64 * sizeof(unsigned long long), hipHostMallocDefault));
+    return host_words_;
+}
+
+void Ctx::spin_sync()
+{
+    for (;;) {
+        const hipError_t e = hipStreamQuery(stream);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady) CTDM_HIP(e);
+    }
+}
+
 }  // namespace ctdm
 
 namespace {
@@ -122,6 +138,7 @@ ctdm_status ctdm_destroy(ctdm_ctx* ctx)
     for (auto& kv : ctx->c.scratch)
         if (kv.second.ptr) hipFree(kv.second.ptr);
     for (void* p : ctx->c.retired) hipFree(p);
+    if (ctx->c.host_words_) hipHostFree(ctx->c.host_words_);
     delete ctx;
     return CTDM_OK;
 }

@@ -86,9 +86,16 @@ struct Ctx {
     int dc_V = 0;    // volume side the cached data-consistency list ("dc.*") was built for
     int dc_cap = 0;  // its entries per pencil
 
+    unsigned long long* host_words_ = nullptr;  // small pinned host buffer (hipHostMalloc)
+
     const float2* tw(int n);
     // Scratch buffer `name` of at least `bytes` bytes (contents undefined).
     void* buf(const std::string& name, size_t bytes);
+    // Pinned host words for small results read back from the device (at most 64; contents undefined).
+    unsigned long long* host_words(int n);
+    // Waits for the context stream by polling it: a host-synchronous search returns as soon as its last
+    // command has finished, without the wake-up latency of a blocking synchronisation.
+    void spin_sync();
     Tune& tuning(int n) { return tune[n]; }
 };
 

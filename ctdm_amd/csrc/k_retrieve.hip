@@ -191,12 +191,18 @@ __global__ void k_argmax_band(const float2* __restrict__ spec, int H, int W, int
                                        (0xffffffffull - (unsigned long long)(i + (long long)W * j));
         local = key > local ? key : local;
     }
-    // wave reduction then one atomic per wave
+    // wave reduction, then the block's waves through LDS: one atomic per block
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_xor(local, off);
         local = o > local ? o : local;
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(best, local);
+    __shared__ unsigned long long wmax[16];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) local = wmax[w] > local ? wmax[w] : local;
+        atomicMax(best, local);
+    }
 }
 
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
@@ -309,12 +315,16 @@ void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, 
     // Full spectrum of one real image: real row pass keeping every bin, then column pass.
     row_real(c, bg, 1, bg, 1, H, W, 0, W, spec);
     fft_cols(c, spec, H, W, W, 1, 0, -1, 1.f, false);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c.stream, best);
-    hipLaunchKernelGGL(k_argmax_band, dim3(256), dim3(256), 0, c.stream, spec, H, W, row_start, best);
+    // this search synchronises below and is never captured, so a memset may clear the result
+    CTDM_HIP(hipMemsetAsync(best, 0, sizeof(unsigned long long), c.stream));
+    const long long nband = (long long)(H - row_start) * W;
+    hipLaunchKernelGGL(k_argmax_band, dim3((unsigned)std::min<long long>(grid1d(nband, 256), 1024)), dim3(256), 0,
+                       c.stream, spec, H, W, row_start, best);
     CTDM_CHECK_LAUNCH("k_argmax_band");
-    unsigned long long h = 0;
-    CTDM_HIP(hipMemcpyAsync(&h, best, sizeof(h), hipMemcpyDeviceToHost, c.stream));
-    CTDM_HIP(hipStreamSynchronize(c.stream));
+    unsigned long long* hb = c.host_words(1);
+    CTDM_HIP(hipMemcpyAsync(hb, best, sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+    c.spin_sync();
+    const unsigned long long h = *hb;
     const long long idx = (long long)(0xffffffffull - (h & 0xffffffffull));
     out->row = (int)(idx / W);
     out->col = (int)(idx % W);
