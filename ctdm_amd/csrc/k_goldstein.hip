@@ -7,10 +7,9 @@
 //   2. branch cuts, deterministic raster order (the reference runs its search for all residues
 //      concurrently, which is racy; the raster order makes it reproducible). Three phases:
 //      k_gs_cand lists every residue's first hits on the initial charges in parallel, k_gs_commit (one
-//      wavefront per map, charges in LDS at 2 bits per pixel) walks the residues in order and takes the
-//      first listed hit still present, and k_gs_draw draws the chosen cuts in parallel. Maps whose
-//      charges do not fit LDS use k_gs_cuts: the sequential search with each box radius's perimeter
-//      evaluated by 64 lanes at once and the charge words in global memory.
+//      wavefront per map, charges at 2 bits per pixel, in LDS when they fit, else in global memory)
+//      walks the residues in order and takes the first listed hit still present, and k_gs_draw draws
+//      the chosen cuts in parallel.
 //   3. k_gs_bfs       one workgroup per map: level-synchronous breadth-first flood fill over
 //                     non-cut pixels; after a level is discovered each new pixel takes k from its
 //                     first neighbour of the previous level in the order right, left, below, above.
@@ -65,142 +64,30 @@ __global__ void k_gs_residues(const float* __restrict__ phi, int batch, int H, i
 }
 
 // ---------------------------------------------------------------- 2. branch cuts
-// Packed charges: 2 bits per pixel, 16 pixels per word; 0 = none, 1 = +1, 3 = -1.
-__device__ __forceinline__ int unpack(unsigned w, int p) { return (int)((w >> ((p & 15) * 2)) & 3u); }
-
-// Charge words in global memory, changed by one wavefront only (relaxed atomics keep its reads current).
-struct Words {
+// Packed charges: 2 bits per pixel, 16 pixels per word; 0 = none, 1 = +1, 3 = -1. The words live in
+// LDS when a map's words fit, else in global memory; either way one wavefront reads and changes them
+// (in global memory through relaxed atomics, so its reads see its own clears).
+template <bool LDSW>
+struct ChargeWords {
     unsigned* w;
-    __device__ unsigned get(int i) const { return __hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
-    __device__ void clear(int p) const  // caller: one lane
+    __device__ __forceinline__ unsigned get(int i) const
     {
-        __hip_atomic_fetch_and(w + (p >> 4), ~(3u << ((p & 15) * 2)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if constexpr (LDSW)
+            return w[i];
+        else
+            return __hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    __device__ int charge(int p) const
+    __device__ __forceinline__ int at(int pix) const { return (int)((get(pix >> 4) >> ((pix & 15) * 2)) & 3u); }
+    __device__ __forceinline__ void clear(int pix) const  // caller: one lane
     {
-        const int c = unpack(get(p >> 4), p);
-        return c == 1 ? 1 : (c == 3 ? -1 : 0);
+        const unsigned m = ~(3u << ((pix & 15) * 2));
+        if constexpr (LDSW)
+            atomicAnd(w + (pix >> 4), m);
+        else
+            __hip_atomic_fetch_and(w + (pix >> 4), m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 };
 
-// cuPlaceBranchCutInternal with signed coordinates; lanes split the fill points.
-__device__ void gs_place_cut(uint8_t* cut, int W, int r1, int c1, int r2, int c2, int lane)
-{
-    if (lane == 0) {
-        cut[r1 + (long long)W * c1] = 1;
-        cut[r2 + (long long)W * c2] = 1;
-    }
-    const float dR = (float)(r2 - r1), dC = (float)(c2 - c1);
-    const float rad = __fsqrt_rn(__fadd_rn(__fmul_rn(dR, dR), __fmul_rn(dC, dC)));
-    const float ct = __fdiv_rn(dR, rad), st = __fdiv_rn(dC, rad);
-    const int n = (int)rintf(rad);
-    const int rlo = min(r1, r2), rhi = max(r1, r2), clo = min(c1, c2), chi = max(c1, c2);
-    for (int ii = 1 + lane; ii <= n; ii += 64) {
-        const int rf = r1 + (int)rintf(__fmul_rn((float)ii, ct));
-        const int cf = c1 + (int)rintf(__fmul_rn((float)ii, st));
-        if (rf >= rlo && rf <= rhi && cf >= clo && cf <= chi) cut[rf + (long long)W * cf] = 1;
-    }
-}
-
-__global__ __launch_bounds__(64) void k_gs_cuts(const int8_t* __restrict__ res, int H, int W,
-                                                unsigned* __restrict__ gwords, uint8_t* __restrict__ cuts)
-{
-    const int lane = threadIdx.x;
-    const long long n = (long long)H * W;
-    const int nw = (int)((n + 15) / 16);
-    const int8_t* r = res + blockIdx.x * n;
-    uint8_t* cut = cuts + blockIdx.x * n;
-    Words ch{gwords + (long long)blockIdx.x * nw};
-    for (int w = lane; w < nw; w += 64) {
-        unsigned v = 0;
-        for (int k = 0; k < 16; ++k) {
-            const long long p = (long long)w * 16 + k;
-            const int q = p < n ? r[p] : 0;
-            v |= (unsigned)(q == 1 ? 1 : (q == -1 ? 3 : 0)) << (2 * k);
-        }
-        ch.w[w] = v;
-    }
-    __syncthreads();
-    const int R = min(901, W / 2);
-    for (int base = 0; base < nw; base += 64) {
-        const int myw = base + lane;
-        unsigned long long live = __ballot(myw < nw && ch.get(myw) != 0u);
-        while (live) {
-            const int l = __ffsll((long long)live) - 1;
-            live &= live - 1;
-            const int wi = base + l;
-            for (int bit = 0; bit < 16; ++bit) {
-                const int p = wi * 16 + bit;
-                const int q = ch.charge(p);  // re-read: earlier residues may have cleared it
-                if (q == 0) continue;
-                const int i = p % W, j = p / W;
-                if (i <= 0 || i >= W - 1 || j <= 0 || j >= H - 1) {
-                    if (lane == 0) {
-                        cut[p] = 1;
-                        ch.clear(p);
-                    }
-                    __syncthreads();
-                    continue;
-                }
-                bool done = false;
-                for (int rr = 1; rr < R && !done; ++rr) {
-                    const int m1 = max(i - rr, 0), m2 = min(i + rr, W - 1);
-                    const int n1 = max(j - rr, 0), n2 = min(j + rr, H - 1);
-                    const int nrow = n2 - n1 + 1, ncol = m2 - m1 + 1;
-                    const int P = 2 * nrow + 2 * (ncol - 2);
-                    for (int q0 = 0; q0 < P && !done; q0 += 64) {
-                        const int qi = q0 + lane;
-                        int kind = 0, mm = 0, nn = 0;
-                        if (qi < P) {
-                            if (qi < nrow) {
-                                mm = m1;
-                                nn = n1 + qi;
-                            } else if (qi < nrow + 2 * (ncol - 2)) {
-                                const int q2 = qi - nrow;
-                                mm = m1 + 1 + q2 / 2;
-                                nn = (q2 & 1) ? n2 : n1;
-                            } else {
-                                mm = m2;
-                                nn = n1 + (qi - nrow - 2 * (ncol - 2));
-                            }
-                            if (mm <= 0 || mm >= W - 1)
-                                kind = 1;
-                            else if (nn <= 0 || nn >= H - 1)
-                                kind = 2;
-                            else if (q * ch.charge(mm + W * nn) == -1)
-                                kind = 3;
-                        }
-                        const unsigned long long hit = __ballot(kind != 0);
-                        if (hit) {
-                            const int src = __ffsll((long long)hit) - 1;
-                            kind = __shfl(kind, src);
-                            mm = __shfl(mm, src);
-                            nn = __shfl(nn, src);
-                            if (kind == 1)
-                                gs_place_cut(cut, W, i, j, mm, j, lane);
-                            else if (kind == 2)
-                                gs_place_cut(cut, W, i, j, i, nn, lane);
-                            else
-                                gs_place_cut(cut, W, i, j, mm, nn, lane);
-                            if (lane == 0) {
-                                ch.clear(p);
-                                if (kind == 3) ch.clear(mm + W * nn);
-                            }
-                            done = true;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-    }
-}
-
-// Whole-map variant for maps whose charges (2 bits) and cut mask (1 bit per pixel) fit in LDS: the cut
-// mask is built in LDS and written out once at the end, so the residue loop issues no global stores
-// and needs no workgroup barrier (one wavefront: its LDS operations complete in issue order, and every
-// branch below is uniform across the wave). The winner of a perimeter scan is read with readlane, and
-// charges are cleared with LDS atomics whose result nothing waits for.
 // ---------------------------------------------------------------- 2b. branch cuts in three phases
 // The same result as the raster-order search, with most of the work in parallel. Charges only ever
 // disappear, and a border point is always a hit, so the first hit of a residue's scan at the time it is
@@ -312,22 +199,23 @@ __global__ __launch_bounds__(1024) void k_gs_cand(const int8_t* __restrict__ res
 
 __device__ __forceinline__ int sel4(const int4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
 
-// Phase 2: one wavefront per map, residues in raster order. Charges in LDS, 2 bits per pixel. Residues
-// are taken 64 at a time, one per lane: each lane reads, once, whether its residue and its listed hits are
-// still present; the residues are then committed in order from those flags, and after each commit every
-// lane clears the flags of the pixels that commit removed (a few vector compares), so the loop itself
-// waits for no memory. The LDS copy receives the same clears for later batches and rescans.
-__device__ __forceinline__ int charge_of(const unsigned* chw, int pix) { return (int)((chw[pix >> 4] >> ((pix & 15) * 2)) & 3u); }
-
+// Phase 2: one wavefront per map, residues in raster order, charges 2 bits per pixel (LDS, or global
+// memory for maps whose words exceed LDS). Residues are taken 64 at a time, one per lane: each lane
+// reads, once, whether its residue and its listed hits are still present; the residues are then
+// committed in order from those flags, and after each commit every lane clears the flags of the pixels
+// that commit removed (a few vector compares), so the loop itself waits for no memory. The charge words
+// receive the same clears for later batches and rescans.
+template <bool LDSW>
 __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res, int H, int W,
                                                   const int* __restrict__ nres, const int2* __restrict__ rec_pm,
-                                                  const int4* __restrict__ rec_c, int2* __restrict__ pairs,
-                                                  int* __restrict__ npairs)
+                                                  const int4* __restrict__ rec_c, unsigned* __restrict__ gwords,
+                                                  int2* __restrict__ pairs, int* __restrict__ npairs)
 {
-    extern __shared__ unsigned chw[];
+    extern __shared__ unsigned lwords[];
     const int lane = threadIdx.x;
     const long long n = (long long)H * W;
     const int nw = (int)((n + 15) / 16);
+    const ChargeWords<LDSW> ch{LDSW ? lwords : gwords + (long long)blockIdx.x * nw};
     const int8_t* r = res + blockIdx.x * n;
     for (int w = lane; w < nw; w += 64) {
         unsigned v = 0;
@@ -336,7 +224,7 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
             const int q = p < n ? r[p] : 0;
             v |= (unsigned)(q == 1 ? 1 : (q == -1 ? 3 : 0)) << (2 * k);
         }
-        chw[w] = v;
+        ch.w[w] = v;
     }
     __syncthreads();
     const int total = nres[blockIdx.x];
@@ -359,13 +247,13 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
         int valid = 0;
         int cpix[GS_K];
         if (lane < cnt_batch) {
-            if (charge_of(chw, mypm.x)) valid |= 1;
+            if (ch.at(mypm.x)) valid |= 1;
 #pragma unroll
             for (int k = 0; k < GS_K; ++k) {
                 const unsigned c = (unsigned)sel4(myc, k);
                 cpix[k] = (int)(c & 0x3fffffffu);
                 if (k < cn) {
-                    if ((c >> 30) != 3u || charge_of(chw, cpix[k])) valid |= 2 << k;
+                    if ((c >> 30) != 3u || ch.at(cpix[k])) valid |= 2 << k;
                 }
                 if ((c >> 30) != 3u) cpix[k] = -1;  // not a charge: no clear can touch it
             }
@@ -393,7 +281,7 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
                 found = true;
             } else if (!(meta & (GS_META_TERM | GS_META_EXHAUSTED))) {
                 // every listed charge is gone: scan again on the current charges
-                const int cp2 = charge_of(chw, p);
+                const int cp2 = ch.at(p);
                 const int q = cp2 == 1 ? 1 : -1;
                 const int i = p % W, j = p / W;
                 for (int rr = 1; rr < R && !found; ++rr) {
@@ -426,7 +314,7 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
                                 hit_here = true;
                             } else {
                                 const int pp = mm + W * nn;
-                                const int cc = charge_of(chw, pp);
+                                const int cc = ch.at(pp);
                                 if (q * (cc == 1 ? 1 : (cc == 3 ? -1 : 0)) == -1) {
                                     c = (int)((3u << 30) | (unsigned)pp);
                                     hit_here = true;
@@ -445,8 +333,8 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
             const int pix = ((unsigned)chosen >> 30) == 3u ? (chosen & 0x3fffffff) : -1;
             if (lane == 0) {
                 out[emitted] = make_int2(p, chosen);
-                atomicAnd(chw + (p >> 4), ~(3u << ((p & 15) * 2)));
-                if (pix >= 0) atomicAnd(chw + (pix >> 4), ~(3u << ((pix & 15) * 2)));
+                ch.clear(p);
+                if (pix >= 0) ch.clear(pix);
             }
             ++emitted;
             // the batch's flags follow the two clears
@@ -734,29 +622,30 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
     hipLaunchKernelGGL(k_gs_residues, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, batch, H, W, r);
     const long long nw = (n + 15) / 16, nb = (n + 31) / 32;
     const size_t lds_max = (size_t)(c.max_lds > 0 ? c.max_lds : 64 * 1024);
+    // branch cuts in three phases: candidates in parallel, raster-order commit, drawing
     const size_t commit_lds = sizeof(unsigned) * (size_t)nw;
-    if (commit_lds <= lds_max) {  // three phases: candidates in parallel, raster-order commit, drawing
-        const int nblk = (int)((n + 1023) / 1024);
-        int* blk = (int*)c.buf("gs.blk", sizeof(int) * (size_t)nblk * batch);
-        int* nres = (int*)c.buf("gs.nres", sizeof(int) * (size_t)batch);
-        int* npairs = (int*)c.buf("gs.npairs", sizeof(int) * (size_t)batch);
-        int2* rec_pm = (int2*)c.buf("gs.recpm", sizeof(int2) * (size_t)tot);
-        int4* rec_c = (int4*)c.buf("gs.recc", sizeof(int4) * (size_t)tot);
-        int2* pairs = (int2*)c.buf("gs.pairs", sizeof(int2) * (size_t)tot);
-        hipLaunchKernelGGL(k_gs_count, dim3((unsigned)(nblk * batch)), dim3(1024), 0, c.stream, r, n, nblk, blk);
-        hipLaunchKernelGGL(k_gs_scan, dim3((batch + 63) / 64), dim3(64), 0, c.stream, blk, nblk, batch, nres);
-        hipLaunchKernelGGL(k_gs_cand, dim3((unsigned)(nblk * batch)), dim3(1024), 0, c.stream, r, H, W, nblk, blk,
-                           rec_pm, rec_c);
-        hipLaunchKernelGGL(k_gs_commit, dim3(batch), dim3(64), commit_lds, c.stream, r, H, W, nres, rec_pm, rec_c,
-                           pairs, npairs);
-        hipLaunchKernelGGL(k_gs_fill_u8, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, ct, tot);
-        hipLaunchKernelGGL(k_gs_draw, dim3((unsigned)((n + 255) / 256), (unsigned)batch), dim3(256), 0, c.stream,
-                           pairs, npairs, H, W, ct);
-    } else {  // charges beyond LDS: sequential search over charge words in global memory
-        hipLaunchKernelGGL(k_gs_fill_u8, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, ct, tot);
+    const int nblk = (int)((n + 1023) / 1024);
+    int* blk = (int*)c.buf("gs.blk", sizeof(int) * (size_t)nblk * batch);
+    int* nres = (int*)c.buf("gs.nres", sizeof(int) * (size_t)batch);
+    int* npairs = (int*)c.buf("gs.npairs", sizeof(int) * (size_t)batch);
+    int2* rec_pm = (int2*)c.buf("gs.recpm", sizeof(int2) * (size_t)tot);
+    int4* rec_c = (int4*)c.buf("gs.recc", sizeof(int4) * (size_t)tot);
+    int2* pairs = (int2*)c.buf("gs.pairs", sizeof(int2) * (size_t)tot);
+    hipLaunchKernelGGL(k_gs_count, dim3((unsigned)(nblk * batch)), dim3(1024), 0, c.stream, r, n, nblk, blk);
+    hipLaunchKernelGGL(k_gs_scan, dim3((batch + 63) / 64), dim3(64), 0, c.stream, blk, nblk, batch, nres);
+    hipLaunchKernelGGL(k_gs_cand, dim3((unsigned)(nblk * batch)), dim3(1024), 0, c.stream, r, H, W, nblk, blk,
+                       rec_pm, rec_c);
+    if (commit_lds <= lds_max) {
+        hipLaunchKernelGGL(k_gs_commit<true>, dim3(batch), dim3(64), commit_lds, c.stream, r, H, W, nres, rec_pm,
+                           rec_c, nullptr, pairs, npairs);
+    } else {  // a map's charge words exceed LDS: they stay in global memory
         unsigned* gw = (unsigned*)c.buf("gs.words", sizeof(unsigned) * (size_t)nw * batch);
-        hipLaunchKernelGGL(k_gs_cuts, dim3(batch), dim3(64), 0, c.stream, r, H, W, gw, ct);
+        hipLaunchKernelGGL(k_gs_commit<false>, dim3(batch), dim3(64), 0, c.stream, r, H, W, nres, rec_pm, rec_c,
+                           gw, pairs, npairs);
     }
+    hipLaunchKernelGGL(k_gs_fill_u8, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, ct, tot);
+    hipLaunchKernelGGL(k_gs_draw, dim3((unsigned)((n + 255) / 256), (unsigned)batch), dim3(256), 0, c.stream,
+                       pairs, npairs, H, W, ct);
     CTDM_CHECK_LAUNCH("k_gs_cuts");
     const size_t bfs_lds = sizeof(unsigned) * (size_t)(4 * nb);
     if (bfs_lds + 64 <= lds_max) {
