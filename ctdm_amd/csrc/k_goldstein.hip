@@ -646,7 +646,7 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
     hipLaunchKernelGGL(k_gs_fill_u8, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, ct, tot);
     hipLaunchKernelGGL(k_gs_draw, dim3((unsigned)((n + 255) / 256), (unsigned)batch), dim3(256), 0, c.stream,
                        pairs, npairs, H, W, ct);
-    CTDM_CHECK_LAUNCH("k_gs_cuts");
+    CTDM_CHECK_LAUNCH("k_gs_draw");
     const size_t bfs_lds = sizeof(unsigned) * (size_t)(4 * nb);
     if (bfs_lds + 64 <= lds_max) {
         int* qc = (int*)c.buf("gs.qc", sizeof(int) * tot);
