@@ -85,29 +85,22 @@ __device__ __forceinline__ Cap ewald_cap(int ii, int jj, double fx0, double fy0,
 
 __device__ __forceinline__ int signed_freq(int n, int V) { return n < V / 2 ? n : n - V; }
 
-// LDS chunk of per-pencil samples staged at a time by the gridding and fit kernels.
-constexpr int GRID_CHUNK = 64;
+// LDS chunk of per-pencil samples staged at a time by the gridding and fit kernels (one bit each in
+// a 32-bit owner mask). 32 keeps the gridding pass's LDS at 7 workgroups per CU for 256^3.
+constexpr int GRID_CHUNK = 32;
+static_assert(GRID_CHUNK <= 32, "owner masks are 32-bit");
 
-// Entries of a staged chunk that fall in a thread's z range: bit k is set when entry k < n has its z
-// index in [z0, z0 + ZR). The chunk's z indices (row: GRID_CHUNK shorts, 16-byte aligned) are read 16
-// bytes at a time and tested in registers, so a thread only touches LDS for its own entries.
-template <int ZR>
-__device__ __forceinline__ unsigned long long chunk_mask(const short* row, int n, int z0)
+// Owner masks of a staged chunk. Thread t of a pencil's T threads owns the z indices z = t + T j;
+// interleaved ownership spreads a cap's neighbouring z values over the threads, so the per-thread
+// serial accumulation stays short. Whoever stages entry k (z in [0, V)) sets bit k of its owner's
+// 32-bit word with an LDS atomic OR, which is order-independent; after the barrier the owner reads its
+// word and walks the set bits in increasing k, i.e. in angle (or list) order, so sums are
+// deterministic. The owner clears its word before the next chunk is staged.
+template <int T>
+__device__ __forceinline__ void mark_owner(unsigned* masks, int z, int k)
 {
-    unsigned long long m = 0;
-    const uint4* r4 = reinterpret_cast<const uint4*>(row);
-#pragma unroll
-    for (int q = 0; q < GRID_CHUNK / 8; ++q) {
-        const uint4 w4 = r4[q];
-        const unsigned wd[4] = {w4.x, w4.y, w4.z, w4.w};
-#pragma unroll
-        for (int h = 0; h < 8; ++h) {
-            const int k = 8 * q + h;
-            const int z = (short)(wd[h >> 1] >> (16 * (h & 1)));
-            if (k < n && (unsigned)(z - z0) < (unsigned)ZR) m |= 1ull << k;
-        }
-    }
-    return m;
+    static_assert((T & (T - 1)) == 0, "owner index needs a power-of-two thread count");
+    atomicOr(&masks[z & (T - 1)], 1u << k);
 }
 
 // Device copy of the per-angle rows (fx0, fy0, fz0).
@@ -679,20 +672,20 @@ __global__ void k_pgd_build(const float2* __restrict__ spec, const double* __res
 }
 
 // Z pass of the fit: forward z-FFT, R = G^T (G F - m) accumulated per pencil in angle order (thread t
-// owns z range [t V/T, (t+1) V/T)), F -= tau R / hits, inverse z-FFT.
+// owns the z indices t + T j), F -= tau R / hits, inverse z-FFT.
 template <int V, int B>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__ W, const float2* __restrict__ tw,
                                                             MeasList ml, float tau)
 {
     constexpr int T = Plan<V>::T;
-    constexpr int ZR = V / T;
     constexpr int LS = LdsStride<V, B>::value;
     __shared__ float2 lds[B * LS];
     __shared__ float2 twl[LdsSize<V>::value];
     __shared__ float2 racc[B][V + 1];
     __shared__ unsigned short hit[B][V + 2];
-    __shared__ __align__(16) short lnz[B][GRID_CHUNK + 8];  // rows 16-byte aligned for chunk_mask
+    __shared__ short lnz[B][GRID_CHUNK];
     __shared__ float2 lval[B][GRID_CHUNK + 1];
+    __shared__ unsigned lmask[B][T];  // owner masks of the staged chunk
     stage_twiddles<V>(twl, tw);
     const TwPad twp{twl};
     Tile<V, 2, B> tl;
@@ -701,6 +694,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__
         racc[tl.b][z] = make_float2(0.f, 0.f);
         hit[tl.b][z] = 0;
     }
+    lmask[tl.b][tl.t] = 0;  // the forward transform's barrier orders this before the first marks
     float2* pen = W + tl.base;
     auto gl = [&](int z) -> float2 { return pen[(long long)z * tl.stride]; };
     auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
@@ -708,17 +702,19 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__
     __syncthreads();
     const long long pencil = tl.base;  // x + V*y
     const int n = min(ml.cnt[pencil], ml.cap);
-    const int z0 = tl.t * ZR;
     for (int e0 = 0; e0 < n; e0 += GRID_CHUNK) {
         const int ne = min(GRID_CHUNK, n - e0);
         for (int k = tl.t; k < ne; k += T) {
-            lnz[tl.b][k] = ml.idx[pencil * ml.cap + e0 + k];
+            const int z = ml.idx[pencil * ml.cap + e0 + k];
+            lnz[tl.b][k] = (short)z;
             lval[tl.b][k] = ml.val[pencil * ml.cap + e0 + k];
+            if ((unsigned)z < (unsigned)V) mark_owner<T>(lmask[tl.b], z, k);
         }
         __syncthreads();
-        unsigned long long mine = chunk_mask<ZR>(&lnz[tl.b][0], ne, z0);  // this thread's entries, in order
+        unsigned mine = lmask[tl.b][tl.t];  // this thread's entries, in list order
+        lmask[tl.b][tl.t] = 0;  // the barrier closing this chunk orders the clear before new marks
         while (mine) {
-            const int k = __ffsll((long long)mine) - 1;
+            const int k = __ffs(mine) - 1;
             mine &= mine - 1;
             const int z = lnz[tl.b][k];
             const float2 F = s[lds_pad(z)], m = lval[tl.b][k];
@@ -728,7 +724,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__
         }
         __syncthreads();
     }
-    for (int z = z0; z < z0 + ZR; ++z) {
+    for (int z = tl.t; z < V; z += T) {
         const int h = hit[tl.b][z];
         if (h) {
             const float c = tau / (float)h;
@@ -785,8 +781,10 @@ __global__ void k_dc_build(const float2* __restrict__ Fm, const float* __restric
 // Gridding pass: per k-space pencil (x, y) accumulate every angle's sample in angle order, average
 // by hit count, scale by 1/dx (Initial_F_F2), emit the compact list, then inverse z-FFT.
 // The pencil's T threads evaluate the caps of interleaved angles into an LDS list (chunks of
-// GRID_CHUNK angles); thread t then owns the z range [t V/T, (t+1) V/T) and adds the list entries
-// that land there in angle order, so the sums are deterministic.
+// GRID_CHUNK angles); thread t then owns the z indices t + T j and adds the list entries that land
+// there in angle order, so the sums are deterministic. A pencil's list holds thread t's entries
+// (ascending z) after those of threads below t; the data-consistency step only needs each measured z
+// once, in any order.
 template <int V, int B>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __restrict__ spec,
                                                             const double* __restrict__ rows, int A,
@@ -795,73 +793,112 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
                                                             float2* __restrict__ W, DcList dc)
 {
     constexpr int T = Plan<V>::T;
-    constexpr int ZR = V / T;  // z values owned by one thread
+    constexpr int APT = (GRID_CHUNK + T - 1) / T;  // angles per thread in one chunk
+    constexpr int ZR = V / T;                        // z indices owned by one thread
+    constexpr int HW = (ZR + 1) / 2;                 // 16-bit hit counters, two per register
     constexpr int LS = LdsStride<V, B>::value;
     __shared__ float2 lds[B * LS];
-    __shared__ unsigned short hit[B][V + 2];
-    __shared__ __align__(16) short lnz[B][GRID_CHUNK + 8];  // rows 16-byte aligned for the wide reads
+    __shared__ short lnz[B][GRID_CHUNK];
     __shared__ float2 lval[B][GRID_CHUNK + 1];
-    __shared__ int lcnt[B][T + 1];
-    __shared__ float2 twl[LdsSize<V>::value];
-    stage_twiddles<V>(twl, tw);
-    const TwPad twp{twl};
+    __shared__ unsigned lmask[B][T];  // owner masks of the staged chunk
     Tile<V, 2, B> tl;
-    float2* s = lds + tl.b * LS;
-    for (int z = tl.t; z < V; z += T) {
-        s[lds_pad(z)] = make_float2(0.f, 0.f);
-        hit[tl.b][z] = 0;
-    }
-    __syncthreads();  // zeroing (interleaved z) precedes the owner-range accumulation
     const long long VV = (long long)V * V;
     const long long pencil = tl.base;  // x + V*y
-    const int ii = signed_freq(tl.px, V), jj = signed_freq(tl.py, V);
-    const int z0 = tl.t * ZR;
-    for (int a0 = 0; a0 < A; a0 += GRID_CHUNK) {
+    // This thread's share of a chunk of angles: the angles a0 + t + T i. fetch() issues the loads
+    // (cap centres first, then spectrum samples) into registers, stage() evaluates the caps into the
+    // LDS list and marks each entry's owner. The first chunk is fetched before the twiddle staging and
+    // the zeroing, every later one during the previous chunk's accumulation, so the spectrum's HBM
+    // latency overlaps other work.
+    float2 u[APT];
+    double fx0[APT], fy0[APT], fz0[APT];
+    auto fetch = [&](int a0) {
         const int na = min(GRID_CHUNK, A - a0);
-        for (int k = tl.t; k < na; k += T) {
-            const int a = a0 + k;
-            const Cap cp = ewald_cap(ii, jj, rows[3 * a], rows[3 * a + 1], rows[3 * a + 2], fm02, df, V);
-            short nz = -1;
-            float2 val = make_float2(0.f, 0.f);
-            if (cp.ok) {
-                const float2 u = spec[(long long)a * VV + pencil];
-                nz = (short)cp.nz;
-                val = make_float2(-cp.w * u.y, cp.w * u.x);
-            }
-            lnz[tl.b][k] = nz;
-            lval[tl.b][k] = val;
+#pragma unroll
+        for (int i = 0; i < APT; ++i) {
+            const int k = tl.t + i * T;
+            const int a = a0 + (k < na ? k : 0);
+            fx0[i] = rows[3 * a];
+            fy0[i] = rows[3 * a + 1];
+            fz0[i] = rows[3 * a + 2];
         }
+#pragma unroll
+        for (int i = 0; i < APT; ++i) {
+            const int k = tl.t + i * T;
+            const int a = a0 + (k < na ? k : 0);
+            u[i] = spec[(long long)a * VV + pencil];
+        }
+    };
+    const int ii = signed_freq(tl.px, V), jj = signed_freq(tl.py, V);
+    auto stage = [&](int a0) {
+        const int na = min(GRID_CHUNK, A - a0);
+#pragma unroll
+        for (int i = 0; i < APT; ++i) {
+            const int k = tl.t + i * T;
+            if (k < na) {
+                const Cap cp = ewald_cap(ii, jj, fx0[i], fy0[i], fz0[i], fm02, df, V);
+                short nz = -1;
+                float2 val = make_float2(0.f, 0.f);
+                if (cp.ok) {
+                    nz = (short)cp.nz;
+                    val = make_float2(-cp.w * u[i].y, cp.w * u[i].x);
+                }
+                lnz[tl.b][k] = nz;
+                lval[tl.b][k] = val;
+                if (nz >= 0) mark_owner<T>(lmask[tl.b], nz, k);
+            }
+        }
+    };
+    if (A > 0) fetch(0);
+    float2* s = lds + tl.b * LS;
+    for (int z = tl.t; z < V; z += T) s[lds_pad(z)] = make_float2(0.f, 0.f);  // owned z indices
+    // Hit counts of the owned z indices t + T j, in registers: counter j is bits 16 (j & 1) of hc[j / 2].
+    unsigned hc[HW];
+#pragma unroll
+    for (int w = 0; w < HW; ++w) hc[w] = 0;
+    lmask[tl.b][tl.t] = 0;
+    __syncthreads();  // cleared masks before any mark (the loads stay in flight across the barrier)
+    for (int a0 = 0; a0 < A; a0 += GRID_CHUNK) {
+        if (a0 > 0) __syncthreads();  // every owner is done with the previous chunk and cleared its mask
+        stage(a0);
+        if (a0 + GRID_CHUNK < A) fetch(a0 + GRID_CHUNK);  // in flight during this chunk's accumulation
         __syncthreads();
-        // this thread's entries, in angle order
-        unsigned long long mine = chunk_mask<ZR>(&lnz[tl.b][0], na, z0);
+        unsigned mine = lmask[tl.b][tl.t];  // this thread's entries, in angle order
+        lmask[tl.b][tl.t] = 0;
         while (mine) {
-            const int k = __ffsll((long long)mine) - 1;
+            const int k = __ffs(mine) - 1;
             mine &= mine - 1;
             const int nz = lnz[tl.b][k];
             float2& acc = s[lds_pad(nz)];
             const float2 v = lval[tl.b][k];
             acc.x += v.x;
             acc.y += v.y;
-            hit[tl.b][nz] += 1;
+            const int j = nz / T;
+#pragma unroll
+            for (int w = 0; w < HW; ++w)
+                if (w == (j >> 1)) hc[w] += 1u << (16 * (j & 1));
         }
-        __syncthreads();
     }
-    // normalise the owned range and count list entries
+    auto hits = [&](int j) -> int { return (int)((hc[j >> 1] >> (16 * (j & 1))) & 0xffffu); };
+    // normalise the owned z indices (each touched only by its owner so far) and count list entries
     int cnt = 0;
-    for (int z = z0; z < z0 + ZR; ++z) {
-        const int h = hit[tl.b][z];
+#pragma unroll
+    for (int j = 0; j < ZR; ++j) {
+        const int z = tl.t + j * T;
+        const int h = hits(j);
         float2 f = s[lds_pad(z)];
         if (h) f = make_float2(f.x / (float)h, f.y / (float)h);
         f = make_float2(f.x * inv_dx, f.y * inv_dx);
         s[lds_pad(z)] = f;
         if (h && !(refcompat && !(f.x != 0.f && f.y != 0.f))) ++cnt;
     }
-    lcnt[tl.b][tl.t] = cnt;
+    lmask[tl.b][tl.t] = cnt;  // the owner's own (cleared) mask word holds its list count from here on
     __syncthreads();
     int off = 0;
-    for (int k = 0; k < tl.t; ++k) off += lcnt[tl.b][k];
-    for (int z = z0; z < z0 + ZR; ++z) {
-        const int h = hit[tl.b][z];
+    for (int k = 0; k < tl.t; ++k) off += (int)lmask[tl.b][k];
+#pragma unroll
+    for (int j = 0; j < ZR; ++j) {
+        const int z = tl.t + j * T;
+        const int h = hits(j);
         const float2 f = s[lds_pad(z)];
         if (h && !(refcompat && !(f.x != 0.f && f.y != 0.f))) {
             dc.idx[pencil * dc.cap + off] = (short)z;
@@ -873,7 +910,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
     float2* pen = W + tl.base;
     auto gl = [&](int k) -> float2 { return s[lds_pad(k)]; };
     auto gs = [&](int z, float2 v) { pen[VV * z] = v; };
-    Plan<V>::template run<+1>(tl.t, s, twp, gl, gs);
+    Plan<V>::template run<+1>(tl.t, s, tw, gl, gs);  // twiddles from the (cached) global table
 }
 
 static DcList dc_alloc(Ctx& c, int V, int cap)
