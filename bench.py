@@ -172,7 +172,8 @@ def main() -> int:
 
 def run_config4(ctx, args):
     """Eight cells, A = {30,45,60,60,90,90,120,120}, V cyclic over {128,192,256}, 1024^2 holograms,
-    10 iterations: one captured graph for all cells vs the plain per-cell pipeline."""
+    10 iterations: one captured graph for all cells (two lanes; one lane timed beside it) vs the plain
+    per-cell pipeline."""
     import torch
 
     import ctdm_amd
@@ -203,6 +204,17 @@ def run_config4(ctx, args):
     torch.cuda.synchronize()
     ms_batch = e0.elapsed_time(e1) / reps
     outs = [c["ri"].clone() for c in cells]
+    # the same sequence on one lane (cells strictly one after another)
+    ctx.reconstruct(cells, mode=1, graph=True, lanes=1)
+    torch.cuda.synchronize()
+    e0, e1 = ev(), ev()
+    e0.record()
+    for _ in range(reps):
+        ctx.reconstruct(cells, mode=1, graph=True, lanes=1)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_one_lane = e0.elapsed_time(e1) / reps
+    lane_diff = max((c["ri"] - o).abs().max().item() for c, o in zip(cells, outs))
     per_cell = []
     for c in cells:  # each cell alone, its own captured graph
         ctx.reconstruct([c], mode=1, graph=True)
@@ -226,10 +238,12 @@ def run_config4(ctx, args):
         dn = max(dn, (rp - ob).abs().max().item())
     res = {"cells": [{"A": a, "V": v} for a, v in zip(angles, sides)],
            "batched_ms_all_cells": round(ms_batch, 3), "volumes_per_s": round(8000.0 / ms_batch, 2),
+           "lanes": 2, "batched_ms_one_lane": round(ms_one_lane, 3), "ri_max_diff_two_vs_one_lane": lane_diff,
            "per_cell_latency_ms": per_cell, "plain_ms_all_cells": round(plain_ms, 2),
            "plain_volumes_per_s": round(8000.0 / plain_ms, 2), "speedup_vs_plain": round(plain_ms / ms_batch, 2),
            "ri_max_diff_batched_vs_plain": dn}
-    log(f"config4: batched {ms_batch:.2f} ms for 8 cells ({res['volumes_per_s']} vol/s), plain {plain_ms:.1f} ms, "
+    log(f"config4: batched {ms_batch:.2f} ms for 8 cells ({res['volumes_per_s']} vol/s; one lane "
+        f"{ms_one_lane:.2f} ms), plain {plain_ms:.1f} ms, "
         f"per-cell {per_cell}")
     for c in cells:
         del c["holo"], c["bg"]

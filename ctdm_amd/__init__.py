@@ -72,8 +72,8 @@ class Cell(C.Structure):
 
 
 class BatchOpts(C.Structure):
-    _fields_ = [("mode", C.c_int), ("graph", C.c_int), ("autotune", C.c_int),
-                ("reserved", C.c_int * 5)]
+    _fields_ = [("mode", C.c_int), ("graph", C.c_int), ("autotune", C.c_int), ("lanes", C.c_int),
+                ("reserved", C.c_int * 4)]
 
 
 class SynthParams(C.Structure):
@@ -447,9 +447,10 @@ class Context:
 
     # -- pipeline
     def reconstruct(self, cells: list[dict], mode: int = 1, graph: bool = False,
-                    autotune: bool = False):
+                    autotune: bool = False, lanes: int = 0):
         """cells: dicts with holo, bg (device, A x H x W), angles (A x 2), V, iters, optics,
-        ri (device V^3 complex64) and optional crop / flags / wrapped / unwrapped / logamp."""
+        ri (device V^3 complex64) and optional crop / flags / wrapped / unwrapped / logamp.
+        lanes: batched multi-cell sequences run on two concurrent lanes unless lanes == 1."""
         arr = (Cell * len(cells))()
         keep = []
         for i, d in enumerate(cells):
@@ -461,7 +462,7 @@ class Context:
                           d.get("unwrap_flags", 0), d.get("recon_flags", 0), d["optics"], crop,
                           _ptr(d["ri"]), _ptr(d.get("wrapped")), _ptr(d.get("unwrapped")),
                           _ptr(d.get("logamp")))
-        opts = BatchOpts(mode, int(graph), int(autotune), (C.c_int * 5)())
+        opts = BatchOpts(mode, int(graph), int(autotune), int(lanes), (C.c_int * 4)())
         self._call("ctdm_reconstruct", arr, len(cells), C.byref(opts))
         for i, d in enumerate(cells):  # report the crop the pipeline used
             d["crop_used"] = arr[i].crop

@@ -29,9 +29,9 @@ const float2* Ctx::tw(int n)
     return d;
 }
 
-void* Ctx::buf(const std::string& name, size_t bytes)
+void* Ctx::buf(const std::string& name, size_t bytes, bool shared)
 {
-    DevBuf& b = scratch[name];
+    DevBuf& b = scratch[lane && !shared ? name + "#lane1" : name];
     if (b.bytes >= bytes && b.ptr) return b.ptr;
     CTDM_REQUIRE(!capturing, "scratch buffer '" + name + "' grown during graph capture");
     if (b.ptr) {
@@ -47,6 +47,14 @@ void* Ctx::buf(const std::string& name, size_t bytes)
     CTDM_HIP(hipMalloc(&b.ptr, bytes ? bytes : 1));
     b.bytes = bytes;
     return b.ptr;
+}
+
+void Ctx::ensure_aux()
+{
+    if (aux) return;
+    CTDM_REQUIRE(!capturing, "auxiliary stream created during graph capture");
+    CTDM_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+    for (hipEvent_t& e : lane_ev) CTDM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 unsigned long long* Ctx::host_words(int n)
@@ -139,6 +147,12 @@ ctdm_status ctdm_destroy(ctdm_ctx* ctx)
         if (kv.second.ptr) hipFree(kv.second.ptr);
     for (void* p : ctx->c.retired) hipFree(p);
     if (ctx->c.host_words_) hipHostFree(ctx->c.host_words_);
+    if (ctx->c.aux) {
+        hipStreamSynchronize(ctx->c.aux);
+        hipStreamDestroy(ctx->c.aux);
+    }
+    for (hipEvent_t e : ctx->c.lane_ev)
+        if (e) hipEventDestroy(e);
     delete ctx;
     return CTDM_OK;
 }

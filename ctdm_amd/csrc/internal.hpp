@@ -88,9 +88,18 @@ struct Ctx {
 
     unsigned long long* host_words_ = nullptr;  // small pinned host buffer (hipHostMalloc)
 
+    // Lanes of a batched multi-cell sequence: cells alternate between lane 0 (the context stream) and
+    // lane 1 (an auxiliary stream forked from it and joined back). Scratch named while lane 1 is
+    // active is lane 1's own copy, so the two lanes never share a buffer.
+    int lane = 0;
+    hipStream_t aux = nullptr;                    // lane 1 stream (default priority), made on first use
+    hipEvent_t lane_ev[2] = {nullptr, nullptr};   // fork and join events of lane 1
+    void ensure_aux();
+
     const float2* tw(int n);
-    // Scratch buffer `name` of at least `bytes` bytes (contents undefined).
-    void* buf(const std::string& name, size_t bytes);
+    // Scratch buffer `name` of at least `bytes` bytes (contents undefined). Lane 1 has its own copy
+    // unless `shared` (read-only tables written once, outside any lane's work).
+    void* buf(const std::string& name, size_t bytes, bool shared = false);
     // Pinned host words for small results read back from the device (at most 64; contents undefined).
     unsigned long long* host_words(int n);
     // Waits for the context stream by polling it: a host-synchronous search returns as soon as its last

@@ -916,8 +916,10 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
 static DcList dc_alloc(Ctx& c, int V, int cap)
 {
     const size_t VV = (size_t)V * V;
-    c.dc_V = V;
-    c.dc_cap = cap;
+    if (c.lane == 0) {  // the list the pass-timing helpers reuse is lane 0's
+        c.dc_V = V;
+        c.dc_cap = cap;
+    }
     DcList d;
     d.cap = cap;
     d.cnt = (int*)c.buf("dc.cnt", sizeof(int) * VV);

@@ -176,32 +176,70 @@ __global__ __launch_bounds__(2 * GR * Plan<M>::T) void k_row_final(
 // Key = (bits of |F|^2) << 32 | (0xffffffff - raster index): the maximum key is the largest
 // magnitude, ties resolved to the first pixel in raster order of the shifted spectrum
 // (obtainRadius scans rows j then columns i with a strict '>', HilbertTransform.cu:3439-3450).
-__global__ void k_argmax_band(const float2* __restrict__ spec, int H, int W, int row_start,
-                              unsigned long long* __restrict__ best)
+__device__ __forceinline__ unsigned long long band_key(float2 v, int i_shifted, int j_shifted, int W)
 {
-    const long long n = (long long)(H - row_start) * W;
-    unsigned long long local = 0;
-    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n;
-         g += (long long)gridDim.x * blockDim.x) {
-        const int j = row_start + (int)(g / W), i = (int)(g % W);  // shifted coordinates
-        const int ky = (j + H / 2) % H, kx = (i + W / 2) % W;     // unshifted
-        const float2 v = spec[(long long)ky * W + kx];
-        const float m = v.x * v.x + v.y * v.y;
-        const unsigned long long key = ((unsigned long long)__float_as_uint(m) << 32) |
-                                       (0xffffffffull - (unsigned long long)(i + (long long)W * j));
-        local = key > local ? key : local;
-    }
-    // wave reduction, then the block's waves through LDS: one atomic per block
+    const float m = v.x * v.x + v.y * v.y;
+    return ((unsigned long long)__float_as_uint(m) << 32) |
+           (0xffffffffull - (unsigned long long)(i_shifted + (long long)W * j_shifted));
+}
+
+// Maximum of a block's per-thread keys (block size a multiple of 64, at most 1024 threads).
+__device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v)
+{
     for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(local, off);
-        local = o > local ? o : local;
+        const unsigned long long o = __shfl_xor(v, off);
+        v = o > v ? o : v;
     }
     __shared__ unsigned long long wmax[16];
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = local;
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
     __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = wmax[w] > v ? wmax[w] : v;
+    return v;  // valid in thread 0
+}
+
+// Column pass of the search fused with the band maximum. The row pass has left the full-width row
+// spectrum spec[ky][kx] (unshifted). C columns per workgroup (lanes walk columns, so the loads
+// coalesce) get their H-point forward transform along y; every output bin whose shifted row is at or
+// beyond row_start enters the key maximum straight from the transform's last stage, and each
+// workgroup writes its maximum to part[blockIdx.x] (no spectrum is written back).
+template <int H, int C>
+__global__ __launch_bounds__(C* Plan<H>::T) void k_col_argmax(const float2* __restrict__ spec, int W, int row_start,
+                                                              const float2* __restrict__ tw,
+                                                              unsigned long long* __restrict__ part)
+{
+    constexpr int T = Plan<H>::T;
+    constexpr int LS = LdsStride<H, C>::value;
+    __shared__ float2 lds[C * LS];
+    const int cidx = threadIdx.x % C, t = threadIdx.x / C;
+    const int kx = blockIdx.x * C + cidx;  // the grid covers W / C workgroups exactly
+    const int i_sh = (kx + W / 2) % W;     // shifted column
+    const float2* col = spec + kx;
+    float2* s = lds + cidx * LS;
+    unsigned long long local = 0;
+    auto gl = [&](int ky) -> float2 { return col[(long long)ky * W]; };
+    auto gs = [&](int ky, float2 v) {
+        const int j = (ky + H / 2) % H;  // shifted row
+        if (j >= row_start) {
+            const unsigned long long key = band_key(v, i_sh, j, W);
+            local = key > local ? key : local;
+        }
+    };
+    Plan<H>::template run<-1>(t, s, tw, gl, gs);
+    local = block_max_u64(local);
+    if (threadIdx.x == 0) part[blockIdx.x] = local;
+}
+
+// Final maximum over the workgroup maxima, written to pinned host memory (one block).
+__global__ __launch_bounds__(256) void k_publish_max(const unsigned long long* __restrict__ part, int n,
+                                                     unsigned long long* __restrict__ host)
+{
+    unsigned long long v = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v = part[i] > v ? part[i] : v;
+    v = block_max_u64(v);
     if (threadIdx.x == 0) {
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) local = wmax[w] > local ? wmax[w] : local;
-        atomicMax(best, local);
+        *host = v;
+        __threadfence_system();
     }
 }
 
@@ -304,6 +342,19 @@ void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int
     }
 }
 
+template <int H>
+static int launch_col_argmax(Ctx& c, const float2* spec, int W, int row_start, unsigned long long* part)
+{
+    constexpr int C = 256 / Plan<H>::T;
+    static_assert(C <= H, "columns per workgroup");
+    const int nblk = W / C;
+    CTDM_REQUIRE(W % C == 0 && nblk <= 256, "find_sideband: width has no column tiling");
+    hipLaunchKernelGGL((k_col_argmax<H, C>), dim3(nblk), dim3(C * Plan<H>::T), 0, c.stream, spec, W, row_start,
+                       c.tw(H), part);
+    CTDM_CHECK_LAUNCH("k_col_argmax");
+    return nblk;
+}
+
 void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, ctdm_crop* out)
 {
     CTDM_REQUIRE(bg && out, "find_sideband: null pointer");
@@ -311,18 +362,21 @@ void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, 
     if (row_start <= 0) row_start = 600 * H / 1024;
     CTDM_REQUIRE(row_start < H, "find_sideband: row_start beyond the spectrum");
     float2* spec = (float2*)c.buf("sb.spec", sizeof(float2) * (size_t)H * W);
-    unsigned long long* best = (unsigned long long*)c.buf("sb.best", sizeof(unsigned long long));
-    // Full spectrum of one real image: real row pass keeping every bin, then column pass.
+    // Full-width row spectrum of the real image, then the column transforms fused with the band
+    // maximum; the final maximum lands in pinned host memory.
     row_real(c, bg, 1, bg, 1, H, W, 0, W, spec);
-    fft_cols(c, spec, H, W, W, 1, 0, -1, 1.f, false);
-    // this search synchronises below and is never captured, so a memset may clear the result
-    CTDM_HIP(hipMemsetAsync(best, 0, sizeof(unsigned long long), c.stream));
-    const long long nband = (long long)(H - row_start) * W;
-    hipLaunchKernelGGL(k_argmax_band, dim3((unsigned)std::min<long long>(grid1d(nband, 256), 1024)), dim3(256), 0,
-                       c.stream, spec, H, W, row_start, best);
-    CTDM_CHECK_LAUNCH("k_argmax_band");
     unsigned long long* hb = c.host_words(1);
-    CTDM_HIP(hipMemcpyAsync(hb, best, sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+    unsigned long long* hb_dev = nullptr;
+    CTDM_HIP(hipHostGetDevicePointer((void**)&hb_dev, hb, 0));
+    int nblk = 0;
+    unsigned long long* part = (unsigned long long*)c.buf("sb.part", sizeof(unsigned long long) * 256);
+    switch (H) {
+    case 256: nblk = launch_col_argmax<256>(c, spec, W, row_start, part); break;
+    case 512: nblk = launch_col_argmax<512>(c, spec, W, row_start, part); break;
+    default: nblk = launch_col_argmax<1024>(c, spec, W, row_start, part); break;
+    }
+    hipLaunchKernelGGL(k_publish_max, dim3(1), dim3(256), 0, c.stream, part, nblk, hb_dev);
+    CTDM_CHECK_LAUNCH("k_publish_max");
     c.spin_sync();
     const unsigned long long h = *hb;
     const long long idx = (long long)(0xffffffffull - (h & 0xffffffffull));

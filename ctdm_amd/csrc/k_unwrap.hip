@@ -272,7 +272,7 @@ static const double* eig_table(Ctx& c, int N, bool refcompat)
             h[k] = sv * sv;
         }
     }
-    double* d = (double*)c.buf(name, sizeof(double) * N);
+    double* d = (double*)c.buf(name, sizeof(double) * N, true);  // one table for both lanes
     CTDM_HIP(hipMemcpy(d, h.data(), sizeof(double) * N, hipMemcpyHostToDevice));
     return d;
 }

@@ -219,6 +219,41 @@ static void enqueue_batched(Ctx& c, ctdm_cell& cl)
     volume_batched(c, spec, cl.angles, A, V, cl.optics, cl.iters, cl.recon_flags, (float2*)cl.ri);
 }
 
+// Enqueues the batched sequence. With two lanes, cell i goes to lane i % 2: lane 0 is the context
+// stream, lane 1 the auxiliary stream, forked from the context stream before the first cell and joined
+// back after the last, so the call's work still ends on the context stream (also inside a graph
+// capture, where the fork and join become graph edges). A cell's latency-bound retrieval, unwrapping
+// and gridding then overlap the bandwidth-bound positivity loop of the cell on the other lane.
+static void enqueue_cells(Ctx& c, ctdm_cell* cells, int ncells, int lanes)
+{
+    if (lanes == 1 || ncells < 2) {
+        for (int i = 0; i < ncells; ++i) enqueue_batched(c, cells[i]);
+        return;
+    }
+    c.ensure_aux();
+    const hipStream_t main = c.stream;
+    CTDM_HIP(hipEventRecord(c.lane_ev[0], main));
+    CTDM_HIP(hipStreamWaitEvent(c.aux, c.lane_ev[0], 0));
+    auto join = [&] {
+        c.lane = 0;
+        c.stream = main;
+        const hipError_t e1 = hipEventRecord(c.lane_ev[1], c.aux);
+        const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(main, c.lane_ev[1], 0) : e1;
+        return e2;
+    };
+    try {
+        for (int i = 0; i < ncells; ++i) {
+            c.lane = i & 1;
+            c.stream = c.lane ? c.aux : main;
+            enqueue_batched(c, cells[i]);
+        }
+    } catch (...) {
+        join();  // a capture can only end once lane 1 has joined back
+        throw;
+    }
+    CTDM_HIP(join());
+}
+
 static GraphKey graph_key(const ctdm_cell* cells, int ncells)
 {
     GraphKey k;
@@ -255,6 +290,7 @@ void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& op
         return;
     }
     CTDM_REQUIRE(opts.mode == 1, "reconstruct: mode must be 0 (plain) or 1 (batched)");
+    CTDM_REQUIRE(opts.lanes >= 0 && opts.lanes <= 2, "reconstruct: lanes must be 0, 1 or 2");
     for (int i = 0; i < ncells; ++i) cells[i].crop = cell_crop(c, cells[i]);  // host-synchronous, never captured
     if (opts.autotune) {
         std::vector<int> sizes;
@@ -262,14 +298,15 @@ void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& op
         autotune(c, sizes.data(), (int)sizes.size(), false);
     }
     if (!opts.graph) {
-        for (int i = 0; i < ncells; ++i) enqueue_batched(c, cells[i]);
+        enqueue_cells(c, cells, ncells, opts.lanes);
         return;
     }
-    const GraphKey key = graph_key(cells, ncells);
+    GraphKey key = graph_key(cells, ncells);
+    key.words.push_back((uintptr_t)(opts.lanes == 1 ? 1 : 2));
     auto it = c.graphs.find(key);
     if (it == c.graphs.end()) {
         // Eager pass: allocates every buffer and table the sequence needs and produces the result.
-        for (int i = 0; i < ncells; ++i) enqueue_batched(c, cells[i]);
+        enqueue_cells(c, cells, ncells, opts.lanes);
         CTDM_HIP(hipStreamSynchronize(c.stream));
         hipStream_t cs = nullptr;
         CTDM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
@@ -280,7 +317,7 @@ void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& op
         try {
             CTDM_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
             try {
-                for (int i = 0; i < ncells; ++i) enqueue_batched(c, cells[i]);
+                enqueue_cells(c, cells, ncells, opts.lanes);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 hipStreamEndCapture(cs, &g);

@@ -256,7 +256,12 @@ typedef struct ctdm_batch_opts {
                       1 = batched (all angles and cells per launch, fused passes) */
     int graph;     /* batched only: capture the launch sequence in a hipGraph and replay it */
     int autotune;  /* batched only: autotune kernel variants for the sizes present before capture */
-    int reserved[5];
+    int lanes;     /* batched only: 1 = cells one after another on the context stream; 0 (default) or
+                      2 = with two or more cells, cells alternate between the context stream and an
+                      auxiliary stream forked from it and joined back before the call's work ends
+                      (one cell's retrieval overlaps another's positivity loop; lane 2 has its own
+                      scratch, so device memory for scratch doubles) */
+    int reserved[4];
 } ctdm_batch_opts;
 
 ctdm_status ctdm_reconstruct(ctdm_ctx *ctx, ctdm_cell *cells, int ncells,

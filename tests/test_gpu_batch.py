@@ -46,3 +46,29 @@ def test_uneven_cells_one_graph(ctx, orc):
         err = np.abs(b.cpu().numpy() - ref).max()
         print(f"cell A={A} V={V}: max|dn| {err:.2e}")
         assert err <= 1e-5
+
+
+def test_two_lanes_equal_one_lane(ctx):
+    """Cells alternating over two concurrent lanes (the default for several cells) give bit-identical
+    RI to the one-lane sequence, eager and graph-replayed, with ordinary eager work between replays."""
+    spec = [(5, 256, 64), (8, 256, 32), (6, 256, 64), (4, 256, 32), (3, 256, 64)]  # odd count: lane 0 ends
+    cells, _ = _cells(spec)
+    ctx.reconstruct(cells, mode=1, lanes=1)
+    torch.cuda.synchronize()
+    ref = [c["ri"].clone() for c in cells]
+    for graph in (False, True):
+        for rep in range(3):
+            for c in cells:
+                c["ri"].fill_(float("nan"))
+            ctx.reconstruct(cells, mode=1, graph=graph, lanes=2)
+            torch.cuda.synchronize()
+            for i, (c, r) in enumerate(zip(cells, ref)):
+                assert torch.equal(c["ri"], r), (graph, rep, i)
+            scratch = torch.arange(1 << 16, device="cuda", dtype=torch.int64)
+            assert scratch.sum().item() == (1 << 16) * ((1 << 16) - 1) // 2
+    ctx.reconstruct(cells, mode=1, graph=True, lanes=1)  # its own graph (the key holds the lane count)
+    torch.cuda.synchronize()
+    for c, r in zip(cells, ref):
+        assert torch.equal(c["ri"], r)
+    with pytest.raises(RuntimeError, match="lanes"):
+        ctx.reconstruct(cells, mode=1, lanes=3)

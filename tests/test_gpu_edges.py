@@ -29,7 +29,7 @@ def test_empty_batches_are_noops(ctx):
     assert _st(ctx, "ctdm_goldstein", _p(x), 0, 64, 64, None, None, None, _p(x + 1)) == 0
     assert _st(ctx, "ctdm_median3x3", _p(x), _p(x + 1), 0, 8, 8) == 0
     assert _st(ctx, "ctdm_retrieve_rows", _p(x), _p(x), 0, 1024, 1024, 768, _p(x), _p(x)) == 0
-    assert _st(ctx, "ctdm_reconstruct", None, 0, C.byref(ctdm_amd.BatchOpts(1, 0, 0, (C.c_int * 5)()))) == 0
+    assert _st(ctx, "ctdm_reconstruct", None, 0, C.byref(ctdm_amd.BatchOpts(1, 0, 0, 0, (C.c_int * 4)()))) == 0
     ctx.sync()
 
 
