@@ -585,9 +585,11 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_x(float2* __restrict__ W,
 // Z pass with data consistency: forward z-FFT, replace measured voxels, inverse z-FFT.
 // The pencil's list entries are fetched at kernel entry (DC_PRE per thread, independent of the
 // count) so their latency overlaps the data loads; longer lists finish in a loop.
+// With tau != 1 a measured voxel moves only part of the way: F <- F + tau (m - F), the fit's
+// hit-preconditioned gradient step (see fit_pgd); tau == 1 is the exact replacement.
 template <int V, int B>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ W, const float2* __restrict__ tw,
-                                                           DcList dc)
+                                                           DcList dc, float tau)
 {
     constexpr int T = Plan<V>::T;
     constexpr int DC_PRE = 4;
@@ -620,118 +622,24 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
     __syncthreads();
     {
         float2* sp = lds + pb * LS;
+        const bool relax = tau != 1.f;
+        auto put = [&](int z, float2 m) {  // each z appears once in a pencil's list
+            if (relax) {
+                const float2 F = sp[lds_pad(z)];
+                m = make_float2(F.x + tau * (m.x - F.x), F.y + tau * (m.y - F.y));
+            }
+            sp[lds_pad(z)] = m;
+        };
 #pragma unroll
         for (int r = 0; r < DC_PRE; ++r) {
             const int k = k0 + r * T;
             const int z = pz[r];
-            if (k < n && z >= 0 && z < V) sp[lds_pad(z)] = pv[r];
+            if (k < n && z >= 0 && z < V) put(z, pv[r]);
         }
         for (int k = k0 + DC_PRE * T; k < n; k += T) {
             const long long e = dpencil * dc.cap + k;
             const int z = dc.idx[e];
-            if (z >= 0 && z < V) sp[lds_pad(z)] = dc.val[e];
-        }
-    }
-    __syncthreads();
-    auto gl2 = [&](int k) -> float2 { return s[lds_pad(k)]; };
-    auto gs2 = [&](int z, float2 v) { pen[(long long)z * tl.stride] = v; };
-    Plan<V>::template run<+1>(tl.t, s, twp, gl2, gs2);
-}
-
-// ---------------------------------------------------------------- projected-gradient fit
-// Measurement list of the fit: for pencil p = x + V*y, every (angle, cap sample) landing in the pencil
-// in angle order: z index and m = 4 pi i f_z spec / dx (the angle's own k-space value before hit
-// averaging). With G the unit-weight Ewald gather (F -> F[v_a(p)]) and G^T its scatter, the
-// objective J(f) = 1/2 sum_a |G_a FFT3(f) - m_a|^2 has the k-space gradient R = G^T (G F - m);
-// the fit steps F <- F - tau R / hits (hit-preconditioned) on every pencil, which for tau = 1 is
-// the reference's replacement by the averaged measurement (check_F3D).
-struct MeasList {
-    int* cnt;
-    short* idx;
-    float2* val;
-    int cap;
-};
-
-__global__ void k_pgd_build(const float2* __restrict__ spec, const double* __restrict__ rows, int A, int V,
-                            double fm02, double df, float inv_dx, MeasList ml)
-{
-    const long long VV = (long long)V * V;
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= VV) return;
-    const int ii = signed_freq((int)(p % V), V), jj = signed_freq((int)(p / V), V);
-    int n = 0;
-    for (int a = 0; a < A; ++a) {
-        const Cap cp = ewald_cap(ii, jj, rows[3 * a], rows[3 * a + 1], rows[3 * a + 2], fm02, df, V);
-        if (!cp.ok) continue;
-        const float2 u = spec[(long long)a * VV + p];
-        ml.idx[p * ml.cap + n] = (short)cp.nz;
-        ml.val[p * ml.cap + n] = make_float2(-cp.w * u.y * inv_dx, cp.w * u.x * inv_dx);
-        ++n;
-    }
-    ml.cnt[p] = n;
-}
-
-// Z pass of the fit: forward z-FFT, R = G^T (G F - m) accumulated per pencil in angle order (thread t
-// owns the z indices t + T j), F -= tau R / hits, inverse z-FFT.
-template <int V, int B>
-__global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zpgd(float2* __restrict__ W, const float2* __restrict__ tw,
-                                                            MeasList ml, float tau)
-{
-    constexpr int T = Plan<V>::T;
-    constexpr int LS = LdsStride<V, B>::value;
-    __shared__ float2 lds[B * LS];
-    __shared__ float2 twl[LdsSize<V>::value];
-    __shared__ float2 racc[B][V + 1];
-    __shared__ unsigned short hit[B][V + 2];
-    __shared__ short lnz[B][GRID_CHUNK];
-    __shared__ float2 lval[B][GRID_CHUNK + 1];
-    __shared__ unsigned lmask[B][T];  // owner masks of the staged chunk
-    stage_twiddles<V>(twl, tw);
-    const TwPad twp{twl};
-    Tile<V, 2, B> tl;
-    float2* s = lds + tl.b * LS;
-    for (int z = tl.t; z < V; z += T) {
-        racc[tl.b][z] = make_float2(0.f, 0.f);
-        hit[tl.b][z] = 0;
-    }
-    lmask[tl.b][tl.t] = 0;  // the forward transform's barrier orders this before the first marks
-    float2* pen = W + tl.base;
-    auto gl = [&](int z) -> float2 { return pen[(long long)z * tl.stride]; };
-    auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
-    Plan<V>::template run<-1>(tl.t, s, twp, gl, gs);
-    __syncthreads();
-    const long long pencil = tl.base;  // x + V*y
-    const int n = min(ml.cnt[pencil], ml.cap);
-    for (int e0 = 0; e0 < n; e0 += GRID_CHUNK) {
-        const int ne = min(GRID_CHUNK, n - e0);
-        for (int k = tl.t; k < ne; k += T) {
-            const int z = ml.idx[pencil * ml.cap + e0 + k];
-            lnz[tl.b][k] = (short)z;
-            lval[tl.b][k] = ml.val[pencil * ml.cap + e0 + k];
-            if ((unsigned)z < (unsigned)V) mark_owner<T>(lmask[tl.b], z, k);
-        }
-        __syncthreads();
-        unsigned mine = lmask[tl.b][tl.t];  // this thread's entries, in list order
-        lmask[tl.b][tl.t] = 0;  // the barrier closing this chunk orders the clear before new marks
-        while (mine) {
-            const int k = __ffs(mine) - 1;
-            mine &= mine - 1;
-            const int z = lnz[tl.b][k];
-            const float2 F = s[lds_pad(z)], m = lval[tl.b][k];
-            racc[tl.b][z].x += F.x - m.x;
-            racc[tl.b][z].y += F.y - m.y;
-            hit[tl.b][z] += 1;
-        }
-        __syncthreads();
-    }
-    for (int z = tl.t; z < V; z += T) {
-        const int h = hit[tl.b][z];
-        if (h) {
-            const float c = tau / (float)h;
-            float2 F = s[lds_pad(z)];
-            F.x -= c * racc[tl.b][z].x;
-            F.y -= c * racc[tl.b][z].y;
-            s[lds_pad(z)] = F;
+            if (z >= 0 && z < V) put(z, dc.val[e]);
         }
     }
     __syncthreads();
@@ -779,13 +687,14 @@ __global__ void k_dc_build(const float2* __restrict__ Fm, const float* __restric
 }
 
 // Gridding pass: per k-space pencil (x, y) accumulate every angle's sample in angle order, average
-// by hit count, scale by 1/dx (Initial_F_F2), emit the compact list, then inverse z-FFT.
+// by hit count, scale by 1/dx (Initial_F_F2), emit the compact list, then (VOLUME) inverse z-FFT
+// into W.
 // The pencil's T threads evaluate the caps of interleaved angles into an LDS list (chunks of
 // GRID_CHUNK angles); thread t then owns the z indices t + T j and adds the list entries that land
 // there in angle order, so the sums are deterministic. A pencil's list holds thread t's entries
 // (ascending z) after those of threads below t; the data-consistency step only needs each measured z
 // once, in any order.
-template <int V, int B>
+template <int V, int B, bool VOLUME>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __restrict__ spec,
                                                             const double* __restrict__ rows, int A,
                                                             double fm02, double df, float inv_dx,
@@ -907,25 +816,31 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
         }
     }
     if (tl.t == T - 1) dc.cnt[pencil] = off;
+    if constexpr (!VOLUME) return;  // list only (the fit's measurement list; W is not touched)
     float2* pen = W + tl.base;
     auto gl = [&](int k) -> float2 { return s[lds_pad(k)]; };
     auto gs = [&](int z, float2 v) { pen[VV * z] = v; };
     Plan<V>::template run<+1>(tl.t, s, tw, gl, gs);  // twiddles from the (cached) global table
 }
 
-static DcList dc_alloc(Ctx& c, int V, int cap)
+static DcList list_alloc(Ctx& c, const std::string& prefix, int V, int cap)
 {
     const size_t VV = (size_t)V * V;
+    DcList d;
+    d.cap = cap;
+    d.cnt = (int*)c.buf(prefix + ".cnt", sizeof(int) * VV);
+    d.idx = (short*)c.buf(prefix + ".idx", sizeof(short) * VV * cap);
+    d.val = (float2*)c.buf(prefix + ".val", sizeof(float2) * VV * cap);
+    return d;
+}
+
+static DcList dc_alloc(Ctx& c, int V, int cap)
+{
     if (c.lane == 0) {  // the list the pass-timing helpers reuse is lane 0's
         c.dc_V = V;
         c.dc_cap = cap;
     }
-    DcList d;
-    d.cap = cap;
-    d.cnt = (int*)c.buf("dc.cnt", sizeof(int) * VV);
-    d.idx = (short*)c.buf("dc.idx", sizeof(short) * VV * cap);
-    d.val = (float2*)c.buf("dc.val", sizeof(float2) * VV * cap);
-    return d;
+    return list_alloc(c, "dc", V, cap);
 }
 
 // Launchers of the fused passes for one pencils-per-workgroup choice B.
@@ -968,22 +883,22 @@ struct VolOps {
             vlaunch(c, &k_vol_y<V, B, +1>, grid(), dim3(NT), W, tw);
         CTDM_CHECK_LAUNCH("k_vol_y");
     }
-    static void zdc(Ctx& c, float2* W, const DcList& dc)
+    static void zdc(Ctx& c, float2* W, const DcList& dc, float tau)
     {
         const float2* tw = c.tw(V);
-        vlaunch(c, &k_vol_zdc<V, B>, grid(), dim3(NT), W, tw, dc);
+        vlaunch(c, &k_vol_zdc<V, B>, grid(), dim3(NT), W, tw, dc, tau);
         CTDM_CHECK_LAUNCH("k_vol_zdc");
     }
-    static void zpgd(Ctx& c, float2* W, const MeasList& ml, float tau)
-    {
-        hipLaunchKernelGGL((k_vol_zpgd<V, B>), grid(), dim3(NT), 0, c.stream, W, c.tw(V), ml, tau);
-        CTDM_CHECK_LAUNCH("k_vol_zpgd");
-    }
+    // W == nullptr: the list only.
     static void grid_pass(Ctx& c, const float2* spec, const double* rows, int A, const EwaldGeom& g, bool ref,
                           float2* W, const DcList& dc)
     {
-        hipLaunchKernelGGL((k_vol_grid<V, B>), grid(), dim3(NT), 0, c.stream, spec, rows, A, g.fm02, g.df,
-                           (float)(1.0 / g.dx), (int)ref, c.tw(V), W, dc);
+        if (W)
+            hipLaunchKernelGGL((k_vol_grid<V, B, true>), grid(), dim3(NT), 0, c.stream, spec, rows, A, g.fm02,
+                               g.df, (float)(1.0 / g.dx), (int)ref, c.tw(V), W, dc);
+        else
+            hipLaunchKernelGGL((k_vol_grid<V, B, false>), grid(), dim3(NT), 0, c.stream, spec, rows, A, g.fm02,
+                               g.df, (float)(1.0 / g.dx), (int)ref, c.tw(V), W, dc);
         CTDM_CHECK_LAUNCH("k_vol_grid");
     }
 };
@@ -1004,10 +919,10 @@ struct Vol {
         if (pick(c, PASS_Y) == 8) VolOps<V, 8>::y(c, W, dir);
         else VolOps<V, 16>::y(c, W, dir);
     }
-    static void zdc(Ctx& c, float2* W, const DcList& dc)
+    static void zdc(Ctx& c, float2* W, const DcList& dc, float tau = 1.f)
     {
-        if (pick(c, PASS_ZDC) == 8) VolOps<V, 8>::zdc(c, W, dc);
-        else VolOps<V, 16>::zdc(c, W, dc);
+        if (pick(c, PASS_ZDC) == 8) VolOps<V, 8>::zdc(c, W, dc, tau);
+        else VolOps<V, 16>::zdc(c, W, dc, tau);
     }
     static void grid_pass(Ctx& c, const float2* spec, const double* rows, int A, const EwaldGeom& g, bool ref,
                           float2* W, const DcList& dc)
@@ -1015,16 +930,11 @@ struct Vol {
         if (pick(c, PASS_GRID) == 8) VolOps<V, 8>::grid_pass(c, spec, rows, A, g, ref, W, dc);
         else VolOps<V, 16>::grid_pass(c, spec, rows, A, g, ref, W, dc);
     }
-    static void zpgd(Ctx& c, float2* W, const MeasList& ml, float tau)
-    {
-        if (pick(c, PASS_ZDC) == 8) VolOps<V, 8>::zpgd(c, W, ml, tau);
-        else VolOps<V, 16>::zpgd(c, W, ml, tau);
-    }
     // Loop after the initial inverse transform of the measured spectrum is in W up to the x pass:
     // `first_x_mode` 0 if W holds real-space f, 1 if W still needs its inverse x-FFT. The z step is
-    // the data-consistency replacement (dc) or, when `ml` is given, the fit's gradient step.
+    // the data-consistency replacement (tau = 1) or the fit's relaxed step F <- F + tau (m - F).
     static void loop(Ctx& c, float2* W, float2* n_out, const DcList& dc, const Proj& p, int iters, bool shift,
-                     int first_x_mode, const MeasList* ml = nullptr, float tau = 1.f)
+                     int first_x_mode, float tau = 1.f)
     {
         const float s = (float)(1.0 / ((double)V * V * V));
         if (iters == 0) {
@@ -1037,10 +947,7 @@ struct Vol {
         x(c, first_x_mode, W, n_out, p, s, shift);
         for (int it = 0; it < iters; ++it) {
             y(c, W, -1);
-            if (ml)
-                zpgd(c, W, *ml, tau);
-            else
-                zdc(c, W, dc);
+            zdc(c, W, dc, tau);
             y(c, W, +1);
             x(c, it + 1 < iters ? 1 : 2, W, n_out, p, s, shift);
         }
@@ -1105,21 +1012,22 @@ void volume_batched(Ctx& c, const float2* spec, const ctdm_angle* ang, int A, in
     }
 }
 
+// Projected-gradient fit. With F = FFT3(f), m_a(p) = 4 pi i f_z spec_a(p) / dx (the sample the reference
+// scatters, Recon_kernel.cu:224-225, with the 1/dx of Initial_F_F2, :269-270) and G the unit-weight
+// Ewald gather (F -> F[v_a(p)], adjoint G^T = the scatter), J(f) = 1/2 sum_a |G_a F - m_a|^2 has the
+// k-space gradient R = G^T (G F - m). G selects one voxel per (angle, pencil), so G^T G = H, the
+// diagonal of hit counts, and R = H F - G^T m. The hit-preconditioned step F <- F - tau H^-1 R is
+// therefore F <- F + tau (m-bar - F) on the measured voxels, with m-bar = H^-1 G^T m the averaged
+// measurement. m-bar is formed once by the gridding kernel (list only); each iteration then recomputes
+// the 3-D FFT and runs the relaxed data-consistency pass, then positivity (est_n) in real space.
+// tau = 1 is the reference's loop (check_F3D replacement).
 template <int V>
 static void fit_pgd_v(Ctx& c, float2* f, const float2* spec, const double* rows, int A, const EwaldGeom& g,
                       int iters, float tau, float2* n, unsigned flags)
 {
-    const size_t VV = (size_t)V * V;
-    MeasList ml;
-    ml.cap = std::max(A, 1);
-    ml.cnt = (int*)c.buf("pgd.cnt", sizeof(int) * VV);
-    ml.idx = (short*)c.buf("pgd.idx", sizeof(short) * VV * ml.cap);
-    ml.val = (float2*)c.buf("pgd.val", sizeof(float2) * VV * ml.cap);
-    hipLaunchKernelGGL(k_pgd_build, dim3(grid1d((long long)VV, 256)), dim3(256), 0, c.stream, spec, rows, A, V,
-                       g.fm02, g.df, (float)(1.0 / g.dx), ml);
-    CTDM_CHECK_LAUNCH("k_pgd_build");
-    DcList none{};
-    Vol<V>::loop(c, f, n, none, make_proj(g), iters, !(flags & CTDM_POS_NO_SHIFT), 0, &ml, tau);
+    const DcList ml = list_alloc(c, "pgd", V, std::max(A, 1));
+    Vol<V>::grid_pass(c, spec, rows, A, g, false, nullptr, ml);  // m-bar = H^-1 G^T m per measured voxel
+    Vol<V>::loop(c, f, n, ml, make_proj(g), iters, !(flags & CTDM_POS_NO_SHIFT), 0, tau);
 }
 
 void fit_pgd(Ctx& c, float2* f, const float2* spec, const ctdm_angle* ang, int A, int V, const ctdm_optics& o,
@@ -1198,7 +1106,7 @@ static double time_pass_v(Ctx& c, int kind, int B, int reps, double* bytes)
         case PASS_X: B == 8 ? VolOps<V, 8>::x(c, 1, W, nbuf, p, 1.f / (float)vol, true)
                             : VolOps<V, 16>::x(c, 1, W, nbuf, p, 1.f / (float)vol, true); break;
         case PASS_Y: B == 8 ? VolOps<V, 8>::y(c, W, (r & 1) ? 1 : -1) : VolOps<V, 16>::y(c, W, (r & 1) ? 1 : -1); break;
-        case PASS_ZDC: B == 8 ? VolOps<V, 8>::zdc(c, W, dc) : VolOps<V, 16>::zdc(c, W, dc); break;
+        case PASS_ZDC: B == 8 ? VolOps<V, 8>::zdc(c, W, dc, 1.f) : VolOps<V, 16>::zdc(c, W, dc, 1.f); break;
         default: fail(CTDM_EINVAL, "time_pass: unknown pass");
         }
     };
