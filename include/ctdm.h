@@ -314,10 +314,12 @@ ctdm_status ctdm_rimap_read(const char *path, ctdm_complex *ri, size_t n, int pa
  * data-consistency loop of ctdm_positivity). Minimises
  *     J(f) = 1/2 sum_a sum_p |FFT3(f)[v_a(p)] - m_a(p)|^2,   m_a(p) = 4*pi*i*f_z * spec_a(p) / dx,
  * over the scattering potential f (v_a(p) and f_z as in ctdm_ewald_scatter). Each of `iters` steps
- * recomputes F = FFT3(f), forms the gradient R = G^T (G F - m) with the hand-written gather G and its
- * transpose, takes F -= step * R / hits (hit-preconditioned), returns to real space and projects onto
- * the positivity set (Re n >= n_med, Im n >= 0). step = 1 reproduces ctdm_positivity fed with the
- * averaged measurement. spec: A x V x V Rytov spectra (ctdm_rytov_spectrum). f: start potential,
+ * recomputes F = FFT3(f), takes the hit-preconditioned gradient step F -= step * R / hits with
+ * R = G^T (G F - m) (G the gather, G^T the scatter), returns to real space and projects onto the
+ * positivity set (Re n >= n_med, Im n >= 0). G picks one voxel per (angle, pencil), so G^T G is the
+ * diagonal of hit counts and the step equals F += step * (m_bar - F) on measured voxels, m_bar the
+ * hit-averaged scatter of m; m_bar is formed once per call. step = 1 reproduces ctdm_positivity fed
+ * with the averaged measurement. spec: A x V x V Rytov spectra (ctdm_rytov_spectrum). f: start potential,
  * used as workspace (overwritten). n: est_n of the result, octant-shifted unless CTDM_POS_NO_SHIFT. */
 ctdm_status ctdm_fit_pgd(ctdm_ctx *ctx, ctdm_complex *f, const ctdm_complex *spec,
                          const ctdm_angle *angles, int A, int V, const ctdm_optics *optics,
