@@ -150,7 +150,10 @@ __device__ __forceinline__ void idct_refcompat(int t, const float* X, float2* s,
 }
 
 // ---------------------------------------------------------------- 2. columns: DCT-II, / lambda, DCT-III
-// LDS per column: complex area (LdsSize<LN>) + N real coefficients.
+// LDS per column: one complex area (LdsSize<LN>). The N real coefficients between the two transforms
+// are held in registers while the area is read, then written into it as floats: the DCT-III reads them
+// only in its first stage, before its first write to the area. One area per column keeps 60 maps of
+// 256 columns within one round of workgroups.
 template <int N, int C, int LN>
 __global__ __launch_bounds__(C* Plan<LN>::T) void k_uw_cols(const float* __restrict__ in, int batch,
                                                             unsigned flags, const double* __restrict__ eig,
@@ -161,31 +164,43 @@ __global__ __launch_bounds__(C* Plan<LN>::T) void k_uw_cols(const float* __restr
 {
     constexpr int TL = Plan<LN>::T;  // threads per column (sized for the longest transform used)
     constexpr int LS = LdsStride<LN, C>::value;
+    static_assert(2 * LdsSize<LN>::value >= N, "coefficients must fit the complex area");
+    constexpr int J = (N + TL - 1) / TL;  // coefficients per thread
     __shared__ float2 lds[C * LS];
-    __shared__ float coef[C][N + 1];
     const int cidx = threadIdx.x % C, t = threadIdx.x / C;
     const int tiles = N / C;
     const long long b = blockIdx.x / tiles;
     const int kx = (blockIdx.x % tiles) * C + cidx;
     const float* col = in + b * N * N + kx;
     float2* s = lds + cidx * LS;
-    float* X = coef[cidx];
+    float* X = reinterpret_cast<float*>(s);  // valid once the forward transform's output has been read
     auto gl = [&](int n) -> float2 { return make_float2(col[(long long)mk_src(n, N) * N], 0.f); };
     auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };
     Plan<N>::template run<-1>(t, s, tw_n, gl, gs);
     __syncthreads();
     const float s0 = sqrtf(1.f / N), s1 = sqrtf(2.f / N);
     const bool refeig = flags & F_REF_EIG;
-    for (int ky = t; ky < N; ky += TL) {
-        const float2 V = s[lds_pad(ky)];
-        const float2 w = tw_4n[ky];
-        const float Xv = (V.x * w.x - V.y * w.y) * (ky == 0 ? s0 : s1);
-        double lam;
-        if (refeig)
-            lam = eig[kx] + eig[ky] - 4.0;
-        else
-            lam = -4.0 * (eig[kx] + eig[ky]);
-        X[ky] = (kx == 0 && ky == 0) ? 0.f : (float)((double)Xv / lam);
+    float xv[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int ky = t + j * TL;
+        if (t < TL && ky < N) {
+            const float2 V = s[lds_pad(ky)];
+            const float2 w = tw_4n[ky];
+            const float Xv = (V.x * w.x - V.y * w.y) * (ky == 0 ? s0 : s1);
+            double lam;
+            if (refeig)
+                lam = eig[kx] + eig[ky] - 4.0;
+            else
+                lam = -4.0 * (eig[kx] + eig[ky]);
+            xv[j] = (kx == 0 && ky == 0) ? 0.f : (float)((double)Xv / lam);
+        }
+    }
+    __syncthreads();  // every read of the transform's output precedes the coefficients' writes
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int ky = t + j * TL;
+        if (t < TL && ky < N) X[ky] = xv[j];
     }
     __syncthreads();
     float* dcol = out + b * N * N + kx;
@@ -209,15 +224,14 @@ __global__ __launch_bounds__(G* Plan<LN>::T) void k_uw_rows_inv(const float* __r
 {
     constexpr int TL = Plan<LN>::T;
     __shared__ float2 lds[G * LdsSize<LN>::value];
-    __shared__ float coef[G][N + 1];
     const int g = threadIdx.x / TL, t = threadIdx.x % TL;
     const long long row = (long long)blockIdx.x * G + g;
     const bool valid = row < (long long)batch * N;
     const long long r = valid ? row : 0;
     float2* s = lds + g * LdsSize<LN>::value;
-    float* X = coef[g];
-    for (int k = t; k < N; k += TL) X[k] = in[r * N + k];
-    __syncthreads();
+    // The DCT-III's first stage reads coefficients k and N - k of the row straight from the input (the
+    // row's 1 KiB stays in cache); no LDS copy, so 60 maps of 256 rows fit one round of workgroups.
+    const float* X = in + r * N;
     const bool cong = (flags & F_CONGRUENT) && wrapped;
     auto put = [&](int x, float v) {
         if (!valid) return;
