@@ -1,5 +1,6 @@
 // C ABI of libctdm: argument checking, status conversion and the context object. The compute lives
 // in the k_*.hip modules; this file never launches kernels itself.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -59,8 +60,15 @@ void Ctx::ensure_aux()
 
 unsigned long long* Ctx::host_words(int n)
 {
-    CTDM_REQUIRE(n > 0 && n <= 64, "host_words: at most 64 words");
-    if (!host_words_) CTDM_HIP(hipHostMalloc((void**)&host_words_, 64 * sizeof(unsigned long long), hipHostMallocDefault));
+    CTDM_REQUIRE(n > 0, "host_words: empty request");
+    if (n > host_words_n_) {
+        if (host_words_) CTDM_HIP(hipHostFree(host_words_));
+        host_words_ = nullptr;
+        host_words_n_ = 0;
+        const int cap = std::max(n, 64);
+        CTDM_HIP(hipHostMalloc((void**)&host_words_, cap * sizeof(unsigned long long), hipHostMallocDefault));
+        host_words_n_ = cap;
+    }
     return host_words_;
 }
 
@@ -70,6 +78,15 @@ void Ctx::spin_sync()
         const hipError_t e = hipStreamQuery(stream);
         if (e == hipSuccess) return;
         if (e != hipErrorNotReady) CTDM_HIP(e);
+    }
+}
+
+void Ctx::spin_event(hipEvent_t e)
+{
+    for (;;) {
+        const hipError_t q = hipEventQuery(e);
+        if (q == hipSuccess) return;
+        if (q != hipErrorNotReady) CTDM_HIP(q);
     }
 }
 
@@ -153,6 +170,7 @@ ctdm_status ctdm_destroy(ctdm_ctx* ctx)
     }
     for (hipEvent_t e : ctx->c.lane_ev)
         if (e) hipEventDestroy(e);
+    if (ctx->c.wait_ev) hipEventDestroy(ctx->c.wait_ev);
     delete ctx;
     return CTDM_OK;
 }

@@ -98,18 +98,40 @@ struct Ctx {
 
     const float2* tw(int n);
     // Scratch buffer `name` of at least `bytes` bytes (contents undefined). Lane 1 has its own copy
-    // unless `shared` (read-only tables written once, outside any lane's work).
+    // unless `shared` (buffers written only on the context stream outside the lanes' work: read-only
+    // tables, per-cell crop slots).
     void* buf(const std::string& name, size_t bytes, bool shared = false);
-    // Pinned host words for small results read back from the device (at most 64; contents undefined).
+    // Pinned host words for small results read back from the device (contents undefined; a larger
+    // request replaces the buffer, so no device write to it may be pending).
     unsigned long long* host_words(int n);
+    int host_words_n_ = 0;
     // Waits for the context stream by polling it: a host-synchronous search returns as soon as its last
     // command has finished, without the wake-up latency of a blocking synchronisation.
     void spin_sync();
+    // The same for an event (made on first use): waits for the work recorded before it only.
+    hipEvent_t wait_ev = nullptr;
+    void spin_event(hipEvent_t e);
     Tune& tuning(int n) { return tune[n]; }
 };
 
 // ---------------------------------------------------------------- module launchers
 // Retrieval (k_retrieve.hip)
+// Crop of the batched retrieval as its kernels read it from device memory, so that one launch sequence
+// (and one captured graph) serves any carrier position: the crop itself and the passband it implies
+// (band [lo, lo + n) of offsets, the same for rows and columns; k_lo = col - W/2 + lo; r0 = row - H/2).
+struct CropDev {
+    int row, col, radius, M;
+    int lo, n, k_lo, r0;
+};
+// Writes `slot` from a host crop with a one-thread kernel (no copy node, so it may be captured).
+void set_crop(Ctx& c, CropDev* slot, const ctdm_crop& cr, int H, int W);
+// Queues the sideband search of one background image without waiting: the crop goes to `slot` (if not
+// null) and the 64-bit search key to `key_dev` (device-visible pinned host memory).
+void enqueue_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, CropDev* slot,
+                      unsigned long long* key_dev);
+// The crop a search key stands for (obtainRadius's row, column and radius).
+ctdm_crop decode_sideband(unsigned long long key, int H, int W, int M);
+void validate_crop(int H, int W, const ctdm_crop& cr);
 void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, ctdm_crop* out);
 int find_sideband_rows(Ctx& c, const float* bg, int H, int W);
 void angle_from_image(Ctx& c, const float* img, int batch, int H, int W, double pixel_um, double mag,
@@ -118,6 +140,9 @@ void retrieve_rows(Ctx& c, const float* holo, const float* bg, int batch, int H,
                    float* logamp);
 void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int H, int W,
                       const ctdm_crop& crop, float* wrapped, float* logamp, float2* field);
+// The same with the crop read from device memory (crop size M fixed on the host).
+void retrieve_batched_dev(Ctx& c, const float* holo, const float* bg, int batch, int H, int W, int M,
+                          const CropDev* cd, float* wrapped, float* logamp, float2* field);
 // Reference-structured retrieval of one hologram pair (full 2-D FFTs, shift, crop, inverse,
 // scale+conjugate, calcWrapPhase semantics: phase in (-2pi, 2pi), amplitude ratio).
 void retrieve_plain_one(Ctx& c, const float* holo, const float* bg, int H, int W,

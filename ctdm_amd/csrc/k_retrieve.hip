@@ -35,6 +35,30 @@ static Band passband(int radius, int M)
     return Band{lo, hi - lo + 1};
 }
 
+// Crop and passband as the kernels read them (same arithmetic as passband()). For any row, column and
+// radius >= 0 the band has n <= M entries and every index derived from it is taken modulo H or W, so
+// a located crop that later fails validation cannot make a kernel access memory out of bounds.
+__host__ __device__ inline CropDev make_crop_dev(int row, int col, int radius, int M, int H, int W)
+{
+    CropDev d;
+    d.row = row;
+    d.col = col;
+    d.radius = radius;
+    d.M = M;
+    const int r = radius < 0 ? 0 : radius;
+    d.lo = -M / 2 > -r ? -M / 2 : -r;
+    const int hi = M / 2 - 1 < r ? M / 2 - 1 : r;
+    d.n = hi - d.lo + 1;
+    d.k_lo = col - W / 2 + d.lo;
+    d.r0 = row - H / 2;
+    return d;
+}
+
+__global__ void k_set_crop(CropDev* __restrict__ slot, int row, int col, int radius, int M, int H, int W)
+{
+    *slot = make_crop_dev(row, col, radius, M, H, W);
+}
+
 // ---------------------------------------------------------------- 1. real row FFT, pruned output
 // Each row of W reals is read as W/2 complex z[n] = x[2n] + i x[2n+1]; Z = FFT_{W/2}(z);
 // X[k] = (Z[k] + conj Z[-k]) / 2 - i/2 * exp(-2 pi i k / W) * (Z[k] - conj Z[-k]).
@@ -44,9 +68,13 @@ template <int W, int G>
 __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
     const float* __restrict__ src0, int n0, const float* __restrict__ src1, int nimg, int H,
     const float2* __restrict__ tw_half, const float2* __restrict__ tw_full, int k_lo, int nout,
-    float2* __restrict__ out)
+    const CropDev* __restrict__ cd, float2* __restrict__ out)
 {
     constexpr int NH = W / 2;
+    if (cd) {  // crop located on the device
+        k_lo = cd->k_lo;
+        nout = cd->n;
+    }
     constexpr int T = Plan<NH>::T;
     __shared__ float2 lds[G * LdsSize<NH>::value];
     __shared__ float2 twl[LdsSize<NH>::value];
@@ -87,16 +115,23 @@ __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
 // out: G[img][y'][q], y' < M.
 template <int H, int M, int C>
 __global__ __launch_bounds__(C* Plan<H>::T) void k_col_band(
-    const float2* __restrict__ in, int nimg, int nin, const float2* __restrict__ tw_h,
-    const float2* __restrict__ tw_m, int r0, int q_lo, int q_n, float2* __restrict__ out)
+    const float2* __restrict__ in, int nimg, int nin, int tiles, const float2* __restrict__ tw_h,
+    const float2* __restrict__ tw_m, int r0, int q_lo, int q_n, const CropDev* __restrict__ cd,
+    float2* __restrict__ out)
 {
     constexpr int T = Plan<H>::T;
     constexpr int LS = LdsStride<H, C>::value;
     static_assert(LdsSize<M>::value <= LS, "band transform must fit the column area");
     __shared__ float2 lds[C * LS];
+    if (cd) {  // crop located on the device; the grid covers the widest band (M columns)
+        nin = cd->n;
+        r0 = cd->r0;
+        q_lo = cd->lo;
+        q_n = cd->n;
+    }
     const int cidx = threadIdx.x % C, t = threadIdx.x / C;  // lanes walk columns (coalesced)
-    const int tiles = (nin + C - 1) / C;
     const int blk = xcd_block(blockIdx.x, gridDim.x);  // neighbouring column tiles share lines
+    if ((blk % tiles) * C >= nin) return;  // a whole tile beyond the band (uniform, before any barrier)
     const int img = blk / tiles;
     const int q = (blk % tiles) * C + cidx;
     const bool valid = q < nin && img < nimg;
@@ -128,10 +163,15 @@ __global__ __launch_bounds__(C* Plan<H>::T) void k_col_band(
 //   field = g_SP / g_BG (optional).
 template <int M, int GR>
 __global__ __launch_bounds__(2 * GR * Plan<M>::T) void k_row_final(
-    const float2* __restrict__ in, int batch, int nin, int p_lo, const float2* __restrict__ tw_m,
-    float* __restrict__ wrapped, float* __restrict__ logamp, float2* __restrict__ field)
+    const float2* __restrict__ in, int batch, int nin, int p_lo, const CropDev* __restrict__ cd,
+    const float2* __restrict__ tw_m, float* __restrict__ wrapped, float* __restrict__ logamp,
+    float2* __restrict__ field)
 {
     constexpr int T = Plan<M>::T;
+    if (cd) {  // crop located on the device
+        nin = cd->n;
+        p_lo = cd->lo;
+    }
     __shared__ float2 lds[2 * GR * LdsSize<M>::value];
     const int tr = threadIdx.x / T, t = threadIdx.x % T;  // tr in [0, 2*GR): SP rows then BG rows
     const int which = tr / GR, r = tr % GR;
@@ -231,13 +271,21 @@ __global__ __launch_bounds__(C* Plan<H>::T) void k_col_argmax(const float2* __re
 }
 
 // Final maximum over the workgroup maxima, written to pinned host memory (one block).
+// When `slot` is given, the crop the key stands for (decode_sideband) is written there too.
 __global__ __launch_bounds__(256) void k_publish_max(const unsigned long long* __restrict__ part, int n,
-                                                     unsigned long long* __restrict__ host)
+                                                     unsigned long long* __restrict__ host,
+                                                     CropDev* __restrict__ slot, int H, int W, int M)
 {
     unsigned long long v = 0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) v = part[i] > v ? part[i] : v;
     v = block_max_u64(v);
     if (threadIdx.x == 0) {
+        if (slot) {
+            const long long idx = (long long)(0xffffffffull - (v & 0xffffffffull));
+            const int row = (int)(idx / W), col = (int)(idx % W);
+            const int dr = row - H / 2;
+            *slot = make_crop_dev(row, col, (dr < 0 ? -dr : dr) / 3, M, H, W);
+        }
         *host = v;
         __threadfence_system();
     }
@@ -247,62 +295,63 @@ __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
 template <int W>
 static void launch_row_real(Ctx& c, const float* src0, int n0, const float* src1, int nimg, int H,
-                            int k_lo, int nout, float2* out)
+                            int k_lo, int nout, const CropDev* cd, float2* out)
 {
     constexpr int G = 256 / Plan<W / 2>::T;
     const long long rows = (long long)nimg * H;
     hipLaunchKernelGGL((k_row_real<W, G>), dim3(grid1d(rows, G)), dim3(G * Plan<W / 2>::T), 0,
-                       c.stream, src0, n0, src1, nimg, H, c.tw(W / 2), c.tw(W), k_lo, nout, out);
+                       c.stream, src0, n0, src1, nimg, H, c.tw(W / 2), c.tw(W), k_lo, nout, cd, out);
     CTDM_CHECK_LAUNCH("k_row_real");
 }
 
+// cd != nullptr: k_lo and nout are read from the device crop instead.
 static void row_real(Ctx& c, const float* src0, int n0, const float* src1, int nimg, int H, int W,
-                     int k_lo, int nout, float2* out)
+                     int k_lo, int nout, const CropDev* cd, float2* out)
 {
     switch (W) {
-    case 256: launch_row_real<256>(c, src0, n0, src1, nimg, H, k_lo, nout, out); break;
-    case 512: launch_row_real<512>(c, src0, n0, src1, nimg, H, k_lo, nout, out); break;
-    case 1024: launch_row_real<1024>(c, src0, n0, src1, nimg, H, k_lo, nout, out); break;
+    case 256: launch_row_real<256>(c, src0, n0, src1, nimg, H, k_lo, nout, cd, out); break;
+    case 512: launch_row_real<512>(c, src0, n0, src1, nimg, H, k_lo, nout, cd, out); break;
+    case 1024: launch_row_real<1024>(c, src0, n0, src1, nimg, H, k_lo, nout, cd, out); break;
     default: fail(CTDM_EUNSUPPORTED, "hologram width " + std::to_string(W) + " has no kernel");
     }
 }
 
 template <int H, int M>
 static void launch_col_band(Ctx& c, const float2* in, int nimg, int nin, int r0, const Band& qb,
-                            float2* out)
+                            const CropDev* cd, float2* out)
 {
     constexpr int C = 256 / Plan<H>::T;
-    const int tiles = (nin + C - 1) / C;
+    const int tiles = (nin + C - 1) / C;  // with a device crop, nin is the widest band (M)
     hipLaunchKernelGGL((k_col_band<H, M, C>), dim3(tiles * nimg), dim3(C * Plan<H>::T), 0, c.stream,
-                       in, nimg, nin, c.tw(H), c.tw(M), r0, qb.lo, qb.n, out);
+                       in, nimg, nin, tiles, c.tw(H), c.tw(M), r0, qb.lo, qb.n, cd, out);
     CTDM_CHECK_LAUNCH("k_col_band");
 }
 
 template <int M>
-static void launch_row_final(Ctx& c, const float2* in, int batch, int nin, int p_lo, float* wrapped,
-                             float* logamp, float2* field)
+static void launch_row_final(Ctx& c, const float2* in, int batch, int nin, int p_lo, const CropDev* cd,
+                             float* wrapped, float* logamp, float2* field)
 {
     constexpr int GR = 128 / Plan<M>::T;
     hipLaunchKernelGGL((k_row_final<M, GR>), dim3(batch * (M / GR)), dim3(2 * GR * Plan<M>::T), 0,
-                       c.stream, in, batch, nin, p_lo, c.tw(M), wrapped, logamp, field);
+                       c.stream, in, batch, nin, p_lo, cd, c.tw(M), wrapped, logamp, field);
     CTDM_CHECK_LAUNCH("k_row_final");
 }
 
 template <int H>
 static void col_band_m(Ctx& c, int M, const float2* in, int nimg, int nin, int r0, const Band& qb,
-                       float2* out)
+                       const CropDev* cd, float2* out)
 {
     switch (M) {
-    case 32: launch_col_band<H, 32>(c, in, nimg, nin, r0, qb, out); break;
-    case 64: launch_col_band<H, 64>(c, in, nimg, nin, r0, qb, out); break;
-    case 128: launch_col_band<H, 128>(c, in, nimg, nin, r0, qb, out); break;
-    case 192: launch_col_band<H, 192>(c, in, nimg, nin, r0, qb, out); break;
-    case 256: launch_col_band<H, 256>(c, in, nimg, nin, r0, qb, out); break;
+    case 32: launch_col_band<H, 32>(c, in, nimg, nin, r0, qb, cd, out); break;
+    case 64: launch_col_band<H, 64>(c, in, nimg, nin, r0, qb, cd, out); break;
+    case 128: launch_col_band<H, 128>(c, in, nimg, nin, r0, qb, cd, out); break;
+    case 192: launch_col_band<H, 192>(c, in, nimg, nin, r0, qb, cd, out); break;
+    case 256: launch_col_band<H, 256>(c, in, nimg, nin, r0, qb, cd, out); break;
     default: fail(CTDM_EUNSUPPORTED, "crop size " + std::to_string(M) + " has no kernel");
     }
 }
 
-static void validate_crop(int H, int W, const ctdm_crop& cr)
+void validate_crop(int H, int W, const ctdm_crop& cr)
 {
     CTDM_REQUIRE(H == W, "retrieval: square holograms required");
     CTDM_REQUIRE(cr.M > 0 && cr.M % 2 == 0 && cr.M <= H, "retrieval: bad crop size");
@@ -312,6 +361,33 @@ static void validate_crop(int H, int W, const ctdm_crop& cr)
                  "retrieval: crop window leaves the spectrum");
 }
 
+// The three kernels for a band of width n (host crop) or of the device crop cd (buffers and grids then
+// sized for the widest band, M).
+static void retrieve_core(Ctx& c, const float* holo, const float* bg, int batch, int H, int W, int M,
+                          const Band& pb, int k_lo, int r0, const CropDev* cd, float* wrapped, float* logamp,
+                          float2* field)
+{
+    const int nimg = 2 * batch;
+    const int nmax = cd ? M : pb.n;
+    float2* R = (float2*)c.buf("ret.R", sizeof(float2) * (size_t)nimg * H * nmax);
+    float2* G = (float2*)c.buf("ret.G", sizeof(float2) * (size_t)nimg * M * nmax);
+    row_real(c, holo, batch, bg, nimg, H, W, k_lo, pb.n, cd, R);
+    switch (H) {
+    case 256: col_band_m<256>(c, M, R, nimg, nmax, r0, pb, cd, G); break;
+    case 512: col_band_m<512>(c, M, R, nimg, nmax, r0, pb, cd, G); break;
+    case 1024: col_band_m<1024>(c, M, R, nimg, nmax, r0, pb, cd, G); break;
+    default: fail(CTDM_EUNSUPPORTED, "hologram height " + std::to_string(H) + " has no kernel");
+    }
+    switch (M) {
+    case 32: launch_row_final<32>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field); break;
+    case 64: launch_row_final<64>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field); break;
+    case 128: launch_row_final<128>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field); break;
+    case 192: launch_row_final<192>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field); break;
+    case 256: launch_row_final<256>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field); break;
+    default: fail(CTDM_EUNSUPPORTED, "crop size has no kernel");
+    }
+}
+
 void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int H, int W,
                       const ctdm_crop& crop, float* wrapped, float* logamp, float2* field)
 {
@@ -319,27 +395,25 @@ void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int
     CTDM_REQUIRE(batch >= 0, "retrieve: negative batch");
     validate_crop(H, W, crop);
     if (batch == 0) return;
-    const int M = crop.M;
-    const Band pb = passband(crop.radius, M), qb = passband(crop.radius, M);
-    const int c0 = crop.col - W / 2, r0 = crop.row - H / 2;
-    const int nimg = 2 * batch;
-    float2* R = (float2*)c.buf("ret.R", sizeof(float2) * (size_t)nimg * H * pb.n);
-    float2* G = (float2*)c.buf("ret.G", sizeof(float2) * (size_t)nimg * M * pb.n);
-    row_real(c, holo, batch, bg, nimg, H, W, c0 + pb.lo, pb.n, R);
-    switch (H) {
-    case 256: col_band_m<256>(c, M, R, nimg, pb.n, r0, qb, G); break;
-    case 512: col_band_m<512>(c, M, R, nimg, pb.n, r0, qb, G); break;
-    case 1024: col_band_m<1024>(c, M, R, nimg, pb.n, r0, qb, G); break;
-    default: fail(CTDM_EUNSUPPORTED, "hologram height " + std::to_string(H) + " has no kernel");
-    }
-    switch (M) {
-    case 32: launch_row_final<32>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;
-    case 64: launch_row_final<64>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;
-    case 128: launch_row_final<128>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;
-    case 192: launch_row_final<192>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;
-    case 256: launch_row_final<256>(c, G, batch, pb.n, pb.lo, wrapped, logamp, field); break;
-    default: fail(CTDM_EUNSUPPORTED, "crop size has no kernel");
-    }
+    const Band pb = passband(crop.radius, crop.M);
+    retrieve_core(c, holo, bg, batch, H, W, crop.M, pb, crop.col - W / 2 + pb.lo, crop.row - H / 2, nullptr,
+                  wrapped, logamp, field);
+}
+
+void retrieve_batched_dev(Ctx& c, const float* holo, const float* bg, int batch, int H, int W, int M,
+                          const CropDev* cd, float* wrapped, float* logamp, float2* field)
+{
+    CTDM_REQUIRE(holo && bg && wrapped && logamp && cd, "retrieve: null pointer");
+    CTDM_REQUIRE(batch >= 0, "retrieve: negative batch");
+    CTDM_REQUIRE(H == W && M > 0 && M % 2 == 0 && M <= H, "retrieval: bad sizes");
+    if (batch == 0) return;
+    retrieve_core(c, holo, bg, batch, H, W, M, Band{0, M}, 0, 0, cd, wrapped, logamp, field);
+}
+
+void set_crop(Ctx& c, CropDev* slot, const ctdm_crop& cr, int H, int W)
+{
+    hipLaunchKernelGGL(k_set_crop, dim3(1), dim3(1), 0, c.stream, slot, cr.row, cr.col, cr.radius, cr.M, H, W);
+    CTDM_CHECK_LAUNCH("k_set_crop");
 }
 
 template <int H>
@@ -355,19 +429,17 @@ static int launch_col_argmax(Ctx& c, const float2* spec, int W, int row_start, u
     return nblk;
 }
 
-void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, ctdm_crop* out)
+void enqueue_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, CropDev* slot,
+                      unsigned long long* key_dev)
 {
-    CTDM_REQUIRE(bg && out, "find_sideband: null pointer");
+    CTDM_REQUIRE(bg && key_dev, "find_sideband: null pointer");
     CTDM_REQUIRE(H == W && (H == 256 || H == 512 || H == 1024), "find_sideband: square 256, 512 or 1024 holograms");
     if (row_start <= 0) row_start = 600 * H / 1024;
     CTDM_REQUIRE(row_start < H, "find_sideband: row_start beyond the spectrum");
     float2* spec = (float2*)c.buf("sb.spec", sizeof(float2) * (size_t)H * W);
     // Full-width row spectrum of the real image, then the column transforms fused with the band
-    // maximum; the final maximum lands in pinned host memory.
-    row_real(c, bg, 1, bg, 1, H, W, 0, W, spec);
-    unsigned long long* hb = c.host_words(1);
-    unsigned long long* hb_dev = nullptr;
-    CTDM_HIP(hipHostGetDevicePointer((void**)&hb_dev, hb, 0));
+    // maximum; the final maximum lands in pinned host memory (and the crop in `slot`).
+    row_real(c, bg, 1, bg, 1, H, W, 0, W, nullptr, spec);
     int nblk = 0;
     unsigned long long* part = (unsigned long long*)c.buf("sb.part", sizeof(unsigned long long) * 256);
     switch (H) {
@@ -375,15 +447,30 @@ void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, 
     case 512: nblk = launch_col_argmax<512>(c, spec, W, row_start, part); break;
     default: nblk = launch_col_argmax<1024>(c, spec, W, row_start, part); break;
     }
-    hipLaunchKernelGGL(k_publish_max, dim3(1), dim3(256), 0, c.stream, part, nblk, hb_dev);
+    hipLaunchKernelGGL(k_publish_max, dim3(1), dim3(256), 0, c.stream, part, nblk, key_dev, slot, H, W, M);
     CTDM_CHECK_LAUNCH("k_publish_max");
+}
+
+ctdm_crop decode_sideband(unsigned long long key, int H, int W, int M)
+{
+    const long long idx = (long long)(0xffffffffull - (key & 0xffffffffull));
+    ctdm_crop out{};
+    out.row = (int)(idx / W);
+    out.col = (int)(idx % W);
+    out.radius = std::abs(out.row - H / 2) / 3;
+    out.M = M;
+    return out;
+}
+
+void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, ctdm_crop* out)
+{
+    CTDM_REQUIRE(bg && out, "find_sideband: null pointer");
+    unsigned long long* hb = c.host_words(1);
+    unsigned long long* hb_dev = nullptr;
+    CTDM_HIP(hipHostGetDevicePointer((void**)&hb_dev, hb, 0));
+    enqueue_sideband(c, bg, H, W, M, row_start, nullptr, hb_dev);
     c.spin_sync();
-    const unsigned long long h = *hb;
-    const long long idx = (long long)(0xffffffffull - (h & 0xffffffffull));
-    out->row = (int)(idx / W);
-    out->col = (int)(idx % W);
-    out->radius = std::abs(out->row - H / 2) / 3;
-    out->M = M;
+    *out = decode_sideband(*hb, H, W, M);
 }
 
 // ---------------------------------------------------------------- illumination angle (AngCal_GPU)
@@ -429,7 +516,7 @@ void angle_from_image(Ctx& c, const float* img, int batch, int H, int W, double 
     std::vector<unsigned long long> h(batch);
     for (int b = 0; b < batch; ++b) {
         const float* im = img + (size_t)b * N;
-        row_real(c, im, 1, im, 1, H, W, 0, W, spec);
+        row_real(c, im, 1, im, 1, H, W, 0, W, nullptr, spec);
         fft_cols(c, spec, H, W, W, 1, 0, -1, 1.f, false);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c.stream, best + b);
         hipLaunchKernelGGL(k_argmax_log10, dim3(256), dim3(256), 0, c.stream, spec, H, W, lo, hi, best + b);

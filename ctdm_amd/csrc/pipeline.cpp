@@ -33,6 +33,14 @@ static void validate_cell(const ctdm_cell& cl)
     CTDM_REQUIRE(cl.optics.wavelength > 0 && cl.optics.n_med > 0 && cl.optics.dx > 0, "cell: bad optics");
 }
 
+// Device crop of a batched cell, one slot per cell (keyed by its output pointer). It is written on the
+// context stream before the cell's work each call (by the sideband search or from the given crop), so
+// the captured sequence does not depend on the carrier position.
+static CropDev* crop_slot(Ctx& c, const ctdm_cell& cl)
+{
+    return (CropDev*)c.buf("bt.crop." + std::to_string((uintptr_t)cl.ri), sizeof(CropDev), true);
+}
+
 static ctdm_crop cell_crop(Ctx& c, const ctdm_cell& cl)
 {
     if (cl.crop.radius >= 0) return cl.crop;
@@ -204,7 +212,7 @@ static void enqueue_batched(Ctx& c, ctdm_cell& cl)
     if (cl.recon_flags & CTDM_RETRIEVE_ROWS)
         retrieve_rows(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.crop.col, wr, la);
     else
-        retrieve_batched(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.crop, wr, la, nullptr);
+        retrieve_batched_dev(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.V, crop_slot(c, cl), wr, la, nullptr);
     unwrap_dct_batched(c, wr, un, A, V, cl.unwrap_flags);
     const float* un_r = un;
     const float* la_r = la;
@@ -259,11 +267,15 @@ static GraphKey graph_key(const ctdm_cell* cells, int ncells)
     GraphKey k;
     for (int i = 0; i < ncells; ++i) {
         const ctdm_cell& cl = cells[i];
+        // The batched retrieval reads its crop from the cell's device slot; only the row-wise retrieval
+        // takes the sideband column as a launch argument.
+        const bool rows = cl.recon_flags & CTDM_RETRIEVE_ROWS;
         for (uintptr_t w : {(uintptr_t)cl.holo, (uintptr_t)cl.bg, (uintptr_t)cl.ri, (uintptr_t)cl.wrapped_opt,
                             (uintptr_t)cl.unwrapped_opt, (uintptr_t)cl.logamp_opt, (uintptr_t)cl.A,
                             (uintptr_t)cl.H, (uintptr_t)cl.W, (uintptr_t)cl.V, (uintptr_t)cl.iters,
-                            (uintptr_t)cl.unwrap_flags, (uintptr_t)cl.recon_flags, (uintptr_t)cl.crop.row,
-                            (uintptr_t)cl.crop.col, (uintptr_t)cl.crop.radius, (uintptr_t)cl.crop.M})
+                            (uintptr_t)cl.unwrap_flags, (uintptr_t)cl.recon_flags,
+                            (uintptr_t)(rows ? cl.crop.row : 0), (uintptr_t)(rows ? cl.crop.col : 0),
+                            (uintptr_t)(rows ? cl.crop.radius : 0), (uintptr_t)cl.V})
             k.words.push_back(w);
         uint64_t bits[3];
         std::memcpy(&bits[0], &cl.optics.wavelength, 8);
@@ -291,7 +303,46 @@ void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& op
     }
     CTDM_REQUIRE(opts.mode == 1, "reconstruct: mode must be 0 (plain) or 1 (batched)");
     CTDM_REQUIRE(opts.lanes >= 0 && opts.lanes <= 2, "reconstruct: lanes must be 0, 1 or 2");
-    for (int i = 0; i < ncells; ++i) cells[i].crop = cell_crop(c, cells[i]);  // host-synchronous, never captured
+    // Crops, all before any cell's work and never captured. A given crop is validated and written to
+    // the cell's slot; a row-wise cell locates its column synchronously (frame 1, HilbertTransform.cu:1634).
+    // The others queue the sideband search (frame 1, HilbertTransform.cu:1572), which writes the slot on
+    // the device: the host reads the keys back only after the cells' work is queued (finish_crops), so
+    // the GPU goes from the searches straight into the cells without waiting for the host.
+    std::vector<int> located;
+    for (int i = 0; i < ncells; ++i)
+        if (!(cells[i].recon_flags & CTDM_RETRIEVE_ROWS) && cells[i].crop.radius < 0) located.push_back(i);
+    unsigned long long* keys = nullptr;
+    unsigned long long* keys_dev = nullptr;
+    if (!located.empty()) {
+        keys = c.host_words((int)located.size());
+        CTDM_HIP(hipHostGetDevicePointer((void**)&keys_dev, keys, 0));
+    }
+    for (int i = 0, k = 0; i < ncells; ++i) {
+        ctdm_cell& cl = cells[i];
+        if (cl.recon_flags & CTDM_RETRIEVE_ROWS) {
+            cl.crop = cell_crop(c, cl);
+        } else if (cl.crop.radius >= 0) {
+            validate_crop(cl.H, cl.W, cl.crop);
+            set_crop(c, crop_slot(c, cl), cl.crop, cl.H, cl.W);
+        } else {
+            enqueue_sideband(c, cl.bg, cl.H, cl.W, cl.V, 0, crop_slot(c, cl), keys_dev + k++);
+        }
+    }
+    if (!located.empty()) {
+        if (!c.wait_ev) CTDM_HIP(hipEventCreateWithFlags(&c.wait_ev, hipEventDisableTiming));
+        CTDM_HIP(hipEventRecord(c.wait_ev, c.stream));
+    }
+    // Reports the located crops (waiting for the searches only) and rejects one whose window leaves the
+    // spectrum; the cells' work is queued by then and reads the same crop, always in bounds.
+    auto finish_crops = [&] {
+        if (located.empty()) return;
+        c.spin_event(c.wait_ev);
+        for (size_t k = 0; k < located.size(); ++k) {
+            ctdm_cell& cl = cells[located[k]];
+            cl.crop = decode_sideband(keys[k], cl.H, cl.W, cl.V);
+        }
+        for (int i : located) validate_crop(cells[i].H, cells[i].W, cells[i].crop);
+    };
     if (opts.autotune) {
         std::vector<int> sizes;
         for (int i = 0; i < ncells; ++i) sizes.push_back(cells[i].V);
@@ -299,6 +350,7 @@ void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& op
     }
     if (!opts.graph) {
         enqueue_cells(c, cells, ncells, opts.lanes);
+        finish_crops();
         return;
     }
     GraphKey key = graph_key(cells, ncells);
@@ -337,9 +389,11 @@ void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& op
         CTDM_HIP(hipStreamDestroy(cs));
         for (auto& kv : c.scratch) kv.second.pinned = true;  // the graph holds these pointers
         c.graphs[key] = ge;
+        finish_crops();
         return;
     }
     CTDM_HIP(hipGraphLaunch(it->second.exec, c.stream));
+    finish_crops();
 }
 
 }  // namespace ctdm

@@ -72,3 +72,48 @@ def test_two_lanes_equal_one_lane(ctx):
         assert torch.equal(c["ri"], r)
     with pytest.raises(RuntimeError, match="lanes"):
         ctx.reconstruct(cells, mode=1, lanes=3)
+
+
+def test_graph_follows_a_moved_carrier(ctx):
+    """The batched retrieval reads its crop from device memory, written by the sideband search queued
+    ahead of the cell: a replayed graph follows a carrier that moved between calls in the same buffers,
+    reports the crop it used, and equals a fresh eager run."""
+    A, H, V = 5, 256, 64
+    srcs = [ctdm_amd.synth_cell(7, A, ctdm_amd.synth_params(H=H, carrier=cr)) for cr in ((0.25, 0.25), (0.22, 0.27))]
+    assert np.array_equal(srcs[0][2], srcs[1][2])  # same angles: the second call replays the first graph
+    hd, bd = torch.from_numpy(srcs[0][0]).cuda(), torch.from_numpy(srcs[0][1]).cuda()
+    ri = torch.empty((V, V, V), dtype=torch.complex64, device="cuda")
+    cell = dict(holo=hd, bg=bd, angles=srcs[0][2], V=V, iters=3, optics=ctdm_amd.optics_for(V, H), ri=ri)
+    ctx.reconstruct([cell], mode=1, graph=True)
+    torch.cuda.synchronize()
+    crop_a = cell["crop_used"].astuple()
+    hd.copy_(torch.from_numpy(srcs[1][0]))
+    bd.copy_(torch.from_numpy(srcs[1][1]))
+    ctx.reconstruct([cell], mode=1, graph=True)
+    torch.cuda.synchronize()
+    crop_b = cell["crop_used"].astuple()
+    print("crops", crop_a, crop_b)
+    assert crop_a != crop_b
+    fresh = dict(cell, ri=torch.empty_like(ri))
+    ctx.reconstruct([fresh], mode=1)
+    torch.cuda.synchronize()
+    assert fresh["crop_used"].astuple() == crop_b
+    assert torch.equal(ri, fresh["ri"])
+
+
+def test_located_crop_outside_the_spectrum_is_rejected(ctx):
+    """A carrier whose crop window would leave the spectrum is reported as an error once the search has
+    been read back (the queued work reads the same crop modulo the spectrum size, so it stays in bounds);
+    the context keeps working afterwards."""
+    A, H, V = 3, 256, 64
+    holo, bg, ang = ctdm_amd.synth_cell(8, A, ctdm_amd.synth_params(H=H, carrier=(0.45, 0.25)))
+    ri = torch.empty((V, V, V), dtype=torch.complex64, device="cuda")
+    cell = dict(holo=torch.from_numpy(holo).cuda(), bg=torch.from_numpy(bg).cuda(), angles=ang, V=V, iters=2,
+                optics=ctdm_amd.optics_for(V, H), ri=ri)
+    with pytest.raises(RuntimeError, match="leaves the spectrum"):
+        ctx.reconstruct([cell], mode=1)
+    torch.cuda.synchronize()
+    good = _cells([(3, 256, 64)])[0][0]
+    ctx.reconstruct([good], mode=1, graph=True)
+    torch.cuda.synchronize()
+    assert torch.isfinite(good["ri"]).all()
