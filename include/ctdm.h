@@ -244,7 +244,8 @@ typedef struct ctdm_cell {
     unsigned unwrap_flags;      /* CTDM_UNWRAP_* */
     unsigned recon_flags;       /* CTDM_DC_* | CTDM_POS_NO_SHIFT | CTDM_MAPS_MEDIAN | CTDM_RETRIEVE_ROWS */
     ctdm_optics optics;
-    ctdm_crop crop;             /* crop.M == V; crop.radius < 0: locate from bg[0] (reference frame 1) */
+    ctdm_crop crop;             /* crop.M == V; crop.radius < 0: locate from bg[0] (reference frame 1);
+                                   ctdm_reconstruct writes back the crop it used */
     ctdm_complex *ri;           /* V^3 complex RI output (device) */
     float *wrapped_opt;         /* optional A x V x V outputs (device), may be NULL */
     float *unwrapped_opt;
@@ -264,6 +265,13 @@ typedef struct ctdm_batch_opts {
     int reserved[4];
 } ctdm_batch_opts;
 
+/* Reconstructs every cell (no single reference counterpart: HilbertTransform()'s per-frame loop,
+ * HilbertTransform.cu:951-1063, then BatchRecon). Batched mode locates each crop on the device, ahead
+ * of the cell's work, and returns once those searches have finished (the cells' work is still running
+ * on the context stream); cells[i].crop then holds the crop used. A located crop whose window leaves
+ * the spectrum yields CTDM_EINVAL after the work was queued: the outputs are then undefined (the queued
+ * kernels read the crop modulo the spectrum size and stay in bounds). Plain mode and row-wise
+ * retrieval locate crops synchronously first. */
 ctdm_status ctdm_reconstruct(ctdm_ctx *ctx, ctdm_cell *cells, int ncells,
                              const ctdm_batch_opts *opts);
 
