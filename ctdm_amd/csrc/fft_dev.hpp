@@ -46,7 +46,7 @@ struct Tw32 {
         0.70710678118654735, 0.83146961230254524, 0.92387953251128652, 0.98078528040323043};
 };
 
-// In-register DFT of length R (power of two up to 32, or 3), direction DIR.
+// In-register DFT of length R (power of two up to 32, 3 or 12), direction DIR.
 template <int R, int DIR>
 struct DFT;
 
@@ -96,10 +96,34 @@ struct DFT<4, DIR> {
     }
 };
 
+// Length 12 by the Good-Thomas prime-factor map (12 = 3 x 4, coprime, no twiddles): with
+// n = (4 n1 + 3 n2) mod 12 and k = (4 k1 + 9 k2) mod 12, exp(2 pi i n k / 12) = W3^(n1 k1) W4^(n2 k2),
+// so four-point DFTs over n2 followed by three-point DFTs over n1 give the twelve-point DFT.
+template <int DIR>
+struct DFT<12, DIR> {
+    __device__ __forceinline__ static void run(float2* a)
+    {
+        float2 b[3][4];
+#pragma unroll
+        for (int n1 = 0; n1 < 3; ++n1) {
+#pragma unroll
+            for (int n2 = 0; n2 < 4; ++n2) b[n1][n2] = a[(4 * n1 + 3 * n2) % 12];
+            DFT<4, DIR>::run(b[n1]);
+        }
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) {
+            float2 c[3] = {b[0][k2], b[1][k2], b[2][k2]};
+            DFT<3, DIR>::run(c);
+#pragma unroll
+            for (int k1 = 0; k1 < 3; ++k1) a[(4 * k1 + 9 * k2) % 12] = c[k1];
+        }
+    }
+};
+
 // Radix-2 decimation in time over two half-length register DFTs (R = 8, 16, 32).
 template <int R, int DIR>
 struct DFT {
-    static_assert(R % 2 == 0 && R <= 32, "register DFT supports 2^k <= 32 and 3");
+    static_assert(R >= 8 && R <= 32 && (R & (R - 1)) == 0, "register DFT supports 2^k <= 32, 3 and 12");
     __device__ __forceinline__ static void run(float2* a)
     {
         float2 e[R / 2], o[R / 2];
@@ -278,7 +302,7 @@ struct Plan<192> {
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
-        Stockham<192, T, DIR, 1, 3, 4, 16>::run(t, s, tw, gl, gs);
+        Stockham<192, T, DIR, 1, 12, 16>::run(t, s, tw, gl, gs);
     }
 };
 template <>
