@@ -334,11 +334,11 @@ struct Plan<512> {  // three radix-8 stages keep all 64 threads busy (16 x 32 le
 };
 template <>
 struct Plan<1024> {
-    static constexpr int T = 64;
+    static constexpr int T = 32;
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
-        Stockham<1024, T, DIR, 1, 16, 16, 4>::run(t, s, tw, gl, gs);
+        Stockham<1024, T, DIR, 1, 32, 32>::run(t, s, tw, gl, gs);
     }
 };
 
