@@ -276,8 +276,10 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
                 chosen = p;  // kind 0
                 found = true;
             } else if (v & 0x1e) {
-                const int k = __ffs(v >> 1) - 1;
-                chosen = __builtin_amdgcn_readlane(sel4(myc, k), l);
+                const int k = __ffs(v >> 1) - 1;  // wave-uniform: four reads and a scalar select
+                const int c0 = __builtin_amdgcn_readlane(myc.x, l), c1 = __builtin_amdgcn_readlane(myc.y, l);
+                const int c2 = __builtin_amdgcn_readlane(myc.z, l), c3 = __builtin_amdgcn_readlane(myc.w, l);
+                chosen = k == 0 ? c0 : (k == 1 ? c1 : (k == 2 ? c2 : c3));
                 found = true;
             } else if (!(meta & (GS_META_TERM | GS_META_EXHAUSTED))) {
                 // every listed charge is gone: scan again on the current charges
@@ -337,11 +339,13 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
                 if (pix >= 0) ch.clear(pix);
             }
             ++emitted;
-            // the batch's flags follow the two clears
-            if (mypm.x == p || (pix >= 0 && mypm.x == pix)) valid &= ~1;
+            // the batch's flags follow the two clears (pixels are >= 0; -2 never matches, and
+            // cpix[k] = -1 marks a hit that no clear can touch)
+            const int q2 = pix >= 0 ? pix : -2;
+            int drop = (mypm.x == p) | (mypm.x == q2);
 #pragma unroll
-            for (int k = 0; k < GS_K; ++k)
-                if (cpix[k] >= 0 && (cpix[k] == p || cpix[k] == pix)) valid &= ~(2 << k);
+            for (int k = 0; k < GS_K; ++k) drop |= ((cpix[k] == p) | (cpix[k] == q2)) << (k + 1);
+            valid &= ~drop;
             live &= __ballot(valid & 1);
         }
     }
@@ -380,6 +384,17 @@ __global__ void k_gs_draw(const int2* __restrict__ pairs, const int* __restrict_
 
 // ---------------------------------------------------------------- 3. flood fill (BFS levels)
 __device__ __forceinline__ int ld_rel(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Queue slot for each calling lane: one atomic per wavefront for the lanes that call it together
+// (this module is built without the compiler's atomic optimizer, so the aggregation is explicit).
+__device__ __forceinline__ int reserve_slot(int* counter)
+{
+    const unsigned long long m = __ballot(1);
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    int base = 0;
+    if (rank == 0) base = atomicAdd(counter, __popcll(m));  // the first active lane
+    return __builtin_amdgcn_readfirstlane(base) + rank;
+}
 __device__ __forceinline__ void st_rel(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 __global__ __launch_bounds__(1024) void k_gs_bfs(const float* __restrict__ phi, const uint8_t* __restrict__ cuts,
@@ -431,7 +446,7 @@ __global__ __launch_bounds__(1024) void k_gs_bfs(const float* __restrict__ phi, 
                 int expected = -1;
                 if (__hip_atomic_compare_exchange_strong(lev + q, &expected, d + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                          __HIP_MEMORY_SCOPE_AGENT)) {
-                    const int pos = atomicAdd(&nnext, 1);
+                    const int pos = reserve_slot(&nnext);
                     st_rel(nxt + pos, q);
                 }
             }
@@ -536,7 +551,7 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ p
                 const unsigned m = 1u << (q & 31);
                 if (atomicOr(vis + (q >> 5), m) & m) continue;
                 atomicOr(lnext + (q >> 5), m);
-                st_rel(nxt + atomicAdd(&nnext, 1), q);
+                st_rel(nxt + reserve_slot(&nnext), q);
             }
         }
         __syncthreads();
