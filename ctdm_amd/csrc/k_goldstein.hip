@@ -478,30 +478,39 @@ __global__ __launch_bounds__(1024) void k_gs_bfs(const float* __restrict__ phi, 
     }
 }
 
-// Whole-map flood fill for maps whose bitmasks fit in LDS (cut, visited, and level membership for two
-// alternating levels: 4 bits per pixel). Same levels and parents as k_gs_bfs; the frontier queues and
-// k stay in global memory. Per level: (A) expand the current frontier, marking new pixels visited and
-// members of level d+1; (B) each new pixel takes k from its first neighbour (right, left, below, above)
-// of level d, and the level d-1 marks are cleared from the mask that level d+1 now shares.
+// Whole-map flood fill for maps whose bitmasks fit in LDS (cut, visited, and membership of the newest
+// level: 3 bits per pixel). Same levels and parents as k_gs_bfs. The frontier queues live in LDS too
+// (Q entries each; entries beyond Q spill to the global queues, so any frontier size is handled); k
+// stays in global memory. Per level d: (A) expand the current frontier, marking new pixels visited
+// and members of level d+1, while the level-d marks leave the level mask; (B) each new pixel takes k
+// from its first neighbour (right, left, below, above) of level d: a visited neighbour outside level
+// d+1 is of level d, since neighbouring levels differ by at most one. The queue counters rotate over
+// three slots, so a level needs two barriers.
 __device__ __forceinline__ bool tbit(const unsigned* b, long long p) { return (b[p >> 5] >> (p & 31)) & 1u; }
 
 __global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ phi, const uint8_t* __restrict__ cuts,
-                                                     int H, int W, int* __restrict__ kk, int* __restrict__ qa,
-                                                     int* __restrict__ qb, int* __restrict__ qc)
+                                                     int H, int W, int Q, int* __restrict__ kk, int* __restrict__ qa,
+                                                     int* __restrict__ qb)
 {
     extern __shared__ unsigned sm[];
-    __shared__ int ncur, nnext;
+    __shared__ int cnt[3];
     const long long n = (long long)H * W;
     const int nb = (int)((n + 31) / 32);
     unsigned* cutb = sm;
     unsigned* vis = sm + nb;
-    unsigned* lv[2] = {sm + 2 * nb, sm + 3 * nb};
+    unsigned* lnext = sm + 2 * nb;
+    int* lq[2] = {reinterpret_cast<int*>(sm + 3 * nb), reinterpret_cast<int*>(sm + 3 * nb) + Q};
     const float* ph = phi + blockIdx.x * n;
     const uint8_t* cut = cuts + blockIdx.x * n;
     int* k = kk + blockIdx.x * n;
-    int* prev = qc + blockIdx.x * n;
-    int* cur = qa + blockIdx.x * n;
-    int* nxt = qb + blockIdx.x * n;
+    int* gq[2] = {qa + blockIdx.x * n, qb + blockIdx.x * n};
+    auto qget = [&](int which, int i) -> int { return i < Q ? lq[which][i] : ld_rel(gq[which] + i); };
+    auto qput = [&](int which, int i, int v) {
+        if (i < Q)
+            lq[which][i] = v;
+        else
+            st_rel(gq[which] + i, v);
+    };
     for (int w = threadIdx.x; w < nb; w += blockDim.x) {
         unsigned v = 0;
         for (int b = 0; b < 32; ++b) {
@@ -510,8 +519,7 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ p
         }
         cutb[w] = v;
         vis[w] = 0u;
-        lv[0][w] = 0u;
-        lv[1][w] = 0u;
+        lnext[w] = 0u;
     }
     for (long long p = threadIdx.x; p < n; p += blockDim.x) st_rel(k + p, INT_MIN);
     __syncthreads();
@@ -521,27 +529,29 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ p
             --sx;
             ++sy;
         }
-        ncur = 0;
+        cnt[0] = 0;
         if (sx >= 0 && sy < H) {
             const int s0 = sx + W * sy;
             vis[s0 >> 5] |= 1u << (s0 & 31);
-            lv[0][s0 >> 5] |= 1u << (s0 & 31);
+            lnext[s0 >> 5] |= 1u << (s0 & 31);
             st_rel(k + s0, 0);
-            st_rel(cur, s0);
-            ncur = 1;
+            qput(0, 0, s0);
+            cnt[0] = 1;
         }
-        nnext = 0;
+        cnt[1] = 0;
+        cnt[2] = 0;
     }
     __syncthreads();
     const int dx4[4] = {1, -1, 0, 0}, dy4[4] = {0, 0, 1, -1};
-    int nprev = 0;
     for (int d = 0;; ++d) {
-        const int nc = ncur;
+        const int c0 = d % 3, c1 = (d + 1) % 3, c2 = (d + 2) % 3;
+        const int nc = cnt[c0];
         if (nc == 0) break;
-        unsigned* lnext = lv[(d + 1) & 1];
-        const unsigned* lcur = lv[d & 1];
+        const int cur = d & 1, nxt = cur ^ 1;
+        if (threadIdx.x == 0) cnt[c2] = 0;  // the level after next appends there (read after two barriers)
         for (int e = threadIdx.x; e < nc; e += blockDim.x) {  // (A)
-            const int p = ld_rel(cur + e);
+            const int p = qget(cur, e);
+            atomicAnd(lnext + (p >> 5), ~(1u << (p & 31)));  // level d leaves the level mask
             const int x = p % W, y = p / W;
             for (int t = 0; t < 4; ++t) {
                 const int nx = x + dx4[t], ny = y + dy4[t];
@@ -551,38 +561,24 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ p
                 const unsigned m = 1u << (q & 31);
                 if (atomicOr(vis + (q >> 5), m) & m) continue;
                 atomicOr(lnext + (q >> 5), m);
-                st_rel(nxt + reserve_slot(&nnext), q);
+                qput(nxt, reserve_slot(&cnt[c1]), q);
             }
         }
         __syncthreads();
-        const int nn = nnext;
+        const int nn = cnt[c1];
         for (int e = threadIdx.x; e < nn; e += blockDim.x) {  // (B)
-            const int q = ld_rel(nxt + e);
+            const int q = qget(nxt, e);
             const int x = q % W, y = q / W;
             for (int t = 0; t < 4; ++t) {
                 const int nx = x + dx4[t], ny = y + dy4[t];
                 if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
                 const int pp = nx + W * ny;
-                if (tbit(lcur, pp)) {
+                if (tbit(vis, pp) && !tbit(lnext, pp)) {
                     st_rel(k + q, ld_rel(k + pp) - gs_step2pi(__fsub_rn(ph[q], ph[pp])));
                     break;
                 }
             }
         }
-        for (int e = threadIdx.x; e < nprev; e += blockDim.x) {  // level d-1 leaves the shared mask
-            const int q = ld_rel(prev + e);
-            atomicAnd(lnext + (q >> 5), ~(1u << (q & 31)));
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            ncur = nn;
-            nnext = 0;
-        }
-        nprev = nc;
-        int* t = prev;
-        prev = cur;
-        cur = nxt;
-        nxt = t;
         __syncthreads();
     }
 }
@@ -662,10 +658,11 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
     hipLaunchKernelGGL(k_gs_draw, dim3((unsigned)((n + 255) / 256), (unsigned)batch), dim3(256), 0, c.stream,
                        pairs, npairs, H, W, ct);
     CTDM_CHECK_LAUNCH("k_gs_draw");
-    const size_t bfs_lds = sizeof(unsigned) * (size_t)(4 * nb);
-    if (bfs_lds + 64 <= lds_max) {
-        int* qc = (int*)c.buf("gs.qc", sizeof(int) * tot);
-        hipLaunchKernelGGL(k_gs_bfs_lds, dim3(batch), dim3(1024), bfs_lds, c.stream, wrapped, ct, H, W, kk, qa, qb, qc);
+    const size_t masks = sizeof(unsigned) * (size_t)(3 * nb);
+    if (masks + 64 + 2 * sizeof(int) * 256 <= lds_max) {  // the queues get what LDS has left (>= 256 each)
+        const int Q = (int)std::min<long long>(n, (long long)((lds_max - 64 - masks) / (2 * sizeof(int))));
+        const size_t bfs_lds = masks + 2 * sizeof(int) * (size_t)Q;
+        hipLaunchKernelGGL(k_gs_bfs_lds, dim3(batch), dim3(1024), bfs_lds, c.stream, wrapped, ct, H, W, Q, kk, qa, qb);
     } else {
         int* lev = (int*)c.buf("gs.level", sizeof(int) * tot);
         hipLaunchKernelGGL(k_gs_bfs, dim3(batch), dim3(1024), 0, c.stream, wrapped, ct, H, W, lev, kk, qa, qb);
