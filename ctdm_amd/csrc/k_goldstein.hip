@@ -384,6 +384,10 @@ __global__ void k_gs_draw(const int2* __restrict__ pairs, const int* __restrict_
 
 // ---------------------------------------------------------------- 3. flood fill (BFS levels)
 __device__ __forceinline__ int ld_rel(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+// The same at workgroup scope, for data one workgroup owns until its kernel ends (its waves share a
+// CU, so the loads may be served by that CU's cache).
+__device__ __forceinline__ int ld_wg(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+__device__ __forceinline__ void st_wg(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 
 // Queue slot for each calling lane: one atomic per wavefront for the lanes that call it together
 // (this module is built without the compiler's atomic optimizer, so the aggregation is explicit).
@@ -504,12 +508,12 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ p
     const uint8_t* cut = cuts + blockIdx.x * n;
     int* k = kk + blockIdx.x * n;
     int* gq[2] = {qa + blockIdx.x * n, qb + blockIdx.x * n};
-    auto qget = [&](int which, int i) -> int { return i < Q ? lq[which][i] : ld_rel(gq[which] + i); };
+    auto qget = [&](int which, int i) -> int { return i < Q ? lq[which][i] : ld_wg(gq[which] + i); };
     auto qput = [&](int which, int i, int v) {
         if (i < Q)
             lq[which][i] = v;
         else
-            st_rel(gq[which] + i, v);
+            st_wg(gq[which] + i, v);
     };
     for (int w = threadIdx.x; w < nb; w += blockDim.x) {
         unsigned v = 0;
@@ -521,7 +525,7 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ p
         vis[w] = 0u;
         lnext[w] = 0u;
     }
-    for (long long p = threadIdx.x; p < n; p += blockDim.x) st_rel(k + p, INT_MIN);
+    for (long long p = threadIdx.x; p < n; p += blockDim.x) st_wg(k + p, INT_MIN);
     __syncthreads();
     if (threadIdx.x == 0) {
         int sx = W / 2, sy = min(20, H - 1);
@@ -534,7 +538,7 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ p
             const int s0 = sx + W * sy;
             vis[s0 >> 5] |= 1u << (s0 & 31);
             lnext[s0 >> 5] |= 1u << (s0 & 31);
-            st_rel(k + s0, 0);
+            st_wg(k + s0, 0);
             qput(0, 0, s0);
             cnt[0] = 1;
         }
@@ -574,7 +578,7 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ p
                 if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
                 const int pp = nx + W * ny;
                 if (tbit(vis, pp) && !tbit(lnext, pp)) {
-                    st_rel(k + q, ld_rel(k + pp) - gs_step2pi(__fsub_rn(ph[q], ph[pp])));
+                    st_wg(k + q, ld_wg(k + pp) - gs_step2pi(__fsub_rn(ph[q], ph[pp])));
                     break;
                 }
             }
