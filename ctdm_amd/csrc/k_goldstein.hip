@@ -414,8 +414,8 @@ __global__ __launch_bounds__(1024) void k_gs_bfs(const float* __restrict__ phi, 
     int* cur = qa + blockIdx.x * n;
     int* nxt = qb + blockIdx.x * n;
     for (long long p = threadIdx.x; p < n; p += blockDim.x) {
-        st_rel(lev + p, -1);
-        st_rel(k + p, INT_MIN);
+        st_wg(lev + p, -1);
+        st_wg(k + p, INT_MIN);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -427,9 +427,9 @@ __global__ __launch_bounds__(1024) void k_gs_bfs(const float* __restrict__ phi, 
         ncur = 0;
         if (sx >= 0 && sy < H) {
             const int s = sx + W * sy;
-            st_rel(lev + s, 0);
-            st_rel(k + s, 0);
-            st_rel(cur, s);
+            st_wg(lev + s, 0);
+            st_wg(k + s, 0);
+            st_wg(cur, s);
             ncur = 1;
         }
         nnext = 0;
@@ -440,7 +440,7 @@ __global__ __launch_bounds__(1024) void k_gs_bfs(const float* __restrict__ phi, 
         const int nc = ncur;
         if (nc == 0) break;
         for (int e = threadIdx.x; e < nc; e += blockDim.x) {
-            const int p = ld_rel(cur + e);
+            const int p = ld_wg(cur + e);
             const int x = p % W, y = p / W;
             for (int t = 0; t < 4; ++t) {
                 const int nx = x + dx4[t], ny = y + dy4[t];
@@ -449,23 +449,23 @@ __global__ __launch_bounds__(1024) void k_gs_bfs(const float* __restrict__ phi, 
                 if (cut[q]) continue;
                 int expected = -1;
                 if (__hip_atomic_compare_exchange_strong(lev + q, &expected, d + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT)) {
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP)) {
                     const int pos = reserve_slot(&nnext);
-                    st_rel(nxt + pos, q);
+                    st_wg(nxt + pos, q);
                 }
             }
         }
         __syncthreads();
         const int nn = nnext;
         for (int e = threadIdx.x; e < nn; e += blockDim.x) {
-            const int q = ld_rel(nxt + e);
+            const int q = ld_wg(nxt + e);
             const int x = q % W, y = q / W;
             for (int t = 0; t < 4; ++t) {
                 const int nx = x + dx4[t], ny = y + dy4[t];
                 if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
                 const int pp = nx + W * ny;
-                if (ld_rel(lev + pp) == d) {
-                    st_rel(k + q, ld_rel(k + pp) - gs_step2pi(__fsub_rn(ph[q], ph[pp])));
+                if (ld_wg(lev + pp) == d) {
+                    st_wg(k + q, ld_wg(k + pp) - gs_step2pi(__fsub_rn(ph[q], ph[pp])));
                     break;
                 }
             }
