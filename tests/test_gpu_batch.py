@@ -117,3 +117,31 @@ def test_located_crop_outside_the_spectrum_is_rejected(ctx):
     ctx.reconstruct([good], mode=1, graph=True)
     torch.cuda.synchronize()
     assert torch.isfinite(good["ri"]).all()
+
+
+def test_cells_with_different_carriers_share_one_graph(ctx):
+    """Cells whose carriers differ get their own located crops (one device slot per cell) inside one
+    two-lane graph; every cell equals its own eager single-cell run, replays included."""
+    specs = [((0.25, 0.25), 64), ((0.22, 0.27), 32), ((0.27, 0.22), 64)]
+    cells = []
+    for i, (carrier, V) in enumerate(specs):
+        holo, bg, ang = ctdm_amd.synth_cell(60 + i, 4, ctdm_amd.synth_params(H=256, carrier=carrier))
+        cells.append(dict(holo=torch.from_numpy(holo).cuda(), bg=torch.from_numpy(bg).cuda(), angles=ang, V=V,
+                          iters=2, optics=ctdm_amd.optics_for(V, 256),
+                          ri=torch.empty((V, V, V), dtype=torch.complex64, device="cuda")))
+    ctx.reconstruct(cells, mode=1, graph=True)
+    torch.cuda.synchronize()
+    crops = [c["crop_used"].astuple() for c in cells]
+    assert len(set(crops)) == 3, crops
+    first = [c["ri"].clone() for c in cells]
+    for c in cells:
+        c["ri"].zero_()
+    ctx.reconstruct(cells, mode=1, graph=True)  # replay
+    torch.cuda.synchronize()
+    for c, f, cr in zip(cells, first, crops):
+        assert torch.equal(c["ri"], f)
+        one = dict(c, ri=torch.empty_like(f))
+        ctx.reconstruct([one], mode=1)
+        torch.cuda.synchronize()
+        assert one["crop_used"].astuple() == cr
+        assert torch.equal(one["ri"], f)
