@@ -89,13 +89,26 @@ __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
     const float* src = (img < n0 ? src0 + img * (long long)H * W : src1 + (img - n0) * (long long)H * W) + (long long)y * W;
     const float2* src2 = reinterpret_cast<const float2*>(src);
     float2* s = lds + g * LdsSize<NH>::value;
+    // this thread's first QP split twiddles exp(-2 pi i k / W), fetched before the transform to hide
+    // their latency (a band of up to QP * T outputs: the whole passband of the batched retrieval);
+    // outputs beyond that (the search's full-width rows) load theirs in the loop
+    constexpr int QN = W / T;  // outputs per thread at most (nout <= W)
+    constexpr int QP = QN < 4 ? QN : 4;
+    float2 wq[QP];
+#pragma unroll
+    for (int j = 0; j < QP; ++j) {
+        const int q = t + j * T;
+        int k = (k_lo + q) % W;
+        if (k < 0) k += W;
+        wq[j] = q < nout ? tw_full[k] : make_float2(0.f, 0.f);
+    }
     auto gl = [&](int i) -> float2 { return valid ? src2[i] : make_float2(0.f, 0.f); };
     auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };
     Plan<NH>::template run<-1>(t, s, twp, gl, gs);
     __syncthreads();
     if (!valid) return;
     float2* dst = out + row * nout;
-    for (int q = t; q < nout; q += T) {
+    auto emit = [&](int q, float2 w) {
         int k = (k_lo + q) % W;
         if (k < 0) k += W;
         const int ka = k % NH;
@@ -103,8 +116,15 @@ __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
         const float2 za = s[lds_pad(ka)], zb = s[lds_pad(kb)];
         const float2 e = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));  // (Z[k] + conj Z[-k]) / 2
         const float2 o = make_float2(0.5f * (za.y + zb.y), -0.5f * (za.x - zb.x)); // (Z[k] - conj Z[-k]) / (2i)
-        const float2 w = tw_full[k];                                               // exp(-2 pi i k / W)
         dst[q] = cadd(e, cmul(w, o));
+    };
+#pragma unroll
+    for (int j = 0; j < QP; ++j)
+        if (t + j * T < nout) emit(t + j * T, wq[j]);
+    for (int q = t + QP * T; q < nout; q += T) {
+        int k = (k_lo + q) % W;
+        if (k < 0) k += W;
+        emit(q, tw_full[k]);
     }
 }
 
