@@ -30,7 +30,7 @@ A, H, V, ITERS = 60, 1024, 256, 10
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (separate runs), fetch scaled by 2.000 and write by
 # 1.000 so that the copy yardstick reads as its known 128 MiB in + 128 MiB out. The profiler cannot run
 # inside this process, so the figures are carried here; profiles/traffic_config2_pmc.md has the runs.
-TRAFFIC_KIB = {0: 131209 + 131072, 1: 131156 + 131072, 2: 170178 + 131072}
+TRAFFIC_KIB = {0: 131209 + 131072, 1: 131157 + 131072, 2: 170176 + 131072}
 PASS_NAMES = {0: "k_vol_x (inverse x-FFT + projection + forward x-FFT)",
               1: "k_vol_y (y-FFT)",
               2: "k_vol_zdc (z-FFT + data consistency + inverse z-FFT)"}
