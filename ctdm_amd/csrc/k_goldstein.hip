@@ -264,12 +264,58 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
         // lanes whose residue is still present, in raster order; residues balanced as partners of an
         // earlier commit drop out of the mask without an iteration
         unsigned long long live = __ballot(lane < cnt_batch && (valid & 1));
+        // Choice of lane ll from its listed hits (or itself on the border), valid while its flags hold.
+        auto listed_choice = [&](int ll, int vv, int pp, int mm) -> int {
+            if (mm & GS_META_SELF) return pp;  // kind 0
+            const int k = __ffs(vv >> 1) - 1;  // wave-uniform: four reads and a scalar select
+            const int c0 = __builtin_amdgcn_readlane(myc.x, ll), c1 = __builtin_amdgcn_readlane(myc.y, ll);
+            const int c2 = __builtin_amdgcn_readlane(myc.z, ll), c3 = __builtin_amdgcn_readlane(myc.w, ll);
+            return k == 0 ? c0 : (k == 1 ? c1 : (k == 2 ? c2 : c3));
+        };
         while (live) {
             const int l = __ffsll((long long)live) - 1;
             live &= l == 63 ? 0ull : (~0ull << (l + 1));
             const int v = __builtin_amdgcn_readlane(valid, l);
             const int p = __builtin_amdgcn_readlane(mypm.x, l);
             const int meta = __builtin_amdgcn_readlane(mypm.y, l);
+            // Charge words in global memory (large maps): two residues at once when the second's
+            // decision cannot depend on the first's commit (both choose from their lists, the second's
+            // residue is not the first's partner, and the second's partner is neither pixel the first
+            // clears); pairs, clears and flags come out as two single steps would make them. This
+            // halves the clears' memory round trips; with the words in LDS it measured slower.
+            if constexpr (!LDSW) {
+                if (live && ((meta & GS_META_SELF) || (v & 0x1e))) {
+                    const int l2 = __ffsll((long long)live) - 1;
+                    const int v2 = __builtin_amdgcn_readlane(valid, l2);
+                    const int meta2 = __builtin_amdgcn_readlane(mypm.y, l2);
+                    if ((meta2 & GS_META_SELF) || (v2 & 0x1e)) {
+                        const int p2 = __builtin_amdgcn_readlane(mypm.x, l2);
+                        const int ch1 = listed_choice(l, v, p, meta);
+                        const int ch2 = listed_choice(l2, v2, p2, meta2);
+                        const int x1 = ((unsigned)ch1 >> 30) == 3u ? (ch1 & 0x3fffffff) : -2;
+                        const int x2 = ((unsigned)ch2 >> 30) == 3u ? (ch2 & 0x3fffffff) : -2;
+                        if (p2 != x1 && x2 != p && (x2 < 0 || x2 != x1)) {
+                            live &= l2 == 63 ? 0ull : (~0ull << (l2 + 1));
+                            if (lane == 0) {
+                                out[emitted] = make_int2(p, ch1);
+                                out[emitted + 1] = make_int2(p2, ch2);
+                                ch.clear(p);
+                                if (x1 >= 0) ch.clear(x1);
+                                ch.clear(p2);
+                                if (x2 >= 0) ch.clear(x2);
+                            }
+                            emitted += 2;
+                            int drop = (mypm.x == p) | (mypm.x == x1) | (mypm.x == p2) | (mypm.x == x2);
+#pragma unroll
+                            for (int k = 0; k < GS_K; ++k)
+                                drop |= ((cpix[k] == p) | (cpix[k] == x1) | (cpix[k] == p2) | (cpix[k] == x2)) << (k + 1);
+                            valid &= ~drop;
+                            live &= __ballot(valid & 1);
+                            continue;
+                        }
+                    }
+                }
+            }
             int chosen = 0;  // encoded choice; kinds 2 and 3 set the sign bit, so `found` tracks it
             bool found = false;
             if (meta & GS_META_SELF) {
