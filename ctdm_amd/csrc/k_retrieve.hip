@@ -129,6 +129,47 @@ __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
 }
 
 // ---------------------------------------------------------------- 2. column FFT over the band
+// 1024-point forward transform of one column by the 32 threads t of that column, as 32 x 32:
+// x[t + 32 r] (r < 32) loaded by thread t, a register DFT over r, an exchange through LDS, twiddles
+// W_1024^(m t), a second register DFT: thread t ends with X[t + 32 k] in v[k]. The exchange moves
+// half of the stage-one outputs at a time (threads t < 16, then t >= 16; each half is two whole
+// wavefronts), so the column needs 512 float2 of LDS instead of 1024: with eight columns a
+// workgroup takes 35 KiB and four fit on a CU, where the one-pass exchange allowed two.
+template <class GL>
+__device__ __forceinline__ void col_fft1024_half_lds(int t, float2* s, const float2* __restrict__ tw,
+                                                     GL& gl, float2 (&v)[32])
+{
+#pragma unroll
+    for (int r = 0; r < 32; ++r) v[r] = gl(r);
+    DFT<32, -1>::run(v);  // v[k] = Y[t][k] = sum_r x[t + 32 r] W_32^(r k)
+    float2 u[32];         // u[m] = Y[m][t]
+    const int h = t >> 4;
+    // Slot 32 m + k of the half holds Y[16 h + m][k]; with t < 32, lds_pad(32 m + k) is
+    // 34 m + lds_pad(k), so both sides address one base with constant offsets.
+    float2* wr = s + 34 * (t & 15);
+    const float2* rd = s + lds_pad(t);
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        if (h == half) {
+#pragma unroll
+            for (int k = 0; k < 32; ++k) wr[lds_pad(k)] = v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 16; ++m) u[16 * half + m] = rd[34 * m];
+        __syncthreads();
+    }
+    // Twiddles W_1024^(m t) (m t < 1024), read here: the opaque index keeps the reads from being
+    // hoisted over the exchange, where their registers would add to its peak.
+    int tt = t;
+    asm volatile("" : "+v"(tt));
+#pragma unroll
+    for (int m = 1; m < 32; ++m) u[m] = cmul(u[m], tw[m * tt]);
+    DFT<32, -1>::run(u);  // u[k] = X[t + 32 k]
+#pragma unroll
+    for (int k = 0; k < 32; ++k) v[k] = u[k];
+}
+
 // in: R[img][y][q] (H rows of nin columns), column q <-> p' = p_lo + q.
 // For each column: F(ky) = FFT_H along y; keep rows ky = (r0 + q') mod H for q' in the band,
 // place at index q' mod M of an M-vector (zero elsewhere), inverse M-point FFT ->
@@ -140,8 +181,10 @@ __global__ __launch_bounds__(C* Plan<H>::T) void k_col_band(
     float2* __restrict__ out)
 {
     constexpr int T = Plan<H>::T;
-    constexpr int LS = LdsStride<H, C>::value;
-    static_assert(LdsSize<M>::value <= LS, "band transform must fit the column area");
+    constexpr bool HALF = (H == 1024);  // two-pass exchange: the column area holds half a column
+    constexpr int LS = LdsStride<HALF ? H / 2 : H, C>::value;
+    static_assert(LdsSize<M>::value + (HALF ? T : 0) <= LS,
+                  "band transform (and the spare slots) must fit the column area");
     __shared__ float2 lds[C * LS];
     if (cd) {  // crop located on the device; the grid covers the widest band (M columns)
         nin = cd->n;
@@ -155,16 +198,44 @@ __global__ __launch_bounds__(C* Plan<H>::T) void k_col_band(
     const int img = blk / tiles;
     const int q = (blk % tiles) * C + cidx;
     const bool valid = q < nin && img < nimg;
-    const float2* col = in + (long long)img * H * nin + (valid ? q : 0);
     float2* s = lds + cidx * LS;
-    auto gl = [&](int i) -> float2 { return valid ? col[(long long)i * nin] : make_float2(0.f, 0.f); };
-    auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };
-    Plan<H>::template run<-1>(t, s, tw_h, gl, gs);
+    if constexpr (HALF) {
+        // Element offsets within an image fit 32 bits (H * nin < 2^31); the image base is uniform.
+        const float2* base = in + (long long)(img < nimg ? img : 0) * H * nin;
+        const unsigned lane = (unsigned)t * nin + (valid ? q : 0);
+        auto gl = [&](int r) -> float2 {
+            return valid ? base[lane + (unsigned)(32 * r * nin)] : make_float2(0.f, 0.f);
+        };
+        float2 v[32];
+        col_fft1024_half_lds(t, s, tw_h, gl, v);
+        // The band rows go to their slots of the M-vector; the last exchange read preceded a barrier.
+        // Row ky = t + 32 k is in the band when e = (ky - r0 - q_lo) mod H < q_n; its q' is q_lo + e
+        // (the band is shorter than H). Rows outside it go to the thread's spare slot past the
+        // M-vector, so that every row is one unconditional store. The opaque copy of t keeps the
+        // slots from being formed (and held in registers) before the transform.
+        int tt = t;
+        asm volatile("" : "+v"(tt));
+        const int e0 = tt - r0 - q_lo;
+        const int spare = LdsSize<M>::value + tt;
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const int e = (e0 + 32 * k) & (H - 1);
+            const int qq = q_lo + e;
+            s[e < q_n ? lds_pad(qq < 0 ? qq + M : qq) : spare] = v[k];
+        }
+    } else {
+        const float2* col = in + (long long)img * H * nin + (valid ? q : 0);
+        auto gl = [&](int i) -> float2 { return valid ? col[(long long)i * nin] : make_float2(0.f, 0.f); };
+        auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };
+        Plan<H>::template run<-1>(t, s, tw_h, gl, gs);
+    }
     __syncthreads();
-    // Inverse M-point transform along y of the band (ifftshift folded into the index).
+    // Inverse M-point transform along y of the band (ifftshift folded into the index): its input i
+    // is the band slot (1024 rows) or the spectrum row ky = (r0 + q') mod H (full column in LDS).
     auto gl2 = [&](int i) -> float2 {
         const int qq = i < M / 2 ? i : i - M;  // q'
         if (qq < q_lo || qq >= q_lo + q_n) return make_float2(0.f, 0.f);
+        if constexpr (HALF) return s[lds_pad(i)];
         int ky = (r0 + qq) % H;
         if (ky < 0) ky += H;
         return s[lds_pad(ky)];
