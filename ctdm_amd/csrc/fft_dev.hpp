@@ -217,10 +217,15 @@ struct Stockham {
         for (int p = 0; p < P; ++p) {
             const int j = t + p * T;
             if (t < T && (S % T == 0 || j < S)) {
+                // Element j + r S (0 <= j < S) of the padded area: one base and constant offsets
+                // lds_pad(r S) when 16 divides S or S divides 16 (else the index per element).
+                const float2* sj = s + (S % 16 == 0 ? lds_pad(j) : j);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     if constexpr (FIRST)
                         v[p][r] = gload(j + r * S);
+                    else if constexpr (S % 16 == 0 || 16 % S == 0)
+                        v[p][r] = sj[lds_pad(r * S)];
                     else
                         v[p][r] = s[lds_pad(j + r * S)];
                 }
@@ -239,10 +244,20 @@ struct Stockham {
                 }
                 DFT<R, DIR>::run(v[p]);
                 const int base = (j - k) * R + k;
+                // Outputs base + r L, where (j - k) R is a multiple of L R and k < L: with 16 | L the
+                // padded index is lds_pad(base) + lds_pad(r L); with L | 16 and 16 | L R it is
+                // lds_pad((j - k) R) + k + lds_pad(r L); with L R | 16 the block lies in one 16-slot
+                // segment: lds_pad((j - k) R) + k + r L.
+                constexpr int WR = L % 16 == 0 ? 1 : (16 % L == 0 && (L * R) % 16 == 0) ? 2 : 16 % (L * R) == 0 ? 3 : 0;
+                float2* sb = s + (WR == 1 ? lds_pad(base) : lds_pad((j - k) * R) + k);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     if constexpr (LAST)
                         gstore(base + r * L, v[p][r]);
+                    else if constexpr (WR == 1 || WR == 2)
+                        sb[lds_pad(r * L)] = v[p][r];
+                    else if constexpr (WR == 3)
+                        sb[r * L] = v[p][r];
                     else
                         s[lds_pad(base + r * L)] = v[p][r];
                 }

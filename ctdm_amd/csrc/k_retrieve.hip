@@ -18,6 +18,7 @@
 #include "fft_dev.hpp"
 #include "internal.hpp"
 
+#include <climits>
 #include <cmath>
 #include <type_traits>
 
@@ -78,20 +79,25 @@ __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
     constexpr int T = Plan<NH>::T;
     __shared__ float2 lds[G * LdsSize<NH>::value];
     __shared__ float2 twl[LdsSize<NH>::value];
+    // The G rows of a workgroup lie in one image (H is a multiple of G): image and first row are
+    // workgroup-uniform, so the division runs once on the scalar unit, and a workgroup past the last
+    // image leaves as a whole before any barrier.
+    const int row0 = blockIdx.x * G;  // over nimg * H rows
+    const int img_u = row0 / H;
+    if (img_u >= nimg) return;
     stage_twiddles<NH>(twl, tw_half);
     const TwPad twp{twl};
     const int g = threadIdx.x / T, t = threadIdx.x % T;
-    const long long row = (long long)blockIdx.x * G + g;  // over nimg*H rows
-    const long long nrows = (long long)nimg * H;
-    const bool valid = row < nrows;
-    const long long img = valid ? row / H : 0;
-    const int y = valid ? (int)(row % H) : 0;
-    const float* src = (img < n0 ? src0 + img * (long long)H * W : src1 + (img - n0) * (long long)H * W) + (long long)y * W;
+    const long long img = img_u;
+    const int y = row0 - img_u * H + g;
+    const float* src = (img < n0 ? src0 + img * (long long)H * W : src1 + (img - n0) * (long long)H * W) +
+                       (long long)y * W;
     const float2* src2 = reinterpret_cast<const float2*>(src);
     float2* s = lds + g * LdsSize<NH>::value;
     // this thread's first QP split twiddles exp(-2 pi i k / W), fetched before the transform to hide
     // their latency (a band of up to QP * T outputs: the whole passband of the batched retrieval);
-    // outputs beyond that (the search's full-width rows) load theirs in the loop
+    // outputs beyond that (the search's full-width rows) load theirs in the loop. The index is in
+    // [0, W) for every q, so the reads need no guard (entries for q >= nout go unused).
     constexpr int QN = W / T;  // outputs per thread at most (nout <= W)
     constexpr int QP = QN < 4 ? QN : 4;
     float2 wq[QP];
@@ -100,14 +106,13 @@ __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
         const int q = t + j * T;
         int k = (k_lo + q) % W;
         if (k < 0) k += W;
-        wq[j] = q < nout ? tw_full[k] : make_float2(0.f, 0.f);
+        wq[j] = tw_full[k];
     }
-    auto gl = [&](int i) -> float2 { return valid ? src2[i] : make_float2(0.f, 0.f); };
+    auto gl = [&](int i) -> float2 { return src2[i]; };
     auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };
     Plan<NH>::template run<-1>(t, s, twp, gl, gs);
     __syncthreads();
-    if (!valid) return;
-    float2* dst = out + row * nout;
+    float2* dst = out + ((long long)img * H + y) * nout;
     auto emit = [&](int q, float2 w) {
         int k = (k_lo + q) % W;
         if (k < 0) k += W;
@@ -390,6 +395,8 @@ static void launch_row_real(Ctx& c, const float* src0, int n0, const float* src1
 {
     constexpr int G = 256 / Plan<W / 2>::T;
     const long long rows = (long long)nimg * H;
+    CTDM_REQUIRE(H % G == 0 && rows <= INT_MAX,
+                 "row transform: image height must be a multiple of the row group");
     hipLaunchKernelGGL((k_row_real<W, G>), dim3(grid1d(rows, G)), dim3(G * Plan<W / 2>::T), 0,
                        c.stream, src0, n0, src1, nimg, H, c.tw(W / 2), c.tw(W), k_lo, nout, cd, out);
     CTDM_CHECK_LAUNCH("k_row_real");
