@@ -275,10 +275,12 @@ typedef struct ctdm_batch_opts {
 ctdm_status ctdm_reconstruct(ctdm_ctx *ctx, ctdm_cell *cells, int ncells,
                              const ctdm_batch_opts *opts);
 
-/* Times every kernel variant for volume/crop side V (FFT radix plan, pencils per workgroup,
- * gridding block) on this device and caches the fastest in the context (no reference
- * counterpart). Every listed side is re-timed, also one tuned before. Synchronises. `report` (HOST,
- * may be NULL) receives a text summary. */
+/* Times the three fused volume passes (x with projection, y, z with data consistency) of each
+ * listed volume side V with 8 and 16 pencils per workgroup on this device, inside the positivity
+ * loop's own pass sequence, and caches the choice in the context: the faster one, except that y and
+ * z take 8 pencils only when at least 3 % faster (their 8-pencil tiles write half lines). Radix plans
+ * and the gridding block are fixed. No reference counterpart. Every listed side is re-timed, also
+ * one tuned before. Synchronises. `report` (HOST, may be NULL) receives a text summary. */
 ctdm_status ctdm_autotune(ctdm_ctx *ctx, const int *sizes, int nsizes, char *report,
                           size_t report_len);
 
