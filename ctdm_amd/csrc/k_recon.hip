@@ -788,15 +788,17 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
         }
     }
     auto hits = [&](int j) -> int { return (int)((hc[j >> 1] >> (16 * (j & 1))) & 0xffffu); };
-    // normalise the owned z indices (each touched only by its owner so far) and count list entries
+    // normalise the owned z indices (each touched only by its owner so far) and count list entries:
+    // the mean over the h hits times 1/dx, as one scale inv_dx / h with the hardware reciprocal
+    // (v_rcp_f32, about 1 ulp) in place of two correctly rounded divisions per voxel
     int cnt = 0;
 #pragma unroll
     for (int j = 0; j < ZR; ++j) {
         const int z = tl.t + j * T;
         const int h = hits(j);
         float2 f = s[lds_pad(z)];
-        if (h) f = make_float2(f.x / (float)h, f.y / (float)h);
-        f = make_float2(f.x * inv_dx, f.y * inv_dx);
+        const float sc = h ? inv_dx * __builtin_amdgcn_rcpf((float)h) : inv_dx;
+        f = make_float2(f.x * sc, f.y * sc);
         s[lds_pad(z)] = f;
         if (h && !(refcompat && !(f.x != 0.f && f.y != 0.f))) ++cnt;
     }
