@@ -107,10 +107,11 @@ struct Dct3 {
     __device__ __forceinline__ static void exact(int t, const float* X, float2* s, const float2* tw_n,
                                                  const float2* tw_4n, Put& put)
     {
-        const float s0 = sqrtf(1.f / N), s1 = sqrtf(2.f / N);
+        // 1 / sqrt(1/N) and 1 / sqrt(2/N) (compile-time constants): the un-normalisation multiplies
+        const float r0 = 1.f / sqrtf(1.f / N), r1 = 1.f / sqrtf(2.f / N);
         auto gl = [&](int k) -> float2 {
-            const float ck = X[k] / (k == 0 ? s0 : s1);
-            const float cn = (k == 0) ? 0.f : X[N - k] / s1;
+            const float ck = X[k] * (k == 0 ? r0 : r1);
+            const float cn = (k == 0) ? 0.f : X[N - k] * r1;
             const float2 w = tw_4n[k];  // exp(-i pi k/2N); we need exp(+i pi k/2N) = conj
             // (ck - i cn) * (w.x - i w.y)
             return make_float2(ck * w.x - cn * w.y, -ck * w.y - cn * w.x);
@@ -193,7 +194,9 @@ __global__ __launch_bounds__(C* Plan<LN>::T) void k_uw_cols(const float* __restr
                 lam = eig[kx] + eig[ky] - 4.0;
             else
                 lam = -4.0 * (eig[kx] + eig[ky]);
-            xv[j] = (kx == 0 && ky == 0) ? 0.f : (float)((double)Xv / lam);
+            // Xv / lambda as a product with the hardware reciprocal (v_rcp_f32, about 1 ulp) of the
+            // eigenvalue sum formed in double: two ulps at most, in place of a double division
+            xv[j] = (kx == 0 && ky == 0) ? 0.f : Xv * __builtin_amdgcn_rcpf((float)lam);
         }
     }
     __syncthreads();  // every read of the transform's output precedes the coefficients' writes
