@@ -37,17 +37,52 @@ void* Ctx::buf(const std::string& name, size_t bytes, bool shared)
     CTDM_REQUIRE(!capturing, "scratch buffer '" + name + "' grown during graph capture");
     if (b.ptr) {
         CTDM_HIP(hipStreamSynchronize(stream));
-        if (b.pinned)
+        if (referenced(b.ptr))
             retired.push_back(b.ptr);
         else
-            CTDM_HIP(hipFree(b.ptr));
+            release(b.ptr);
         b.ptr = nullptr;
         b.bytes = 0;
-        b.pinned = false;
     }
     CTDM_HIP(hipMalloc(&b.ptr, bytes ? bytes : 1));
     b.bytes = bytes;
     return b.ptr;
+}
+
+bool Ctx::referenced(const void* p) const
+{
+    for (const auto& kv : graphs)
+        if (std::binary_search(kv.second.ptrs.begin(), kv.second.ptrs.end(), (void*)p)) return true;
+    return false;
+}
+
+void Ctx::trim_graphs(size_t keep)
+{
+    if (graphs.size() <= keep) return;
+    CTDM_REQUIRE(!capturing, "graph cache trimmed during capture");
+    CTDM_HIP(hipStreamSynchronize(stream));  // no evicted graph may still be running
+    while (graphs.size() > keep) {
+        auto victim = graphs.begin();
+        for (auto it = graphs.begin(); it != graphs.end(); ++it)
+            if (it->second.last_use < victim->second.last_use) victim = it;
+        if (victim->second.exec) CTDM_HIP(hipGraphExecDestroy(victim->second.exec));
+        if (victim->second.graph) CTDM_HIP(hipGraphDestroy(victim->second.graph));
+        graphs.erase(victim);
+    }
+    std::vector<void*> still;
+    for (void* p : retired) {
+        if (referenced(p))
+            still.push_back(p);
+        else
+            release(p);
+    }
+    retired.swap(still);
+}
+
+void Ctx::release(void* p)
+{
+    table_contents.erase(p);
+    CTDM_HIP(hipFree(p));
 }
 
 void Ctx::ensure_aux()

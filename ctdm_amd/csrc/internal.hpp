@@ -45,11 +45,12 @@ struct Error {
         if (!(cond)) ::ctdm::fail(CTDM_EINVAL, msg);                                            \
     } while (0)
 
-// Device buffer owned by the context; grows on demand (never inside graph capture).
+// Device buffer owned by the context; grows on demand (never inside graph capture). A buffer that a
+// cached graph references is retired instead of freed when it grows, and freed once the last graph
+// referencing it has been evicted.
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
-    bool pinned = false;  // referenced by a captured graph: retire instead of freeing on growth
 };
 
 struct Ctx;
@@ -69,7 +70,13 @@ struct GraphKey {
 struct GraphEntry {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<void*> ptrs;     // sorted scratch pointers the captured sequence references
+    std::vector<double*> rows;   // per cell: the angle-row table the graph reads (rewritten before each replay)
+    uint64_t last_use = 0;       // LRU stamp
 };
+
+// Captured graphs kept per context; the least recently used one is evicted beyond this.
+constexpr size_t kMaxGraphs = 8;
 
 struct Ctx {
     hipEvent_t tev[2] = {nullptr, nullptr};  // when set: recorded at start and end of the next volume pass
@@ -80,7 +87,17 @@ struct Ctx {
     std::map<std::string, DevBuf> scratch;
     std::map<int, Tune> tune;
     std::map<GraphKey, GraphEntry> graphs;
-    std::vector<void*> retired;  // pinned buffers replaced by larger ones (freed with the context)
+    uint64_t graph_clock = 0;
+    std::vector<void*> retired;  // grown buffers still referenced by a cached graph
+    // Host copy of what each per-cell angle-row table holds, so that a replay with unchanged angles issues
+    // no copy. Entries are dropped when their buffer is freed.
+    std::map<const void*, std::vector<double>> table_contents;
+    void release(void* p);  // frees p (after the caller synchronised) and forgets its table contents
+    // True when a cached graph references device pointer p.
+    bool referenced(const void* p) const;
+    // Evicts least recently used graphs until at most `keep` remain, then frees retired buffers that no
+    // remaining graph references (synchronises the context stream first).
+    void trim_graphs(size_t keep);
     bool capturing = false;
     int max_lds = 65536;
     int dc_V = 0;    // volume side the cached data-consistency list ("dc.*") was built for
@@ -191,8 +208,13 @@ void positivity_plain(Ctx& c, float2* f, const float2* Fm, const float* hits, fl
 void positivity_fused(Ctx& c, float2* f, const float2* Fm, const float* hits, float2* n, int V,
                       int iters, const ctdm_optics& o, unsigned flags);
 // Batched cell volume: spectra of all angles -> fused gridding + inverse FFT + positivity loop.
-void volume_batched(Ctx& c, const float2* spec, const ctdm_angle* ang, int A, int V,
+// `rows` is the cell's device table of A angle rows (fx0, fy0, fz0), written by write_rows.
+void volume_batched(Ctx& c, const float2* spec, const double* rows, int A, int V,
                     const ctdm_optics& o, int iters, unsigned flags, float2* n_out);
+// Copies host angle rows to the device table `dst` (3 doubles per angle) on the context stream; never
+// under capture (a host copy must not be recorded). With `cached`, the copy is skipped when `dst`
+// already holds these rows (only for tables written through this function alone).
+void write_rows(Ctx& c, double* dst, const std::vector<AngleRow>& rows, bool cached = false);
 void fit_pgd(Ctx& c, float2* f, const float2* spec, const ctdm_angle* ang, int A, int V,
              const ctdm_optics& o, int iters, double step, float2* n, unsigned flags);
 

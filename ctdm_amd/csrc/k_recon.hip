@@ -25,6 +25,7 @@
 #include <hip/hip_ext.h>
 
 #include <cmath>
+#include <cstring>
 
 namespace ctdm {
 
@@ -103,15 +104,26 @@ __device__ __forceinline__ void mark_owner(unsigned* masks, int z, int k)
     atomicOr(&masks[z & (T - 1)], 1u << k);
 }
 
-// Device copy of the per-angle rows (fx0, fy0, fz0).
+void write_rows(Ctx& c, double* dst, const std::vector<AngleRow>& rows, bool cached)
+{
+    CTDM_REQUIRE(!c.capturing, "angle rows written during graph capture");
+    if (rows.empty()) return;
+    const size_t n = 3 * rows.size();
+    if (cached) {
+        std::vector<double> flat(n);
+        std::memcpy(flat.data(), rows.data(), sizeof(double) * n);
+        auto it = c.table_contents.find(dst);
+        if (it != c.table_contents.end() && it->second == flat) return;
+        c.table_contents[dst] = std::move(flat);
+    }
+    CTDM_HIP(hipMemcpyAsync(dst, rows.data(), sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+}
+
+// Device copy of the per-angle rows (fx0, fy0, fz0) in scratch buffer `name` (eager entry points only).
 static const double* upload_rows(Ctx& c, const std::string& name, const std::vector<AngleRow>& rows)
 {
     double* d = (double*)c.buf(name, sizeof(double) * 3 * std::max<size_t>(rows.size(), 1));
-    // Under graph capture the eager pass has already uploaded identical rows (the graph key holds
-    // the angle values); a pageable host copy must not be recorded.
-    if (!rows.empty() && !c.capturing)
-        CTDM_HIP(hipMemcpyAsync(d, rows.data(), sizeof(double) * 3 * rows.size(), hipMemcpyHostToDevice,
-                                c.stream));
+    write_rows(c, d, rows);
     return d;
 }
 
@@ -997,13 +1009,12 @@ static void volume_batched_v(Ctx& c, const float2* spec, const double* rows, int
     Vol<V>::loop(c, W, n_out, dc, make_proj(g), iters, !(flags & CTDM_POS_NO_SHIFT), 1);
 }
 
-void volume_batched(Ctx& c, const float2* spec, const ctdm_angle* ang, int A, int V, const ctdm_optics& o,
+void volume_batched(Ctx& c, const float2* spec, const double* rows, int A, int V, const ctdm_optics& o,
                     int iters, unsigned flags, float2* n_out)
 {
-    CTDM_REQUIRE(spec && n_out, "volume: null pointer");
+    CTDM_REQUIRE(spec && n_out && (rows || A == 0), "volume: null pointer");
     CTDM_REQUIRE(A >= 0 && A < 32768 && iters >= 0, "volume: bad sizes");
     const EwaldGeom g = ewald_geom(o, V);
-    const double* rows = upload_rows(c, "vol.rows." + std::to_string((uintptr_t)n_out), angle_rows(g, ang, A));
     switch (V) {
     case 32: volume_batched_v<32>(c, spec, rows, A, g, iters, flags, n_out); break;
     case 64: volume_batched_v<64>(c, spec, rows, A, g, iters, flags, n_out); break;

@@ -9,6 +9,7 @@
 // batched (mode 1): all angles of a cell per launch, fused passes (3 retrieval, 3 unwrap,
 //                   2 Rytov, 3 + 4*iters volume kernels), optionally captured once into a hipGraph
 //                   and replayed.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -33,12 +34,19 @@ static void validate_cell(const ctdm_cell& cl)
     CTDM_REQUIRE(cl.optics.wavelength > 0 && cl.optics.n_med > 0 && cl.optics.dx > 0, "cell: bad optics");
 }
 
-// Device crop of a batched cell, one slot per cell (keyed by its output pointer). It is written on the
-// context stream before the cell's work each call (by the sideband search or from the given crop), so
-// the captured sequence does not depend on the carrier position.
-static CropDev* crop_slot(Ctx& c, const ctdm_cell& cl)
+// Device crop of batched cell i of a call. It is written on the context stream before the cells' work
+// each call (by the sideband search or from the given crop), so the captured sequence does not depend
+// on the carrier position. Its size never changes, so every graph sees the same slot.
+static CropDev* crop_slot(Ctx& c, int i)
 {
-    return (CropDev*)c.buf("bt.crop." + std::to_string((uintptr_t)cl.ri), sizeof(CropDev), true);
+    return (CropDev*)c.buf("bt.crop." + std::to_string(i), sizeof(CropDev), true);
+}
+
+// Device angle-row table of batched cell i, sized for A angles. Like the crop slot it is written before
+// the cells' work each call, eager or replayed, so a graph follows new angle values in the same buffers.
+static double* rows_slot(Ctx& c, int i, int A)
+{
+    return (double*)c.buf("bt.rows." + std::to_string(i), sizeof(double) * 3 * (size_t)std::max(A, 1), true);
 }
 
 static ctdm_crop cell_crop(Ctx& c, const ctdm_cell& cl)
@@ -196,35 +204,38 @@ void recon_from_maps(Ctx& c, const float* unwrapped, const float* map2, const ct
     }
     float2* spec = (float2*)c.buf("rm.spec", sizeof(float2) * A * img);
     rytov_spectrum(c, un, la, A, V, spec);
-    volume_batched(c, spec, ang, A, V, o, iters, flags, ri);
+    double* rows = (double*)c.buf("rm.rows", sizeof(double) * 3 * (size_t)A);
+    write_rows(c, rows, angle_rows(ewald_geom(o, V), ang, A));
+    volume_batched(c, spec, rows, A, V, o, iters, flags, ri);
 }
 
-static void enqueue_batched(Ctx& c, ctdm_cell& cl)
+// Enqueues cell i of a batched call. Its maps and spectra live in the current lane's scratch, reused
+// cell after cell within the lane (a lane runs its cells in order); only the crop slot and the angle
+// rows, which the host writes ahead of the cells' work, are per cell.
+static void enqueue_batched(Ctx& c, ctdm_cell& cl, int i, const double* rows)
 {
     const int A = cl.A, V = cl.V;
     const size_t n = (size_t)A * V * V;
-    // per-cell buffers (graph replays see stable pointers)
-    const std::string key = std::to_string((uintptr_t)cl.ri);
-    float* wr = cl.wrapped_opt ? cl.wrapped_opt : (float*)c.buf("bt.wr." + key, sizeof(float) * n);
-    float* la = cl.logamp_opt ? cl.logamp_opt : (float*)c.buf("bt.la." + key, sizeof(float) * n);
-    float* un = cl.unwrapped_opt ? cl.unwrapped_opt : (float*)c.buf("bt.un." + key, sizeof(float) * n);
-    float2* spec = (float2*)c.buf("bt.spec." + key, sizeof(float2) * n);
+    float* wr = cl.wrapped_opt ? cl.wrapped_opt : (float*)c.buf("bt.wr", sizeof(float) * n);
+    float* la = cl.logamp_opt ? cl.logamp_opt : (float*)c.buf("bt.la", sizeof(float) * n);
+    float* un = cl.unwrapped_opt ? cl.unwrapped_opt : (float*)c.buf("bt.un", sizeof(float) * n);
+    float2* spec = (float2*)c.buf("bt.spec", sizeof(float2) * n);
     if (cl.recon_flags & CTDM_RETRIEVE_ROWS)
         retrieve_rows(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.crop.col, wr, la);
     else
-        retrieve_batched_dev(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.V, crop_slot(c, cl), wr, la, nullptr);
+        retrieve_batched_dev(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.V, crop_slot(c, i), wr, la, nullptr);
     unwrap_dct_batched(c, wr, un, A, V, cl.unwrap_flags);
     const float* un_r = un;
     const float* la_r = la;
     if (cl.recon_flags & CTDM_MAPS_MEDIAN) {  // BatchRecon's median + checkMaps on the maps it receives
-        float* un2 = (float*)c.buf("bt.unm." + key, sizeof(float) * n);
-        float* la2 = (float*)c.buf("bt.lam." + key, sizeof(float) * n);
+        float* un2 = (float*)c.buf("bt.unm", sizeof(float) * n);
+        float* la2 = (float*)c.buf("bt.lam", sizeof(float) * n);
         prepare_maps(c, un, la, un2, la2, A, V, CTDM_MAPS_MEDIAN);
         un_r = un2;
         la_r = la2;
     }
     rytov_spectrum(c, un_r, la_r, A, V, spec);
-    volume_batched(c, spec, cl.angles, A, V, cl.optics, cl.iters, cl.recon_flags, (float2*)cl.ri);
+    volume_batched(c, spec, rows, A, V, cl.optics, cl.iters, cl.recon_flags, (float2*)cl.ri);
 }
 
 // Enqueues the batched sequence. With two lanes, cell i goes to lane i % 2: lane 0 is the context
@@ -232,10 +243,10 @@ static void enqueue_batched(Ctx& c, ctdm_cell& cl)
 // back after the last, so the call's work still ends on the context stream (also inside a graph
 // capture, where the fork and join become graph edges). A cell's latency-bound retrieval, unwrapping
 // and gridding then overlap the bandwidth-bound positivity loop of the cell on the other lane.
-static void enqueue_cells(Ctx& c, ctdm_cell* cells, int ncells, int lanes)
+static void enqueue_cells(Ctx& c, ctdm_cell* cells, int ncells, int lanes, const std::vector<double*>& rows)
 {
     if (lanes == 1 || ncells < 2) {
-        for (int i = 0; i < ncells; ++i) enqueue_batched(c, cells[i]);
+        for (int i = 0; i < ncells; ++i) enqueue_batched(c, cells[i], i, rows[i]);
         return;
     }
     c.ensure_aux();
@@ -253,7 +264,7 @@ static void enqueue_cells(Ctx& c, ctdm_cell* cells, int ncells, int lanes)
         for (int i = 0; i < ncells; ++i) {
             c.lane = i & 1;
             c.stream = c.lane ? c.aux : main;
-            enqueue_batched(c, cells[i]);
+            enqueue_batched(c, cells[i], i, rows[i]);
         }
     } catch (...) {
         join();  // a capture can only end once lane 1 has joined back
@@ -262,33 +273,27 @@ static void enqueue_cells(Ctx& c, ctdm_cell* cells, int ncells, int lanes)
     CTDM_HIP(join());
 }
 
+// The captured sequence depends on pointers, shapes, flags and optics (launch arguments). Crops (except
+// the row-wise retrieval's column, a launch argument) and angle rows are read from device tables that
+// are rewritten before every launch, so their values are not part of the key.
 static GraphKey graph_key(const ctdm_cell* cells, int ncells)
 {
     GraphKey k;
     for (int i = 0; i < ncells; ++i) {
         const ctdm_cell& cl = cells[i];
-        // The batched retrieval reads its crop from the cell's device slot; only the row-wise retrieval
-        // takes the sideband column as a launch argument.
         const bool rows = cl.recon_flags & CTDM_RETRIEVE_ROWS;
         for (uintptr_t w : {(uintptr_t)cl.holo, (uintptr_t)cl.bg, (uintptr_t)cl.ri, (uintptr_t)cl.wrapped_opt,
                             (uintptr_t)cl.unwrapped_opt, (uintptr_t)cl.logamp_opt, (uintptr_t)cl.A,
                             (uintptr_t)cl.H, (uintptr_t)cl.W, (uintptr_t)cl.V, (uintptr_t)cl.iters,
                             (uintptr_t)cl.unwrap_flags, (uintptr_t)cl.recon_flags,
                             (uintptr_t)(rows ? cl.crop.row : 0), (uintptr_t)(rows ? cl.crop.col : 0),
-                            (uintptr_t)(rows ? cl.crop.radius : 0), (uintptr_t)cl.V})
+                            (uintptr_t)(rows ? cl.crop.radius : 0)})
             k.words.push_back(w);
         uint64_t bits[3];
         std::memcpy(&bits[0], &cl.optics.wavelength, 8);
         std::memcpy(&bits[1], &cl.optics.n_med, 8);
         std::memcpy(&bits[2], &cl.optics.dx, 8);
         for (uint64_t b : bits) k.words.push_back((uintptr_t)b);
-        for (int a = 0; a < cl.A; ++a) {  // angle values, not just the host pointer
-            uint64_t ax, ay;
-            std::memcpy(&ax, &cl.angles[a].ang_x, 8);
-            std::memcpy(&ay, &cl.angles[a].ang_y, 8);
-            k.words.push_back((uintptr_t)ax);
-            k.words.push_back((uintptr_t)ay);
-        }
     }
     return k;
 }
@@ -323,9 +328,9 @@ void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& op
             cl.crop = cell_crop(c, cl);
         } else if (cl.crop.radius >= 0) {
             validate_crop(cl.H, cl.W, cl.crop);
-            set_crop(c, crop_slot(c, cl), cl.crop, cl.H, cl.W);
+            set_crop(c, crop_slot(c, i), cl.crop, cl.H, cl.W);
         } else {
-            enqueue_sideband(c, cl.bg, cl.H, cl.W, cl.V, 0, crop_slot(c, cl), keys_dev + k++);
+            enqueue_sideband(c, cl.bg, cl.H, cl.W, cl.V, 0, crop_slot(c, i), keys_dev + k++);
         }
     }
     if (!located.empty()) {
@@ -348,51 +353,75 @@ void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& op
         for (int i = 0; i < ncells; ++i) sizes.push_back(cells[i].V);
         autotune(c, sizes.data(), (int)sizes.size(), false);
     }
-    if (!opts.graph) {
-        enqueue_cells(c, cells, ncells, opts.lanes);
+    // Angle rows of every cell, written ahead of the cells' work on the context stream (before lane 1 is
+    // forked from it), into the tables the launch sequence reads.
+    std::vector<std::vector<AngleRow>> host_rows(ncells);
+    for (int i = 0; i < ncells; ++i)
+        host_rows[i] = angle_rows(ewald_geom(cells[i].optics, cells[i].V), cells[i].angles, cells[i].A);
+    GraphKey key;
+    std::map<GraphKey, GraphEntry>::iterator it = c.graphs.end();
+    if (opts.graph) {
+        key = graph_key(cells, ncells);
+        key.words.push_back((uintptr_t)(opts.lanes == 1 ? 1 : 2));
+        it = c.graphs.find(key);
+    }
+    if (it != c.graphs.end()) {  // replay: the graph's own row tables get this call's angles
+        GraphEntry& ge = it->second;
+        for (int i = 0; i < ncells; ++i) write_rows(c, ge.rows[i], host_rows[i], true);
+        ge.last_use = ++c.graph_clock;
+        CTDM_HIP(hipGraphLaunch(ge.exec, c.stream));
         finish_crops();
         return;
     }
-    GraphKey key = graph_key(cells, ncells);
-    key.words.push_back((uintptr_t)(opts.lanes == 1 ? 1 : 2));
-    auto it = c.graphs.find(key);
-    if (it == c.graphs.end()) {
-        // Eager pass: allocates every buffer and table the sequence needs and produces the result.
-        enqueue_cells(c, cells, ncells, opts.lanes);
-        CTDM_HIP(hipStreamSynchronize(c.stream));
-        hipStream_t cs = nullptr;
-        CTDM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-        hipStream_t saved = c.stream;
-        c.stream = cs;
-        c.capturing = true;
-        GraphEntry ge;
+    std::vector<double*> rows(ncells);
+    for (int i = 0; i < ncells; ++i) {
+        rows[i] = rows_slot(c, i, cells[i].A);
+        write_rows(c, rows[i], host_rows[i], true);
+    }
+    if (!opts.graph) {
+        enqueue_cells(c, cells, ncells, opts.lanes, rows);
+        finish_crops();
+        return;
+    }
+    // Eager pass: allocates every buffer and table the sequence needs and produces the result.
+    enqueue_cells(c, cells, ncells, opts.lanes, rows);
+    CTDM_HIP(hipStreamSynchronize(c.stream));
+    c.trim_graphs(kMaxGraphs - 1);  // room for the new graph; frees buffers only evicted graphs held
+    hipStream_t cs = nullptr;
+    CTDM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipStream_t saved = c.stream;
+    c.stream = cs;
+    c.capturing = true;
+    GraphEntry ge;
+    try {
+        CTDM_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         try {
-            CTDM_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            try {
-                enqueue_cells(c, cells, ncells, opts.lanes);
-            } catch (...) {
-                hipGraph_t g = nullptr;
-                hipStreamEndCapture(cs, &g);
-                if (g) hipGraphDestroy(g);
-                throw;
-            }
-            CTDM_HIP(hipStreamEndCapture(cs, &ge.graph));
-            CTDM_HIP(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0));
+            enqueue_cells(c, cells, ncells, opts.lanes, rows);
         } catch (...) {
-            c.capturing = false;
-            c.stream = saved;
-            hipStreamDestroy(cs);
+            hipGraph_t g = nullptr;
+            hipStreamEndCapture(cs, &g);
+            if (g) hipGraphDestroy(g);
             throw;
         }
+        CTDM_HIP(hipStreamEndCapture(cs, &ge.graph));
+        CTDM_HIP(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0));
+    } catch (...) {
         c.capturing = false;
         c.stream = saved;
-        CTDM_HIP(hipStreamDestroy(cs));
-        for (auto& kv : c.scratch) kv.second.pinned = true;  // the graph holds these pointers
-        c.graphs[key] = ge;
-        finish_crops();
-        return;
+        hipStreamDestroy(cs);
+        throw;
     }
-    CTDM_HIP(hipGraphLaunch(it->second.exec, c.stream));
+    c.capturing = false;
+    c.stream = saved;
+    CTDM_HIP(hipStreamDestroy(cs));
+    // The graph holds every scratch pointer it was captured with (a superset of what it reads): such a
+    // buffer is retired rather than freed if it grows while the graph is cached.
+    for (const auto& kv : c.scratch)
+        if (kv.second.ptr) ge.ptrs.push_back(kv.second.ptr);
+    std::sort(ge.ptrs.begin(), ge.ptrs.end());
+    ge.rows = rows;
+    ge.last_use = ++c.graph_clock;
+    c.graphs[key] = ge;
     finish_crops();
 }
 
