@@ -100,6 +100,7 @@ SIGNATURES = {
     "ctdm_angle_from_image": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                C.POINTER(C.c_double)], C.c_int),
     "ctdm_unwrap_dct": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_uint], C.c_int),
+    "ctdm_unwrap_dct_noalloc": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_uint], C.c_int),
     "ctdm_unwrap_dct_adjoint": ([_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_uint], C.c_int),
     "ctdm_goldstein": ([_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp], C.c_int),
     "ctdm_rytov_spectrum": ([_vp, _vp, _vp, C.c_int, C.c_int, _vp], C.c_int),
@@ -354,6 +355,16 @@ class Context:
         out = self._torch.empty_like(w)
         self._call("ctdm_unwrap_dct", _ptr(w), _ptr(out), w.shape[0], w.shape[-1], flags)
         return out if wrapped.dim() == 3 else out[0]
+
+    def unwrap_dct_noalloc(self, wrapped, out, flags: int = 0):
+        """ctdm_unwrap_dct_noalloc: the same solve with the library's fixed device scratch; the caller
+        passes the output buffer too, so the call allocates nothing."""
+        w = wrapped if wrapped.dim() == 3 else wrapped[None]
+        o = out if out.dim() == 3 else out[None]
+        if o.shape != w.shape or o.dtype != w.dtype:
+            raise ValueError("out must match wrapped")
+        self._call("ctdm_unwrap_dct_noalloc", _ptr(w), _ptr(o), w.shape[0], w.shape[-1], flags)
+        return out
 
     def unwrap_dct_adjoint(self, wrapped, grad_out, flags: int = 0):
         w = wrapped if wrapped.dim() == 3 else wrapped[None]

@@ -181,6 +181,11 @@ ctdm_status ctdm_create(ctdm_ctx** out, const ctdm_config* cfg, ctdm_stream stre
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->c.device) == hipSuccess)
         ctx->c.max_lds = (int)prop.sharedMemPerBlock;
+    const ctdm_status st = guarded(ctx, [](ctdm::Ctx& c) { ctdm::noalloc_setup(c); });
+    if (st != CTDM_OK) {
+        ctdm_destroy(ctx);
+        return st;
+    }
     *out = ctx;
     return CTDM_OK;
 }
@@ -281,6 +286,12 @@ ctdm_status ctdm_unwrap_dct(ctdm_ctx* ctx, const float* wrapped, float* unwrappe
             ctdm::unwrap_dct_batched(c, wrapped, unwrapped, batch, M, flags);
         }
     });
+}
+
+ctdm_status ctdm_unwrap_dct_noalloc(ctdm_ctx* ctx, const float* wrapped, float* unwrapped, int batch, int M,
+                                    unsigned flags)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) { ctdm::unwrap_dct_noalloc(c, wrapped, unwrapped, batch, M, flags); });
 }
 
 ctdm_status ctdm_unwrap_dct_adjoint(ctdm_ctx* ctx, const float* wrapped, const float* g_out,

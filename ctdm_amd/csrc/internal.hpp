@@ -99,6 +99,7 @@ struct Ctx {
     // remaining graph references (synchronises the context stream first).
     void trim_graphs(size_t keep);
     bool capturing = false;
+    float* noalloc_scratch = nullptr;  // this device's address of the allocation-free entry point's array
     int max_lds = 65536;
     int dc_V = 0;    // volume side the cached data-consistency list ("dc.*") was built for
     int dc_cap = 0;  // its entries per pencil
@@ -179,6 +180,10 @@ void unwrap_dct_batched(Ctx& c, const float* wrapped, float* unwrapped, int batc
 void unwrap_dct_plain_one(Ctx& c, const float* wrapped, float* unwrapped, int M, unsigned flags);
 void unwrap_dct_adjoint(Ctx& c, const float* wrapped, const float* g_out, float* g_in, int batch,
                         int M, unsigned flags);
+// Allocation-free entry point: device scratch address per device and the unwrap tables (context set-up),
+// then solves with the fixed scratch array (no allocation, no table building).
+void noalloc_setup(Ctx& c);
+void unwrap_dct_noalloc(Ctx& c, const float* wrapped, float* unwrapped, int batch, int M, unsigned flags);
 
 // Goldstein (k_goldstein.hip)
 void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* res, uint8_t* cuts,

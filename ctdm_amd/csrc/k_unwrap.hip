@@ -25,6 +25,7 @@
 #include "internal.hpp"
 
 #include <cmath>
+#include <mutex>
 
 namespace ctdm {
 
@@ -275,11 +276,12 @@ __global__ void k_uw_stencil(const float* __restrict__ v, const float* __restric
 }
 
 // ---------------------------------------------------------------- host side
-static const double* eig_table(Ctx& c, int N, bool refcompat)
+static const double* eig_table(Ctx& c, int N, bool refcompat, bool existing = false)
 {
     const std::string name = std::string(refcompat ? "uw.eigref." : "uw.eig.") + std::to_string(N);
     auto it = c.scratch.find(name);
     if (it != c.scratch.end() && it->second.ptr) return (const double*)it->second.ptr;
+    CTDM_REQUIRE(!existing, "unwrap: eigenvalue table missing (it is built when the context is created)");
     std::vector<double> h(N);
     for (int k = 0; k < N; ++k) {
         if (refcompat)
@@ -295,10 +297,10 @@ static const double* eig_table(Ctx& c, int N, bool refcompat)
 }
 
 template <int N, int LN>
-static void run_cols(Ctx& c, const float* in, int batch, unsigned flags, float* out)
+static void run_cols(Ctx& c, const float* in, int batch, unsigned flags, float* out, bool existing)
 {
     constexpr int C = (256 / Plan<LN>::T) < N ? (256 / Plan<LN>::T) : N;
-    const double* eig = eig_table(c, N, flags & F_REF_EIG);
+    const double* eig = eig_table(c, N, flags & F_REF_EIG, existing);
     const float2* tw2 = (LN == 2 * N) ? c.tw(2 * N) : c.tw(N);
     hipLaunchKernelGGL((k_uw_cols<N, C, LN>), dim3(batch * (N / C)), dim3(C * Plan<LN>::T), 0, c.stream,
                        in, batch, flags, eig, c.tw(N), tw2, c.tw(4 * N), out);
@@ -325,38 +327,103 @@ static void run_rows_fwd(Ctx& c, const float* in, int batch, int mode, bool sinc
     CTDM_CHECK_LAUNCH("k_uw_rows_fwd");
 }
 
+// Intermediate spectra of one solve: context scratch, or (allocation-free entry point) the fixed
+// device array. `existing` also forbids building a missing table.
+struct Work {
+    float* D1 = nullptr;
+    float* D2 = nullptr;
+    bool existing = false;
+};
+
 // Whole solve for one size: mode 0 (wrapped input) or 1 (raw input, adjoint).
 template <int N>
 static void solve(Ctx& c, const float* in, int batch, unsigned flags, int mode, const float* wrapped,
-                  float* out)
+                  float* out, Work w)
 {
-    float* D1 = (float*)c.buf("uw.D1", sizeof(float) * (size_t)batch * N * N);
-    float* D2 = (float*)c.buf("uw.D2", sizeof(float) * (size_t)batch * N * N);
-    run_rows_fwd<N>(c, in, batch, mode, flags & F_SINCOS, D1);
+    if (!w.D1) {
+        w.D1 = (float*)c.buf("uw.D1", sizeof(float) * (size_t)batch * N * N);
+        w.D2 = (float*)c.buf("uw.D2", sizeof(float) * (size_t)batch * N * N);
+    }
+    run_rows_fwd<N>(c, in, batch, mode, flags & F_SINCOS, w.D1);
     if (flags & F_REF_IDCT) {
         if constexpr (N <= 256) {
-            run_cols<N, 2 * N>(c, D1, batch, flags, D2);
-            run_rows_inv<N, 2 * N>(c, D2, batch, flags, wrapped, out);
+            run_cols<N, 2 * N>(c, w.D1, batch, flags, w.D2, w.existing);
+            run_rows_inv<N, 2 * N>(c, w.D2, batch, flags, wrapped, out);
         } else {
             fail(CTDM_EUNSUPPORTED, "REFCOMPAT_IDCT needs N <= 256");
         }
     } else {
-        run_cols<N, N>(c, D1, batch, flags, D2);
-        run_rows_inv<N, N>(c, D2, batch, flags, wrapped, out);
+        run_cols<N, N>(c, w.D1, batch, flags, w.D2, w.existing);
+        run_rows_inv<N, N>(c, w.D2, batch, flags, wrapped, out);
     }
 }
 
 static void solve_any(Ctx& c, const float* in, int batch, int N, unsigned flags, int mode,
-                      const float* wrapped, float* out)
+                      const float* wrapped, float* out, Work w = Work{})
 {
     switch (N) {
-    case 32: solve<32>(c, in, batch, flags, mode, wrapped, out); break;
-    case 64: solve<64>(c, in, batch, flags, mode, wrapped, out); break;
-    case 128: solve<128>(c, in, batch, flags, mode, wrapped, out); break;
-    case 192: solve<192>(c, in, batch, flags, mode, wrapped, out); break;
-    case 256: solve<256>(c, in, batch, flags, mode, wrapped, out); break;
+    case 32: solve<32>(c, in, batch, flags, mode, wrapped, out, w); break;
+    case 64: solve<64>(c, in, batch, flags, mode, wrapped, out, w); break;
+    case 128: solve<128>(c, in, batch, flags, mode, wrapped, out, w); break;
+    case 192: solve<192>(c, in, batch, flags, mode, wrapped, out, w); break;
+    case 256: solve<256>(c, in, batch, flags, mode, wrapped, out, w); break;
     default: fail(CTDM_EUNSUPPORTED, "unwrap size " + std::to_string(N) + " has no kernel");
     }
+}
+
+// ---------------------------------------------------------------- allocation-free entry point
+// The two intermediate spectra of ctdm_unwrap_dct_noalloc, CTDM_NOALLOC_MAX_ELEMS floats each, in one
+// file-scope device array. Every device the library runs on holds its own copy until the process ends.
+// In host code `scratch` is a host-side stand-in: kernels get the per-device address that
+// hipGetSymbolAddress returns (noalloc_setup), never the array's name.
+static __device__ __align__(256) float scratch[2 * (size_t)CTDM_NOALLOC_MAX_ELEMS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void noalloc_setup(Ctx& c)
+{
+    CTDM_REQUIRE(c.device >= 0 && c.device < kMaxDevices, "device ordinal out of range");
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[c.device]) {  // once per device, with that device current
+            float* ptr = nullptr;
+            CTDM_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+            CTDM_REQUIRE(ptr != nullptr, "scratch symbol has no device address");
+            g_scratch_ptr[c.device] = ptr;
+        }
+        c.noalloc_scratch = g_scratch_ptr[c.device];
+    }
+    // Every table the unwrap kernels read, built now so that the allocation-free call finds them.
+    for (int N : {32, 64, 128, 192, 256}) {
+        eig_table(c, N, false);
+        eig_table(c, N, true);
+        c.tw(N);
+        c.tw(2 * N);
+        c.tw(4 * N);
+    }
+}
+
+void unwrap_dct_noalloc(Ctx& c, const float* wrapped, float* unwrapped, int batch, int M, unsigned flags)
+{
+    CTDM_REQUIRE(wrapped && unwrapped, "unwrap_dct_noalloc: null pointer");
+    CTDM_REQUIRE(batch >= 0 && M > 0, "unwrap_dct_noalloc: bad size");
+    CTDM_REQUIRE(wrapped != unwrapped, "unwrap_dct_noalloc: in-place call not supported");
+    CTDM_REQUIRE((flags & ~0xfu) == 0, "unwrap_dct_noalloc: unknown flag");
+    CTDM_REQUIRE((size_t)batch * M * M <= (size_t)CTDM_NOALLOC_MAX_ELEMS,
+                 "unwrap_dct_noalloc: batch * M * M = " + std::to_string((size_t)batch * M * M) +
+                     " exceeds the fixed scratch of " + std::to_string((size_t)CTDM_NOALLOC_MAX_ELEMS) +
+                     " elements");
+    CTDM_REQUIRE(c.noalloc_scratch, "unwrap_dct_noalloc: scratch address not set up");
+    if (batch == 0) return;
+    Work w;
+    w.D1 = c.noalloc_scratch;
+    w.D2 = c.noalloc_scratch + (size_t)CTDM_NOALLOC_MAX_ELEMS;
+    w.existing = true;
+    solve_any(c, wrapped, batch, M, flags, 0, wrapped, unwrapped, w);
 }
 
 void unwrap_dct_batched(Ctx& c, const float* wrapped, float* unwrapped, int batch, int M,

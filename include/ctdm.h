@@ -150,6 +150,19 @@ ctdm_status ctdm_angle_from_image(ctdm_ctx *ctx, const float *img, int batch, in
 ctdm_status ctdm_unwrap_dct(ctdm_ctx *ctx, const float *wrapped, float *unwrapped, int batch, int M,
                             unsigned flags);
 
+/* Allocation-free variant of ctdm_unwrap_dct. It has no counterpart in the reference: it is for callers
+ * that cannot allocate device memory during the call (code under hipGraph stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Its two intermediate spectra live in a fixed device array of
+ * 2 x CTDM_NOALLOC_MAX_ELEMS floats (32 MiB) that every GPU the library runs on holds until the process
+ * ends; ctdm_create sets up that array's address and builds the tables the call reads, so the call itself
+ * allocates nothing. batch * M * M above CTDM_NOALLOC_MAX_ELEMS is rejected with CTDM_EINVAL. All calls
+ * on one GPU share the array (it keeps the last call's contents, which every call overwrites before
+ * reading): calls that overlap on one GPU, from two streams or two contexts, race. Same flags, kernels
+ * and results as ctdm_unwrap_dct (CTDM_EXEC_PLAIN is not accepted). */
+#define CTDM_NOALLOC_MAX_ELEMS (1u << 22)
+ctdm_status ctdm_unwrap_dct_noalloc(ctdm_ctx *ctx, const float *wrapped, float *unwrapped, int batch, int M,
+                                    unsigned flags);
+
 /* Adjoint (transpose) of the Jacobian of ctdm_unwrap_dct at `wrapped` (no reference counterpart):
  * grad_wrapped = J^T grad_unwrapped. Wrapped differences have slope 1 almost everywhere, so for
  * WRAPDIFF the Jacobian is the linear Poisson operator; SINCOS linearises at `wrapped`. */
