@@ -81,7 +81,8 @@ class SynthParams(C.Structure):
                 ("n_med", C.c_double), ("na_ill", C.c_double), ("n_cell", C.c_double),
                 ("n_nucleoli", C.c_double), ("kappa_cell", C.c_double),
                 ("semi_axes", C.c_double * 3), ("n_nuc", C.c_int), ("r_nuc", C.c_double),
-                ("noise", C.c_double), ("carrier", C.c_double * 2), ("field_noise", C.c_double)]
+                ("noise", C.c_double), ("carrier", C.c_double * 2), ("field_noise", C.c_double),
+                ("model", C.c_int)]
 
 
 _vp = C.c_void_p
@@ -124,6 +125,7 @@ SIGNATURES = {
     "ctdm_rimap_read": ([C.c_char_p, _vp, C.c_size_t, C.c_int], C.c_int),
     "ctdm_synth_default_params": ([C.POINTER(SynthParams)], None),
     "ctdm_synth_cell": ([C.c_uint64, C.c_int, C.POINTER(SynthParams), _vp, _vp, C.POINTER(Angle)], C.c_int),
+    "ctdm_synth_phantom": ([C.c_uint64, C.POINTER(SynthParams), C.c_int, C.c_double, _vp], C.c_int),
     "ctdm_synth_hologram": ([C.c_uint64, C.c_int, _vp, _vp], C.c_int),
     "ctdm_synth_wrapped": ([C.c_uint64, C.c_int, C.c_int, C.c_int, _vp], C.c_int),
 }
@@ -246,6 +248,14 @@ def synth_cell(seed: int, A: int, params: SynthParams | None = None):
     _check_free(lib().ctdm_synth_cell(seed, A, C.byref(p), holo.ctypes.data, bg.ctypes.data, ang))
     angles = np.array([(a.ang_x, a.ang_y) for a in ang], np.float64)
     return holo, bg, angles
+
+
+def synth_phantom(seed: int, V: int, dx: float, params: SynthParams | None = None):
+    """Refractive index of the phantom synth_cell(seed, ...) images, on the V^3 output grid ([z][y][x])."""
+    p = params or synth_params()
+    n = np.empty((V, V, V), np.complex64)
+    _check_free(lib().ctdm_synth_phantom(seed, C.byref(p), V, dx, n.ctypes.data))
+    return n
 
 
 def synth_wrapped(seed: int, batch: int, H: int, W: int):

@@ -365,13 +365,21 @@ typedef struct ctdm_synth_params {
     double noise;       /* Gaussian intensity noise sigma */
     double carrier[2];  /* carrier frequency [cycles / pixel] along x and y */
     double field_noise; /* sigma of complex Gaussian noise added to the sample field A exp(i phi) */
+    int model;          /* 1 = first-order Rytov fields of the phantom (default), 0 = straight-ray projections */
 } ctdm_synth_params;
 
 void ctdm_synth_default_params(ctdm_synth_params *p);
-/* A angles on a cone (n_med sin theta = na_ill, azimuth 2*pi*a/A) -> angles_host[A];
- * holo_host/bg_host: A x H x H float32 each (host). */
+/* A angles on a cone (n_med sin theta = na_ill, azimuth 2*pi*a/A; ang_y is the tilt along image y, as the
+ * reference's transposed gridding reads it) -> angles_host[A]; holo_host/bg_host: A x H x H float32 each
+ * (host), H a power of two. */
 ctdm_status ctdm_synth_cell(uint64_t seed, int A, const ctdm_synth_params *p, float *holo_host,
                             float *bg_host, ctdm_angle *angles_host);
+/* The refractive index (n + i kappa) of the phantom ctdm_synth_cell images with the same seed and
+ * parameters, sampled at the voxel centres of a V^3 reconstruction of pitch dx [m] in the pipeline's
+ * output orientation (volume x = image y, volume y = image x, voxel (V/2, V/2, V/2) at the origin);
+ * n_host: V^3 values, idx = x + V*y + V*V*z (host). */
+ctdm_status ctdm_synth_phantom(uint64_t seed, const ctdm_synth_params *p, int V, double dx,
+                               ctdm_complex *n_host);
 /* Single off-axis hologram (sphere phase of radius 80 px, peak 12 rad, plus a 2nd-order
  * background) and a background, H x H each (host). */
 ctdm_status ctdm_synth_hologram(uint64_t seed, int H, float *holo_host, float *bg_host);

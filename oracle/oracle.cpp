@@ -17,8 +17,14 @@
 //                           phase/amplitude is recovered; brute-force cropped DFT at small H
 //   orc_find_sideband       carrier position (0.25, 0.25) -> (768, 768), radius 85
 //   orc_goldstein           hand-enumerated residue / cut / fill cases; integer invariants
-//   orc_ewald_* / recon     single-plane-wave closed form; scatter/gather adjointness; the
-//                           est_n / modify_F identity on feasible n; idempotent positivity
+//   orc_ewald_* / recon     the first-order Born field of a Gaussian potential, computed by direct
+//                           quadrature with the Green's function e^{ikR}/(4 pi R), lands on every
+//                           measured voxel as -O(K)/dx^3 (weight 4 pi f_z, 1/dx, metre units and the
+//                           transposed axes); a literal numpy restatement of fillEdwardSphere and
+//                           Initial_F_F2 with a twice-hit voxel; the whole chain recovers a synthetic
+//                           cell's own refractive index (a planted factor 2 fails);
+//                           scatter/gather adjointness; the est_n / modify_F identity on feasible n;
+//                           idempotent positivity
 //   orc_fit_pgd             step 1 == positivity loop; step 0 keeps a feasible start; step 1/2
 //   orc_median3x3           numpy median of every 3x3 window on finite data, borders unchanged;
 //                           hand-enumerated NaN windows for the selection order

@@ -6,6 +6,7 @@ import pytest
 import torch
 
 import ctdm_amd
+from phantom_metrics import check_recovery
 
 pytestmark = pytest.mark.gpu
 LAM, NM = 532e-9, 1.337
@@ -179,7 +180,10 @@ def test_graph_replay_equals_eager(ctx):
 
 
 def test_config2_full_size(ctx, orc):
-    """60 angles of 1024^2 holograms -> 256^3 RI, 10 positivity iterations (the headline workload)."""
+    """60 angles of 1024^2 holograms -> 256^3 RI, 10 positivity iterations (the headline workload), with
+    first-order Rytov fields of the cell phantom: batched and plain within 1e-5 of the oracle, and the
+    HIP volume recovers the phantom itself (contrast, nucleoli, correlation; bars in
+    test_oracle_phantom.py)."""
     A, H, V, iters = 60, 1024, 256, 10
     holo, bg, ang = _cell(2, A, H, V)
     rb, cb = _run(ctx, holo, bg, ang, V, iters, mode=1, graph=True)
@@ -187,11 +191,14 @@ def test_config2_full_size(ctx, orc):
     assert cr.astuple() == (768, 768, 85, 256)
     o = ctdm_amd.optics_for(V, H)
     ref = orc.reconstruct_cell(holo, bg, ang, V, cr.astuple(), LAM, NM, o.dx, iters)
-    eb = np.abs(rb.cpu().numpy() - ref).max()
+    got = rb.cpu().numpy()
+    eb = np.abs(got - ref).max()
     rp, _ = _run(ctx, holo, bg, ang, V, iters, mode=0)
     ep = np.abs(rp.cpu().numpy() - ref).max()
     print(f"config2: batched max|dn| {eb:.2e}, plain {ep:.2e}; RI {ref.real.min():.4f}..{ref.real.max():.4f}")
     assert eb <= 1e-5 and ep <= 1e-5
+    p = ctdm_amd.synth_params(H=H)
+    assert check_recovery(got, ctdm_amd.synth_phantom(2, V, o.dx, p), p, o.dx)
 
 
 def test_autotune_keeps_results_and_times_passes(ctx):

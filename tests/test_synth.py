@@ -31,3 +31,24 @@ def test_cell_angles_on_cone():
     s = np.sqrt(np.sin(ang[:, 0]) ** 2 + np.sin(ang[:, 1]) ** 2)
     np.testing.assert_allclose(s, p.na_ill / p.n_med, rtol=1e-12)
     assert np.isfinite(holo).all() and np.isfinite(bg).all()
+
+
+def test_phantom_render_and_field_models():
+    """The phantom on the output grid holds the cell and nucleoli indices in the transposed orientation
+    (volume x = image y: the 12 um image-x axis of the cell lies along volume y), and both field models
+    give finite holograms whose angles put the tilt of ang_y along image y."""
+    p = ctdm_amd.synth_params(H=128)
+    V, dx = 64, 0.4e-6
+    n = ctdm_amd.synth_phantom(5, V, dx, p)
+    assert n.shape == (V, V, V)
+    vals = np.unique(n.real[n.real > p.n_med + 1e-6].astype(np.float64))
+    assert np.isclose(vals, p.n_cell, atol=1e-6).any() and vals.max() >= p.n_nucleoli - 1e-6
+    cell = n.real > p.n_med + 1e-6
+    ext_y = np.any(cell, axis=(0, 2)).sum() * dx  # along volume y (image x): 2 x 6 um
+    ext_x = np.any(cell, axis=(0, 1)).sum() * dx  # along volume x (image y): 2 x 5 um
+    assert abs(ext_y - 12e-6) <= dx and abs(ext_x - 10e-6) <= dx
+    for model in (0, 1):
+        holo, bg, ang = ctdm_amd.synth_cell(5, 4, ctdm_amd.synth_params(H=128, model=model))
+        assert np.isfinite(holo).all()
+        # azimuth 90 degrees (a = 1 of 4) tilts along image y: ang_y carries it
+        assert abs(ang[1, 0]) < 1e-12 and ang[1, 1] > 0.5
