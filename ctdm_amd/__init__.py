@@ -26,6 +26,7 @@ UNWRAP_SINCOS = 0x1
 UNWRAP_REFCOMPAT_EIG = 0x2
 UNWRAP_REFCOMPAT_IDCT = 0x4
 UNWRAP_CONGRUENT = 0x8
+UNWRAP_REFCOMPAT_FAST = 0x10
 UNWRAP_REFCOMPAT = UNWRAP_REFCOMPAT_EIG | UNWRAP_REFCOMPAT_IDCT
 
 DC_HITMASK = 0x0
@@ -118,6 +119,8 @@ SIGNATURES = {
     "ctdm_time_pass": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "ctdm_time_loop": ([_vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "ctdm_fit_pgd": ([_vp, _vp, _vp, C.POINTER(Angle), C.c_int, C.c_int, C.POINTER(Optics), C.c_int, C.c_double, _vp, C.c_uint], C.c_int),
+    "ctdm_fit_pgd_trace": ([_vp, _vp, _vp, C.POINTER(Angle), C.c_int, C.c_int, C.POINTER(Optics), C.c_int, C.c_double,
+                            _vp, C.c_uint, C.POINTER(C.c_double)], C.c_int),
     "ctdm_phimap_write": ([C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "ctdm_phimap_read": ([C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
                           C.POINTER(C.c_int), C.POINTER(C.c_int), _vp, _vp, C.c_size_t], C.c_int),
@@ -445,6 +448,15 @@ class Context:
                    C.byref(optics), iters, step, _ptr(n), flags)
         return n
 
+    def fit_pgd_trace(self, f, spec, angles, optics: Optics, iters: int, step: float, flags: int = 0):
+        """(n, objective): the fit and J before each of its steps (list of iters floats)."""
+        arr = angles_array(angles)
+        n = self._torch.empty_like(f)
+        J = (C.c_double * max(iters, 1))()
+        self._call("ctdm_fit_pgd_trace", _ptr(f), _ptr(spec), arr, len(arr), f.shape[0],
+                   C.byref(optics), iters, step, _ptr(n), flags, J)
+        return n, list(J)[:iters]
+
     def median3x3(self, maps):
         """3x3 median of each map (batch x H x W float32), borders copied (MedianFilter_gpu)."""
         m = maps if maps.dim() == 3 else maps[None]
@@ -522,8 +534,9 @@ class Context:
         return self.unwrap_dct(src, flags)
 
     def FastUnwrapping(self, src):
-        """Reference FastUnwrapping (FastUnwrapping.cu:5): sin/cos-Laplacian RHS + congruence."""
-        return self.unwrap_dct(src, UNWRAP_SINCOS | UNWRAP_CONGRUENT)
+        """Reference FastUnwrapping (FastUnwrapping.cu:5): the map rescaled to [0, 2 pi], sin/cos-Laplacian
+        RHS, Poisson solve, congruence, minimum subtracted (the input tensor is not modified)."""
+        return self.unwrap_dct(src, UNWRAP_SINCOS | UNWRAP_CONGRUENT | UNWRAP_REFCOMPAT_FAST)
 
     def cudaGoldsteinUnwrap2D(self, phi):
         """Reference cudaGoldsteinUnwrap2D(phi, out, nx, ny, frame) (Goldstein.cu:5)."""

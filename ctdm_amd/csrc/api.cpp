@@ -432,4 +432,16 @@ ctdm_status ctdm_fit_pgd(ctdm_ctx* ctx, ctdm_complex* f, const ctdm_complex* spe
     });
 }
 
+ctdm_status ctdm_fit_pgd_trace(ctdm_ctx* ctx, ctdm_complex* f, const ctdm_complex* spec,
+                               const ctdm_angle* angles, int A, int V, const ctdm_optics* optics, int iters,
+                               double step, ctdm_complex* n, unsigned flags, double* objective)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        check_optics(optics);
+        CTDM_REQUIRE(objective || iters == 0, "fit_pgd_trace: null objective array");
+        ctdm::fit_pgd(c, (float2*)f, (const float2*)spec, angles, A, V, *optics, iters, step,
+                      (float2*)n, flags, objective);
+    });
+}
+
 }  // extern "C"

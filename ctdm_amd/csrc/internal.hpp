@@ -220,8 +220,9 @@ void volume_batched(Ctx& c, const float2* spec, const double* rows, int A, int V
 // under capture (a host copy must not be recorded). With `cached`, the copy is skipped when `dst`
 // already holds these rows (only for tables written through this function alone).
 void write_rows(Ctx& c, double* dst, const std::vector<AngleRow>& rows, bool cached = false);
+// objective: HOST array of iters values (may be null): J before each step; synchronises when given.
 void fit_pgd(Ctx& c, float2* f, const float2* spec, const ctdm_angle* ang, int A, int V,
-             const ctdm_optics& o, int iters, double step, float2* n, unsigned flags);
+             const ctdm_optics& o, int iters, double step, float2* n, unsigned flags, double* objective = nullptr);
 
 // Average ms of one launch of volume pass `kind` (0 x, 1 y, 2 z+dc) with B pencils per workgroup.
 double time_pass(Ctx& c, int kind, int V, int B, int reps, double* bytes);

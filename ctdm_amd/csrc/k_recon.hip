@@ -510,7 +510,28 @@ struct DcList {
     short* idx;
     float2* val;
     int cap;
+    // The fit's objective (both null otherwise): hit count of every list entry, written by the gridding
+    // kernel, and one double per workgroup for the partial sums of the objective (gridding: its constant
+    // part; z pass: the residual part before the step).
+    short* hit = nullptr;
+    double* part = nullptr;
 };
+
+// Sum of one double per thread over a workgroup of NT threads, in thread order (deterministic); every
+// thread gets the total. `red` is NT doubles of LDS.
+template <int NT>
+__device__ __forceinline__ double workgroup_sum(double v, double* red)
+{
+    red[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < NT; ++i) t += red[i];
+        red[0] = t;
+    }
+    __syncthreads();
+    return red[0];
+}
 
 // Pencil tile geometry of the volume passes. AXIS 0: rows (x), 1: y, 2: z. B pencils per workgroup.
 template <int V, int AXIS, int B>
@@ -599,7 +620,9 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_x(float2* __restrict__ W,
 // count) so their latency overlaps the data loads; longer lists finish in a loop.
 // With tau != 1 a measured voxel moves only part of the way: F <- F + tau (m - F), the fit's
 // hit-preconditioned gradient step (see fit_pgd); tau == 1 is the exact replacement.
-template <int V, int B>
+// JT (the fit with its objective): before the step, each measured voxel adds h |F - m|^2 / 2 (h its
+// hit count, m the averaged measurement) to the workgroup's partial sum, written to dc.part[block].
+template <int V, int B, bool JT>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ W, const float2* __restrict__ tw,
                                                            DcList dc, float tau)
 {
@@ -608,6 +631,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
     constexpr int LS = LdsStride<V, B>::value;
     __shared__ float2 lds[B * LS];
     __shared__ float2 twl[LdsSize<V>::value];
+    __shared__ double red[JT ? B * T : 1];
     stage_twiddles<V>(twl, tw);
     const TwPad twp{twl};
     // list prefetch: lanes walk the entries of one pencil (contiguous in the list)
@@ -615,7 +639,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
     const int blk = Tile<V, 2, B>::block();
     const long long dpencil = (long long)(blk % (V / B)) * B + pb + (long long)V * (blk / (V / B));
     const int n = min(dc.cnt[dpencil], dc.cap);  // a count never exceeds the list capacity
-    short pz[DC_PRE];
+    short pz[DC_PRE], ph[DC_PRE];
     float2 pv[DC_PRE];
 #pragma unroll
     for (int r = 0; r < DC_PRE; ++r) {
@@ -624,7 +648,9 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
         const long long e = dpencil * dc.cap + (in ? k : 0);
         pz[r] = dc.idx[e];
         pv[r] = dc.val[e];
+        if constexpr (JT) ph[r] = dc.hit[e];
     }
+    double jacc = 0.0;
     Tile<V, 2, B> tl;
     float2* s = lds + tl.b * LS;
     float2* pen = W + tl.base;
@@ -635,10 +661,14 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
     {
         float2* sp = lds + pb * LS;
         const bool relax = tau != 1.f;
-        auto put = [&](int z, float2 m) {  // each z appears once in a pencil's list
-            if (relax) {
+        auto put = [&](int z, float2 m, int h) {  // each z appears once in a pencil's list
+            if (relax || JT) {
                 const float2 F = sp[lds_pad(z)];
-                m = make_float2(F.x + tau * (m.x - F.x), F.y + tau * (m.y - F.y));
+                if constexpr (JT) {
+                    const double dr = (double)F.x - m.x, di = (double)F.y - m.y;
+                    jacc += 0.5 * h * (dr * dr + di * di);
+                }
+                if (relax) m = make_float2(F.x + tau * (m.x - F.x), F.y + tau * (m.y - F.y));
             }
             sp[lds_pad(z)] = m;
         };
@@ -646,13 +676,17 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
         for (int r = 0; r < DC_PRE; ++r) {
             const int k = k0 + r * T;
             const int z = pz[r];
-            if (k < n && z >= 0 && z < V) put(z, pv[r]);
+            if (k < n && z >= 0 && z < V) put(z, pv[r], JT ? ph[r] : 0);
         }
         for (int k = k0 + DC_PRE * T; k < n; k += T) {
             const long long e = dpencil * dc.cap + k;
             const int z = dc.idx[e];
-            if (z >= 0 && z < V) put(z, dc.val[e]);
+            if (z >= 0 && z < V) put(z, dc.val[e], JT ? dc.hit[e] : 0);
         }
+    }
+    if constexpr (JT) {
+        const double tot = workgroup_sum<B * T>(jacc, red);  // its barriers also order the LDS updates
+        if (threadIdx.x == 0) dc.part[blockIdx.x] = tot;
     }
     __syncthreads();
     auto gl2 = [&](int k) -> float2 { return s[lds_pad(k)]; };
@@ -722,6 +756,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
     __shared__ short lnz[B][GRID_CHUNK];
     __shared__ float2 lval[B][GRID_CHUNK + 1];
     __shared__ unsigned lmask[B][T];  // owner masks of the staged chunk
+    __shared__ double red[VOLUME ? 1 : B * T];  // the fit's objective constant (list only)
     Tile<V, 2, B> tl;
     const long long VV = (long long)V * V;
     const long long pencil = tl.base;  // x + V*y
@@ -776,6 +811,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
     unsigned hc[HW];
 #pragma unroll
     for (int w = 0; w < HW; ++w) hc[w] = 0;
+    double sq = 0.0;  // list only: sum of |sample|^2 over this thread's entries
     lmask[tl.b][tl.t] = 0;
     __syncthreads();  // cleared masks before any mark (the loads stay in flight across the barrier)
     for (int a0 = 0; a0 < A; a0 += GRID_CHUNK) {
@@ -793,6 +829,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
             const float2 v = lval[tl.b][k];
             acc.x += v.x;
             acc.y += v.y;
+            if constexpr (!VOLUME) sq += (double)v.x * v.x + (double)v.y * v.y;
             const int j = nz / T;
 #pragma unroll
             for (int w = 0; w < HW; ++w)
@@ -818,6 +855,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
     __syncthreads();
     int off = 0;
     for (int k = 0; k < tl.t; ++k) off += (int)lmask[tl.b][k];
+    double hm2 = 0.0;  // list only: sum of h |m|^2 over this thread's measured voxels
 #pragma unroll
     for (int j = 0; j < ZR; ++j) {
         const int z = tl.t + j * T;
@@ -826,11 +864,23 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_grid(const float2* __rest
         if (h && !(refcompat && !(f.x != 0.f && f.y != 0.f))) {
             dc.idx[pencil * dc.cap + off] = (short)z;
             dc.val[pencil * dc.cap + off] = f;
+            if constexpr (!VOLUME) {
+                if (dc.hit) dc.hit[pencil * dc.cap + off] = (short)h;
+                hm2 += h * ((double)f.x * f.x + (double)f.y * f.y);
+            }
             ++off;
         }
     }
     if (tl.t == T - 1) dc.cnt[pencil] = off;
-    if constexpr (!VOLUME) return;  // list only (the fit's measurement list; W is not touched)
+    if constexpr (!VOLUME) {  // list only (the fit's measurement list; W is not touched)
+        if (dc.part) {
+            // objective constant: sum over voxels of (sum |m|^2 - h |m-bar|^2) / 2, m = sample / dx
+            const double idx2 = (double)inv_dx * inv_dx;
+            const double tot = workgroup_sum<B * T>(0.5 * (sq * idx2 - hm2), red);
+            if (threadIdx.x == 0) dc.part[blockIdx.x] = tot;
+        }
+        return;
+    }
     float2* pen = W + tl.base;
     auto gl = [&](int k) -> float2 { return s[lds_pad(k)]; };
     auto gs = [&](int z, float2 v) { pen[VV * z] = v; };
@@ -900,7 +950,10 @@ struct VolOps {
     static void zdc(Ctx& c, float2* W, const DcList& dc, float tau)
     {
         const float2* tw = c.tw(V);
-        vlaunch(c, &k_vol_zdc<V, B>, grid(), dim3(NT), W, tw, dc, tau);
+        if (dc.part && dc.hit)
+            vlaunch(c, &k_vol_zdc<V, B, true>, grid(), dim3(NT), W, tw, dc, tau);
+        else
+            vlaunch(c, &k_vol_zdc<V, B, false>, grid(), dim3(NT), W, tw, dc, tau);
         CTDM_CHECK_LAUNCH("k_vol_zdc");
     }
     // W == nullptr: the list only.
@@ -947,8 +1000,10 @@ struct Vol {
     // Loop after the initial inverse transform of the measured spectrum is in W up to the x pass:
     // `first_x_mode` 0 if W holds real-space f, 1 if W still needs its inverse x-FFT. The z step is
     // the data-consistency replacement (tau = 1) or the fit's relaxed step F <- F + tau (m - F).
+    // `jpart` (the fit with its objective): iteration it's z pass writes its workgroup partial sums to
+    // jpart + it * jstride.
     static void loop(Ctx& c, float2* W, float2* n_out, const DcList& dc, const Proj& p, int iters, bool shift,
-                     int first_x_mode, float tau = 1.f)
+                     int first_x_mode, float tau = 1.f, double* jpart = nullptr, int jstride = 0)
     {
         const float s = (float)(1.0 / ((double)V * V * V));
         if (iters == 0) {
@@ -961,7 +1016,9 @@ struct Vol {
         x(c, first_x_mode, W, n_out, p, s, shift);
         for (int it = 0; it < iters; ++it) {
             y(c, W, -1);
-            zdc(c, W, dc, tau);
+            DcList d = dc;
+            d.part = jpart ? jpart + (size_t)it * jstride : nullptr;
+            zdc(c, W, d, tau);
             y(c, W, +1);
             x(c, it + 1 < iters ? 1 : 2, W, n_out, p, s, shift);
         }
@@ -1034,17 +1091,56 @@ void volume_batched(Ctx& c, const float2* spec, const double* rows, int A, int V
 // measurement. m-bar is formed once by the gridding kernel (list only); each iteration then recomputes
 // the 3-D FFT and runs the relaxed data-consistency pass, then positivity (est_n) in real space.
 // tau = 1 is the reference's loop (check_F3D replacement).
+// out[r] = base[0] (if given) + the sum of part[r * n .. r * n + n), one workgroup per row r, in a fixed
+// order (deterministic).
+__global__ __launch_bounds__(256) void k_sum_rows(const double* __restrict__ part, int n, const double* base,
+                                                  double* __restrict__ out)
+{
+    __shared__ double red[256];
+    const double* row = part + (size_t)blockIdx.x * n;
+    double t = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) t += row[i];
+    const double tot = workgroup_sum<256>(t, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = tot + (base ? base[0] : 0.0);
+}
+
+// objective (host, may be null): J before each step, J_k = 1/2 sum_a sum_p |FFT3(P f_k)[v_a(p)] - m_a(p)|^2
+// with P the positivity projection. Per voxel the samples' residuals sum to h |F - m-bar|^2 + (sum |m|^2 -
+// h |m-bar|^2), so the z pass accumulates the first term and the gridding kernel the constant second one.
 template <int V>
 static void fit_pgd_v(Ctx& c, float2* f, const float2* spec, const double* rows, int A, const EwaldGeom& g,
-                      int iters, float tau, float2* n, unsigned flags)
+                      int iters, float tau, float2* n, unsigned flags, double* objective)
 {
-    const DcList ml = list_alloc(c, "pgd", V, std::max(A, 1));
-    Vol<V>::grid_pass(c, spec, rows, A, g, false, nullptr, ml);  // m-bar = H^-1 G^T m per measured voxel
-    Vol<V>::loop(c, f, n, ml, make_proj(g), iters, !(flags & CTDM_POS_NO_SHIFT), 0, tau);
+    DcList ml = list_alloc(c, "pgd", V, std::max(A, 1));
+    double* jpart = nullptr;
+    double* jdev = nullptr;
+    int jstride = 0;
+    if (objective && iters > 0) {
+        const int gblk = V * V / Vol<V>::pick(c, PASS_GRID), zblk = V * V / Vol<V>::pick(c, PASS_ZDC);
+        jstride = zblk;
+        ml.hit = (short*)c.buf("pgd.hit", sizeof(short) * (size_t)V * V * ml.cap);
+        ml.part = (double*)c.buf("pgd.cpart", sizeof(double) * gblk);
+        jpart = (double*)c.buf("pgd.jpart", sizeof(double) * (size_t)iters * zblk);
+        jdev = (double*)c.buf("pgd.J", sizeof(double) * (iters + 1));
+        Vol<V>::grid_pass(c, spec, rows, A, g, false, nullptr, ml);
+        hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, c.stream, ml.part, gblk, (const double*)nullptr, jdev);
+        CTDM_CHECK_LAUNCH("k_sum_rows");
+        ml.part = nullptr;
+    } else {
+        Vol<V>::grid_pass(c, spec, rows, A, g, false, nullptr, ml);  // m-bar = H^-1 G^T m per measured voxel
+    }
+    Vol<V>::loop(c, f, n, ml, make_proj(g), iters, !(flags & CTDM_POS_NO_SHIFT), 0, tau, jpart, jstride);
+    if (jpart) {
+        hipLaunchKernelGGL(k_sum_rows, dim3(iters), dim3(256), 0, c.stream, jpart, jstride, (const double*)jdev,
+                           jdev + 1);
+        CTDM_CHECK_LAUNCH("k_sum_rows");
+        CTDM_HIP(hipMemcpyAsync(objective, jdev + 1, sizeof(double) * iters, hipMemcpyDeviceToHost, c.stream));
+        CTDM_HIP(hipStreamSynchronize(c.stream));
+    }
 }
 
 void fit_pgd(Ctx& c, float2* f, const float2* spec, const ctdm_angle* ang, int A, int V, const ctdm_optics& o,
-             int iters, double step, float2* n, unsigned flags)
+             int iters, double step, float2* n, unsigned flags, double* objective)
 {
     CTDM_REQUIRE(f && spec && n && f != n, "fit_pgd: bad pointers");
     CTDM_REQUIRE(A > 0 && A < 32768 && iters >= 0, "fit_pgd: bad sizes");
@@ -1052,11 +1148,11 @@ void fit_pgd(Ctx& c, float2* f, const float2* spec, const ctdm_angle* ang, int A
     const EwaldGeom g = ewald_geom(o, V);
     const double* rows = upload_rows(c, "pgd.rows", angle_rows(g, ang, A));
     switch (V) {
-    case 32: fit_pgd_v<32>(c, f, spec, rows, A, g, iters, (float)step, n, flags); break;
-    case 64: fit_pgd_v<64>(c, f, spec, rows, A, g, iters, (float)step, n, flags); break;
-    case 128: fit_pgd_v<128>(c, f, spec, rows, A, g, iters, (float)step, n, flags); break;
-    case 192: fit_pgd_v<192>(c, f, spec, rows, A, g, iters, (float)step, n, flags); break;
-    case 256: fit_pgd_v<256>(c, f, spec, rows, A, g, iters, (float)step, n, flags); break;
+    case 32: fit_pgd_v<32>(c, f, spec, rows, A, g, iters, (float)step, n, flags, objective); break;
+    case 64: fit_pgd_v<64>(c, f, spec, rows, A, g, iters, (float)step, n, flags, objective); break;
+    case 128: fit_pgd_v<128>(c, f, spec, rows, A, g, iters, (float)step, n, flags, objective); break;
+    case 192: fit_pgd_v<192>(c, f, spec, rows, A, g, iters, (float)step, n, flags, objective); break;
+    case 256: fit_pgd_v<256>(c, f, spec, rows, A, g, iters, (float)step, n, flags, objective); break;
     default: fail(CTDM_EUNSUPPORTED, "fit_pgd: volume side has no kernel");
     }
 }
@@ -1228,12 +1324,52 @@ void time_loop(Ctx& c, int V, int iters, double ms[3], double bytes[3])
     }
 }
 
+// The y pass (8 pencils per workgroup) and the copy yardstick at V = 512: a 1 GiB volume, four times the
+// 256 MB last-level cache, so these rates are HBM rates (at 256^3 the 128 MiB volume can stay cached
+// between passes). Only for timing: no reconstruction runs at this size.
+static double time_pass_512(Ctx& c, int kind, int reps, double* bytes)
+{
+    constexpr int V = 512;
+    const size_t vol = (size_t)V * V * V;
+    float2* W = (float2*)c.buf("tp.W512", sizeof(float2) * vol);
+    float2* dst = kind == PASS_COPY ? (float2*)c.buf("tp.n512", sizeof(float2) * vol) : nullptr;
+    fill(c, W, 2 * (long long)vol, 1.f);
+    *bytes = 2.0 * sizeof(float2) * vol;
+    const long long n4 = (long long)vol / 2;
+    const int copy_grid = (int)std::min<long long>(grid1d(n4, 256), 8192);
+    auto launch = [&](int r) {
+        if (kind == PASS_COPY) {
+            hipLaunchKernelGGL(k_copy_vol, dim3(copy_grid), dim3(256), 0, c.stream, (const float4*)W, (float4*)dst, n4);
+            CTDM_CHECK_LAUNCH("k_copy_vol");
+        } else {
+            VolOps<V, 8>::y(c, W, (r & 1) ? 1 : -1);
+        }
+    };
+    launch(0);
+    hipEvent_t e0, e1;
+    CTDM_HIP(hipEventCreate(&e0));
+    CTDM_HIP(hipEventCreate(&e1));
+    CTDM_HIP(hipEventRecord(e0, c.stream));
+    for (int r = 0; r < reps; ++r) launch(r + 1);
+    CTDM_HIP(hipEventRecord(e1, c.stream));
+    CTDM_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    CTDM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return ms / reps;
+}
+
 double time_pass(Ctx& c, int kind, int V, int B, int reps, double* bytes)
 {
     CTDM_REQUIRE(!c.capturing, "time_pass during capture");
     CTDM_REQUIRE(reps > 0 && (B == 8 || B == 16), "time_pass: bad arguments");
     CTDM_REQUIRE(kind >= 0 && kind <= PASS_COPY && kind != PASS_GRID,
                  "time_pass: pass must be 0 (x), 1 (y), 2 (z+dc) or 4 (plain copy)");
+    if (V == 512) {
+        CTDM_REQUIRE(kind == PASS_Y || kind == PASS_COPY, "time_pass: at V = 512 only the y pass and the copy");
+        return time_pass_512(c, kind, reps, bytes);
+    }
     switch (V) {
     case 32: return time_pass_v<32>(c, kind, B, reps, bytes);
     case 64: return time_pass_v<64>(c, kind, B, reps, bytes);

@@ -30,7 +30,8 @@
 namespace ctdm {
 
 enum : unsigned { F_SINCOS = CTDM_UNWRAP_SINCOS, F_REF_EIG = CTDM_UNWRAP_REFCOMPAT_EIG,
-                  F_REF_IDCT = CTDM_UNWRAP_REFCOMPAT_IDCT, F_CONGRUENT = CTDM_UNWRAP_CONGRUENT };
+                  F_REF_IDCT = CTDM_UNWRAP_REFCOMPAT_IDCT, F_CONGRUENT = CTDM_UNWRAP_CONGRUENT,
+                  F_REF_FAST = CTDM_UNWRAP_REFCOMPAT_FAST, F_ALL = 0x1fu };
 
 static constexpr float kPI = 3.14159265358979323846f;
 
@@ -275,6 +276,55 @@ __global__ void k_uw_stencil(const float* __restrict__ v, const float* __restric
     out[g] = acc;
 }
 
+// ---------------------------------------------------------------- FastUnwrapping's map conventions
+// One workgroup per map: minimum (and maximum) of its n values, left in mn[0] (mx[0]) for every thread.
+template <int T>
+__device__ __forceinline__ void map_min_max(const float* __restrict__ m, int n, float* red, float& mn, float& mx)
+{
+    float lo = INFINITY, hi = -INFINITY;
+    for (int i = threadIdx.x; i < n; i += T) {
+        const float v = m[i];
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+    }
+    red[threadIdx.x] = lo;
+    red[T + threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = T / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[threadIdx.x] = fminf(red[threadIdx.x], red[threadIdx.x + w]);
+            red[T + threadIdx.x] = fmaxf(red[T + threadIdx.x], red[T + threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    mn = red[0];
+    mx = red[T];
+}
+
+// PhaseNormalize + RescalePhaseMap (FastUnwrapping.cu:119-142): each map rescaled to
+// 2 pi (x - min) / (max - min), evaluated in double as the reference's double-precision M_PI promotes it.
+__global__ __launch_bounds__(256) void k_uw_normalize(const float* __restrict__ in, float* __restrict__ out, int n)
+{
+    __shared__ float red[512];
+    const float* m = in + (size_t)blockIdx.x * n;
+    float* o = out + (size_t)blockIdx.x * n;
+    float mn, mx;
+    map_min_max<256>(m, n, red, mn, mx);
+    const double span = (double)(mx - mn);
+    for (int i = threadIdx.x; i < n; i += 256) o[i] = (float)(2.0 * M_PI * (double)(m[i] - mn) / span);
+}
+
+// subtractMin + PhaseMapSubMin (FastUnwrapping.cu:346-369): each map minus its own minimum, in place.
+__global__ __launch_bounds__(256) void k_uw_sub_min(float* __restrict__ x, int n)
+{
+    __shared__ float red[512];
+    float* m = x + (size_t)blockIdx.x * n;
+    float mn, mx;
+    map_min_max<256>(m, n, red, mn, mx);
+    __syncthreads();  // every thread has read the minimum before any value changes
+    for (int i = threadIdx.x; i < n; i += 256) m[i] -= mn;
+}
+
 // ---------------------------------------------------------------- host side
 static const double* eig_table(Ctx& c, int N, bool refcompat, bool existing = false)
 {
@@ -426,14 +476,36 @@ void unwrap_dct_noalloc(Ctx& c, const float* wrapped, float* unwrapped, int batc
     solve_any(c, wrapped, batch, M, flags, 0, wrapped, unwrapped, w);
 }
 
+// With CTDM_UNWRAP_REFCOMPAT_FAST the maps are first rescaled to [0, 2 pi] (into scratch `name`) and
+// the result of each map is shifted to a zero minimum, FastUnwrapping's conventions.
+static const float* normalized_input(Ctx& c, const float* wrapped, int batch, int M, const char* name)
+{
+    float* norm = (float*)c.buf(name, sizeof(float) * (size_t)batch * M * M);
+    hipLaunchKernelGGL(k_uw_normalize, dim3(batch), dim3(256), 0, c.stream, wrapped, norm, M * M);
+    CTDM_CHECK_LAUNCH("k_uw_normalize");
+    return norm;
+}
+
+static void subtract_min(Ctx& c, float* out, int batch, int M)
+{
+    hipLaunchKernelGGL(k_uw_sub_min, dim3(batch), dim3(256), 0, c.stream, out, M * M);
+    CTDM_CHECK_LAUNCH("k_uw_sub_min");
+}
+
 void unwrap_dct_batched(Ctx& c, const float* wrapped, float* unwrapped, int batch, int M,
                         unsigned flags)
 {
     CTDM_REQUIRE(wrapped && unwrapped, "unwrap_dct: null pointer");
     CTDM_REQUIRE(batch >= 0 && M > 0, "unwrap_dct: bad size");
     CTDM_REQUIRE(wrapped != unwrapped, "unwrap_dct: in-place call not supported");
-    CTDM_REQUIRE((flags & ~0xfu) == 0, "unwrap_dct: unknown flag");
+    CTDM_REQUIRE((flags & ~F_ALL) == 0, "unwrap_dct: unknown flag");
     if (batch == 0) return;
+    if (flags & F_REF_FAST) {
+        const float* norm = normalized_input(c, wrapped, batch, M, "uw.norm");
+        solve_any(c, norm, batch, M, flags & ~F_REF_FAST, 0, norm, unwrapped);
+        subtract_min(c, unwrapped, batch, M);
+        return;
+    }
     solve_any(c, wrapped, batch, M, flags, 0, wrapped, unwrapped);
 }
 
@@ -441,7 +513,8 @@ void unwrap_dct_adjoint(Ctx& c, const float* wrapped, const float* g_out, float*
                         int M, unsigned flags)
 {
     CTDM_REQUIRE(wrapped && g_out && g_in, "unwrap_dct_adjoint: null pointer");
-    CTDM_REQUIRE(!(flags & F_REF_IDCT), "unwrap_dct_adjoint: the reference IDCT has no adjoint here");
+    CTDM_REQUIRE(!(flags & (F_REF_IDCT | F_REF_FAST)),
+                 "unwrap_dct_adjoint: the reference IDCT and FastUnwrapping's rescaling have no adjoint here");
     if (batch == 0) return;
     const size_t n = (size_t)batch * M * M;
     if (flags & F_CONGRUENT) {  // output = wrapped + 2 pi k with k piecewise constant: J = I
@@ -578,6 +651,12 @@ __global__ void k_congruent(float* __restrict__ o, const float* __restrict__ w, 
 void unwrap_dct_plain_one(Ctx& c, const float* wrapped, float* out, int N, unsigned flags)
 {
     CTDM_REQUIRE(N == 32 || N == 64 || N == 128 || N == 192 || N == 256, "plain unwrap: N in {32, 64, 128, 192, 256}");
+    if (flags & F_REF_FAST) {
+        const float* norm = normalized_input(c, wrapped, 1, N, "puw.norm");
+        unwrap_dct_plain_one(c, norm, out, N, flags & ~F_REF_FAST);
+        subtract_min(c, out, 1, N);
+        return;
+    }
     const size_t n = (size_t)N * N;
     float* inX = (float*)c.buf("puw.inX", sizeof(float) * n);
     float* inY = (float*)c.buf("puw.inY", sizeof(float) * n);

@@ -73,6 +73,9 @@ typedef struct ctdm_angle { double ang_x, ang_y; } ctdm_angle;
 #define CTDM_UNWRAP_REFCOMPAT_EIG   0x2u  /* eigenvalues 2cos(4*pi*i/(4N-1)) + 2cos(4*pi*j/(4N-1)) - 4 (DCT_Unwrapping.cu:310) */
 #define CTDM_UNWRAP_REFCOMPAT_IDCT  0x4u  /* (i+1) twiddle of the reference IDCT (DCT_Unwrapping.cu:230-231) */
 #define CTDM_UNWRAP_CONGRUENT       0x8u  /* phi_w + 2*pi*round((phi_ls - phi_w)/2*pi) (FastUnwrapping.cu:333-344) */
+#define CTDM_UNWRAP_REFCOMPAT_FAST  0x10u /* FastUnwrapping's conventions (FastUnwrapping.cu:7,108,119-142,346-369):
+                                              each input map rescaled to 2*pi*(x - min)/(max - min) first
+                                              (PhaseNormalize), each output map minus its minimum (subtractMin) */
 #define CTDM_UNWRAP_REFCOMPAT       (CTDM_UNWRAP_REFCOMPAT_EIG | CTDM_UNWRAP_REFCOMPAT_IDCT)
 
 /* ---- reconstruction flags ---- */
@@ -158,7 +161,8 @@ ctdm_status ctdm_unwrap_dct(ctdm_ctx *ctx, const float *wrapped, float *unwrappe
  * allocates nothing. batch * M * M above CTDM_NOALLOC_MAX_ELEMS is rejected with CTDM_EINVAL. All calls
  * on one GPU share the array (it keeps the last call's contents, which every call overwrites before
  * reading): calls that overlap on one GPU, from two streams or two contexts, race. Same flags, kernels
- * and results as ctdm_unwrap_dct (CTDM_EXEC_PLAIN is not accepted). */
+ * and results as ctdm_unwrap_dct (CTDM_EXEC_PLAIN and CTDM_UNWRAP_REFCOMPAT_FAST, which needs a
+ * third buffer, are not accepted). */
 #define CTDM_NOALLOC_MAX_ELEMS (1u << 22)
 ctdm_status ctdm_unwrap_dct_noalloc(ctdm_ctx *ctx, const float *wrapped, float *unwrapped, int batch, int M,
                                     unsigned flags);
@@ -347,6 +351,15 @@ ctdm_status ctdm_rimap_read(const char *path, ctdm_complex *ri, size_t n, int pa
 ctdm_status ctdm_fit_pgd(ctdm_ctx *ctx, ctdm_complex *f, const ctdm_complex *spec,
                          const ctdm_angle *angles, int A, int V, const ctdm_optics *optics,
                          int iters, double step, ctdm_complex *n, unsigned flags);
+
+/* ctdm_fit_pgd that also reports its objective (no reference counterpart): objective (HOST, iters values)
+ * receives J before each step, J_k = 1/2 sum_a sum_p |FFT3(P f_k)[v_a(p)] - m_a(p)|^2 with P f_k the
+ * positivity-projected iterate the step starts from (f_0 the given start). The z pass of each step sums the
+ * measured voxels' h |F - m-bar|^2 / 2 and the gridding kernel the constant sum of (sum |m|^2 - h |m-bar|^2) / 2
+ * (h: hits of the voxel), which together equal J exactly. Same result n as ctdm_fit_pgd. Synchronises. */
+ctdm_status ctdm_fit_pgd_trace(ctdm_ctx *ctx, ctdm_complex *f, const ctdm_complex *spec,
+                               const ctdm_angle *angles, int A, int V, const ctdm_optics *optics,
+                               int iters, double step, ctdm_complex *n, unsigned flags, double *objective);
 
 /* ---------------------------------------------------------------- synthetic inputs (host) */
 /* Seeded synthetic phantoms and holograms (host memory; no reference counterpart, the

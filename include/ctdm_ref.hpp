@@ -87,12 +87,16 @@ inline void DCT_UWLS_Unwrapped(const Context& c, float* ImgDst, const float* Img
     c.check(ctdm_unwrap_dct(c.get(), ImgSrc, ImgDst, 1, sizeX, flags));
 }
 
-// FastUnwrapping(float *ImgSrc, float *ImgDst, int sizeX, int sizeY) (FastUnwrapping.cu:5):
-// sin/cos-Laplacian right-hand side, Poisson solve, congruence rounding.
+// FastUnwrapping(float *ImgSrc, float *ImgDst, int sizeX, int sizeY) (FastUnwrapping.cu:5): the map
+// rescaled to [0, 2 pi] (PhaseNormalize, :7), sin/cos-Laplacian right-hand side, Poisson solve,
+// congruence rounding (ObtainUnwrap), minimum subtracted from the result (subtractMin, :108). The
+// reference rescales ImgSrc itself in place; here the input is left untouched and the rescaled copy
+// lives in the context's scratch.
 inline void FastUnwrapping(const Context& c, const float* ImgSrc, float* ImgDst, int sizeX, int sizeY)
 {
     if (sizeX != sizeY) throw Error(CTDM_EINVAL, "FastUnwrapping: square maps only");
-    c.check(ctdm_unwrap_dct(c.get(), ImgSrc, ImgDst, 1, sizeX, CTDM_UNWRAP_SINCOS | CTDM_UNWRAP_CONGRUENT));
+    c.check(ctdm_unwrap_dct(c.get(), ImgSrc, ImgDst, 1, sizeX,
+                            CTDM_UNWRAP_SINCOS | CTDM_UNWRAP_CONGRUENT | CTDM_UNWRAP_REFCOMPAT_FAST));
 }
 
 // cudaGoldsteinUnwrap2D(float *cu_phi, float *cu_UnWrapPhase, int sizeX, int sizeY, int frameNumber)

@@ -66,7 +66,7 @@ _EXTRA: dict = {
     "orc_flood_fill": [_fp, C.c_void_p, C.c_int, C.c_int, _ip, _fp],
     "orc_goldstein": [_fp, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, _ip, _fp],
     "orc_fit_pgd": [_dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
-                    C.c_double, _dp, C.c_uint],
+                    C.c_double, _dp, C.c_uint, _dp],
     "orc_find_sideband_rows": [_fp, C.c_int, C.c_int],
     "orc_retrieve_rows": [_fp, _fp, C.c_int, C.c_int, C.c_int, _dp, _dp],
     "orc_median3x3": [_fp, _fp, C.c_int, C.c_int, C.c_int],
@@ -279,14 +279,16 @@ def goldstein(phi):
     return res, cut, k, un
 
 
-def fit_pgd(f0, spec, angles, wavelength, n_med, dx, iters, step, flags=0):
+def fit_pgd(f0, spec, angles, wavelength, n_med, dx, iters, step, flags=0, objective=False):
+    """n (and, with objective=True, J before each step as an array of iters values)."""
     f = _c(f0)
     sp = _c(spec)
     a = _ang(angles)
     n = np.zeros_like(f)
+    J = np.zeros(max(iters, 1))
     lib().orc_fit_pgd(_cp(f), _cp(sp), _p(a, _dp), len(a), f.shape[0], wavelength, n_med, dx, iters, step,
-                      _cp(n), flags)
-    return n
+                      _cp(n), flags, _p(J, _dp) if objective else None)
+    return (n, J[:iters]) if objective else n
 
 
 def median3x3(maps):

@@ -636,13 +636,28 @@ void orc_dct_chain(const double* in, int N, unsigned flags, int stage_in, double
     if (out) std::memcpy(out, U.data(), n * sizeof(double));
 }
 
-// Batched convenience: unwrap `batch` maps.
+// Batched convenience: unwrap `batch` maps. Flag 0x10 (FastUnwrapping's conventions): each input map
+// is first rescaled to 2 pi (x - min) / (max - min) (PhaseNormalize / RescalePhaseMap,
+// FastUnwrapping.cu:7, 119-142; the reference rounds the rescaled map to float) and each result map
+// then has its minimum subtracted (subtractMin / PhaseMapSubMin, :108, 346-369).
 void orc_unwrap_dct(const double* in, double* out, int batch, int N, unsigned flags)
 {
+    const size_t n = (size_t)N * N;
 #pragma omp parallel for schedule(dynamic)
-    for (int b = 0; b < batch; ++b)
-        orc_dct_chain(in + (size_t)b * N * N, N, flags, 0, nullptr, nullptr, nullptr,
-                      out + (size_t)b * N * N);
+    for (int b = 0; b < batch; ++b) {
+        const double* src = in + b * n;
+        double* dst = out + b * n;
+        if (!(flags & 0x10u)) {
+            orc_dct_chain(src, N, flags, 0, nullptr, nullptr, nullptr, dst);
+            continue;
+        }
+        const double mn = *std::min_element(src, src + n), mx = *std::max_element(src, src + n);
+        std::vector<double> norm(n);
+        for (size_t k = 0; k < n; ++k) norm[k] = (float)(2.0 * PI * (src[k] - mn) / (mx - mn));
+        orc_dct_chain(norm.data(), N, flags & ~0x10u, 0, nullptr, nullptr, nullptr, dst);
+        const double lo = *std::min_element(dst, dst + n);
+        for (size_t k = 0; k < n; ++k) dst[k] -= lo;
+    }
 }
 
 }  // extern "C"
@@ -1228,8 +1243,10 @@ extern "C" {
 // Recon_kernel.cu:224-225, before the 1/dx of Initial_F_F2, :269-270): starting from n = est_n(f),
 // each iteration is f = modify(n); F = FFT3(f); R = G^T (G F - m); F -= step * R / hits;
 // f = iFFT3(F); n = est_n(f). step = 1 reproduces orc_positivity with the averaged measurement.
+// J (may be NULL): iters values, J before each step, 1/2 sum over samples of |F[v] - m|^2 evaluated
+// sample by sample from the gathered F.
 void orc_fit_pgd(const double* f0, const double* spec, const double* angles, int A, int V, double wavelength,
-                 double n_med, double dx, int iters, double step, double* n, unsigned flags)
+                 double n_med, double dx, int iters, double step, double* n, unsigned flags, double* J)
 {
     const Geom g = geom(wavelength, n_med, dx, V);
     const long VV = (long)V * V, cnt = VV * V;
@@ -1263,7 +1280,13 @@ void orc_fit_pgd(const double* f0, const double* spec, const double* angles, int
             R[i] = 0.0;
         }
         fft3(f.data(), V, -1);
-        for (size_t e = 0; e < ev.size(); ++e) R[ev[e]] += f[ev[e]] - em[e];
+        double jk = 0.0;
+        for (size_t e = 0; e < ev.size(); ++e) {
+            const cd r = f[ev[e]] - em[e];  // gathered sample minus measurement
+            R[ev[e]] += r;
+            jk += 0.5 * std::norm(r);
+        }
+        if (J) J[it] = jk;
 #pragma omp parallel for schedule(static)
         for (long i = 0; i < cnt; ++i)
             if (hits[i] > 0) f[i] -= step * R[i] / hits[i];

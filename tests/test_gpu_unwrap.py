@@ -115,3 +115,29 @@ def test_adjoint_dot_product(ctx, orc, flags):
     lhs, rhs = float(np.sum(Jv * g)), float(np.sum(v * JTg))
     print("dot", lhs, rhs)
     assert abs(lhs - rhs) <= 1e-4 * max(1.0, abs(lhs))
+
+
+@pytest.mark.parametrize("N", [128, 256])
+def test_fast_unwrapping_conventions_match_oracle(ctx, orc, N):
+    """CTDM_UNWRAP_REFCOMPAT_FAST (FastUnwrapping's min-max rescaling of the input and minimum subtracted
+    from the result) against the oracle, on full-range wrapped maps and on a map confined to [0, 0.9] rad
+    that the rescaling stretches; every output map has its minimum at exactly 0; the reference-named
+    wrapper and the plain path give the same maps."""
+    flags = ctdm_amd.UNWRAP_SINCOS | ctdm_amd.UNWRAP_CONGRUENT | ctdm_amd.UNWRAP_REFCOMPAT_FAST
+    y, x = np.mgrid[0:N, 0:N] / N
+    bump = (0.9 * np.exp(-((x - 0.5) ** 2 + (y - 0.4) ** 2) / (2 * 0.1 ** 2)))[None].astype(np.float32)
+    w = np.concatenate([_maps(N, 2, 3 * N), bump])
+    wd = torch.from_numpy(w).cuda()
+    got = ctx.unwrap_dct(wd, flags)
+    ref = orc.unwrap_dct(w.astype(np.float64), flags)
+    err = got.cpu().numpy() - ref
+    assert np.mean(np.abs(err) > 1e-3) < 1e-4  # integer snapping: compare modulo exact 2 pi steps
+    err = np.where(np.abs(err) > 1e-3, 0, err)
+    print(N, "rms", _rms(err), "max", np.abs(err).max())
+    assert _rms(err) <= 1e-4 and np.abs(err).max() <= 1e-3
+    assert torch.equal(got.amin(dim=(1, 2)), torch.zeros(3, device="cuda"))
+    assert torch.equal(ctx.FastUnwrapping(wd), got)
+    assert torch.equal(wd.cpu(), torch.from_numpy(w))  # the input is left as it was
+    plain = ctx.unwrap_dct(wd, flags | ctdm_amd.EXEC_PLAIN)
+    d = (plain - got).cpu().numpy()
+    assert _rms(np.where(np.abs(d) > 1e-3, 0, d)) <= 2e-5

@@ -118,6 +118,34 @@ def test_unwrap_least_squares_closed_form(orc, N):
     assert np.abs(d - np.round(d.mean() / (2 * np.pi)) * 2 * np.pi).max() < 1e-9
 
 
+@pytest.mark.parametrize("N", [64, 256])
+def test_fast_unwrapping_conventions_closed_form(orc, N):
+    """FastUnwrapping's conventions (flag 0x10; FastUnwrapping.cu:7, 108, 119-142, 346-369). A wrapped map
+    spanning the whole (-pi, pi] range is only shifted by the min-max rescaling, so the result is the
+    true phase minus its minimum; a map confined to [0, 0.9] rad is stretched to [0, 2 pi] first, so the
+    result is the stretched map minus its minimum, not x. (The congruence step rounds
+    (LS - x) / 2 pi, about -mean(x) / 2 pi since the LS solution has zero mean, so a rescaled map whose
+    mean is near pi gets 2 pi jumps in the reference as well; the bump below keeps its mean near 0.4.)"""
+    flags = orc.SINCOS | orc.CONGRUENT | 0x10
+    phi = _smooth_phase(N, N)
+    w = np.angle(np.exp(1j * phi))
+    span = w.max() - w.min()
+    assert span > 2 * np.pi - 1e-2
+    u = orc.unwrap_dct(w, flags)[0]
+    assert u.min() == 0.0
+    # the rescaled map n = 2 pi (w - min w) / span jumps by 2 pi * 2 pi / span at each wrap; congruence
+    # adds the true wrap counts m back in whole 2 pi steps: u = n + 2 pi m, shifted to a zero minimum
+    m = np.round((phi - w) / (2 * np.pi))
+    n = (2 * np.pi * (w - w.min()) / span).astype(np.float32).astype(np.float64)
+    want = n + 2 * np.pi * m
+    np.testing.assert_allclose(u, want - want.min(), atol=1e-9)
+    y, x = np.mgrid[0:N, 0:N] / N
+    small = 0.9 * np.exp(-((x - 0.5) ** 2 + (y - 0.4) ** 2) / (2 * 0.1 ** 2))  # within [0, 0.9], no wrap
+    us = orc.unwrap_dct(small, flags)[0]
+    stretched = 2 * np.pi * (small - small.min()) / np.ptp(small)
+    np.testing.assert_allclose(us, stretched - stretched.min(), atol=1e-5)
+
+
 def test_sincos_rhs_identity(orc):
     # cos(p) L[sin p] - sin(p) L[cos p] == sum over neighbours of sin(p_n - p)
     rng = np.random.default_rng(5)

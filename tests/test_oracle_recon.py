@@ -191,3 +191,40 @@ def test_recon_from_maps_amplitude_and_log_agree(orc):
     np.testing.assert_allclose(r_amp, r_log, rtol=0, atol=1e-6)
     r_med = orc.recon_from_maps(orc.median3x3(ph), orc.median3x3(la), ang, LAM, NM, dx, 2, flags=0)
     np.testing.assert_allclose(r_med, r_log, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("step", [1.0, 0.5])
+def test_fit_objective_is_the_gathered_residual(orc, step):
+    """J before the first step equals 1/2 sum over samples of |F0[v] - m|^2 computed here from numpy's
+    3-D FFT of the projected start, the cap voxel of every sample and m = 4 pi i f_z spec / dx
+    (Recon_kernel.cu:224-225, 269-270); on consistent data (spectra gathered from a feasible object)
+    J then falls at every step."""
+    V = 16
+    dx, ang, _, f0 = _fit_case(orc, V, seed=11)
+    rng = np.random.default_rng(12)
+    n_true = (NM + rng.uniform(0, 0.02, (V, V, V))) + 1j * rng.uniform(0, 0.002, (V, V, V))
+    F_true = np.fft.fftn(orc.modify_f(n_true, LAM, NM))
+    G = orc.ewald_gather(F_true, ang, LAM, NM, dx)  # conj(4 pi i f_z) F[v]
+    fm0, df = NM / LAM, 1.0 / (V * dx)
+    k = np.fft.fftfreq(V, 1.0 / V)
+    II, JJ = np.meshgrid(k, k, indexing="xy")  # spectrum [y][x]: x = ii, y = jj
+    idx = orc.ewald_index(ang, V, LAM, NM, dx)
+    spec = np.zeros_like(G)
+    w = np.zeros(G.shape)
+    for a, (ax, ay) in enumerate(ang):
+        fx = II * df + fm0 * np.sin(ay)
+        fy = JJ * df + fm0 * np.sin(ax)
+        fz = np.sqrt(np.maximum(fm0 ** 2 - fx ** 2 - fy ** 2, 0.0))
+        w[a] = 4 * np.pi * fz
+        on = idx[a] >= 0
+        spec[a][on] = G[a][on] * dx / (w[a][on] ** 2)  # so that 4 pi i f_z spec / dx = F_true[v]
+    n, J = orc.fit_pgd(f0, spec, ang, LAM, NM, dx, 8, step, objective=True)
+    F0 = np.fft.fftn(orc.modify_f(orc.est_n(f0, LAM, NM), LAM, NM))
+    j0 = 0.0
+    for a in range(len(ang)):
+        on = idx[a] >= 0
+        yy, xx = np.nonzero(on)
+        m = 1j * w[a][on] * spec[a][on] / dx
+        j0 += 0.5 * np.sum(np.abs(F0[idx[a][on], yy, xx] - m) ** 2)
+    assert abs(J[0] - j0) <= 1e-10 * j0
+    assert np.all(np.diff(J) < 0), J
