@@ -1,15 +1,19 @@
 """Benchmark: 256^3 RI volumes/s from 60 x 1024^2 hologram pairs on one MI355X.
 
 Workload (one "step"): one cell of 60 illumination angles, each a 1024^2 float32 sample hologram
-plus background (seeded synthetic phantom, inputs resident in HBM before timing), sideband retrieval
-to 256^2, DCT least-squares unwrapping, Rytov spectra, Ewald-cap gridding into a 256^3 complex64
-k-space, inverse 3-D FFT and 10 positivity iterations -> one 256^3 complex RI volume.
+plus background (seeded synthetic phantom imaged by first-order Rytov fields, inputs resident in HBM
+before timing), sideband retrieval to 256^2, DCT least-squares unwrapping, Rytov spectra, Ewald-cap
+gridding into a 256^3 complex64 k-space, inverse 3-D FFT and 10 positivity iterations -> one 256^3
+complex RI volume.
 
 value       = batched (fused, hipGraph-replayed) volumes/s
 plain_value = the reference-structured plain pipeline on the same GPU (per angle, unfused,
               host-synchronous), reported beside it with the ratio
-roofline    = the dominant kernel (largest share of the step) at its algorithmic bytes per launch
-cpu_baseline= the FP64 host oracle on one whole volume of the same workload
+roofline    = the dominant kernel (largest share of the step) at its algorithmic bytes per launch;
+              traffic = its HBM bytes per launch from the committed counter profile
+cpu_baseline= the FP64 host oracle on whole volumes of the same workload
+config1/3/4/5 = the other workloads: one hologram's retrieval + unwrapping, Goldstein on 64 x 512^2
+              maps, eight uneven cells in one graph, the 120-angle projected-gradient fit
 """
 from __future__ import annotations
 
@@ -26,14 +30,32 @@ sys.path.insert(0, str(ROOT))
 METRIC = "256³ RI vol/s (60×1024² holos): batched vs plain, same GPU, RI err ≤1e-5"
 HBM_PEAK_GBS = 8000.0
 A, H, V, ITERS = 60, 1024, 256, 10
-# HBM bytes per launch of each volume pass at 256^3 (16 pencils per workgroup), measured with
-# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (separate runs), fetch scaled by 2.000 and write by
-# 1.000 so that the copy yardstick reads as its known 128 MiB in + 128 MiB out. The profiler cannot run
-# inside this process, so the figures are carried here; profiles/traffic_config2_pmc.md has the runs.
-TRAFFIC_KIB = {0: 131209 + 131072, 1: 131157 + 131072, 2: 170176 + 131072}
+# HBM bytes per launch of each volume pass come from the committed counter profile (two rocprofv3 --pmc
+# runs, FETCH_SIZE then WRITE_SIZE, scaled by the copy yardstick); its machine-readable line names the
+# libctdm.so build it measured. The profiler cannot run inside this process.
+TRAFFIC_PROFILE = ROOT / "profiles" / "traffic_config2_pmc.md"
+PASS_KEYS = {0: "k_vol_x", 1: "k_vol_y", 2: "k_vol_zdc"}
 PASS_NAMES = {0: "k_vol_x (inverse x-FFT + projection + forward x-FFT)",
               1: "k_vol_y (y-FFT)",
               2: "k_vol_zdc (z-FFT + data consistency + inverse z-FFT)"}
+
+
+def lib_sha16() -> str:
+    import hashlib
+
+    return hashlib.sha256((ROOT / "ctdm_amd" / "libctdm.so").read_bytes()).hexdigest()[:16]
+
+
+def profiled_traffic():
+    """({pass key: HBM bytes per launch}, build hash) from the profile's `bench-traffic:` JSON line."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("bench-traffic:"):
+                d = json.loads(line.split(":", 1)[1])
+                return d["bytes"], d["build"]
+    except (OSError, ValueError, KeyError):
+        pass
+    return None, None
 
 
 def log(*a):
@@ -50,6 +72,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-config4", action="store_true", help="skip the 8-cell uneven batch")
+    ap.add_argument("--no-extra", action="store_true", help="skip the config 1, 3 and 5 legs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -120,14 +143,27 @@ def main() -> int:
     cms, cbytes = ctx.time_pass(4, V, 0, 20)  # plain streaming copy of the same volume: the yardstick
     dom = max(kernels, key=lambda k: per_vol[k] * kernels[k]["ms"])
     d = kernels[dom]
+    traffic, traffic_build = profiled_traffic()
+    build = lib_sha16()
+    # the same y pass and copy on a 512^3 volume (1 GiB, four times the 256 MB last-level cache): HBM rates
+    # without the cache's help, beside the 256^3 rates above
+    yms, ybytes = ctx.time_pass(1, 512, 8, 6)
+    c5ms, c5bytes = ctx.time_pass(4, 512, 8, 6)
     roofline = {"bound": "hbm", "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_KIB[dom] * 1024,
+                "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
+                "traffic": traffic.get(PASS_KEYS[dom]) if traffic else None,
+                "traffic_source": {"profile": str(TRAFFIC_PROFILE.relative_to(ROOT)), "profiled_build": traffic_build,
+                                   "this_build": build, "same_build": traffic_build == build},
                 "kernel": d["name"], "bytes_per_launch": int(d["bytes"]),
                 "avg_launch_ms": round(d["ms"], 5), "share_of_step": round(d["share"], 3),
                 "other_kernels": {PASS_NAMES[k]: {"GB/s": round(v["gbs"], 1), "avg_ms": round(v["ms"], 5)}
                                   for k, v in kernels.items() if k != dom},
                 "copy_yardstick": {"GB/s": round(cbytes / cms / 1e6, 1), "avg_ms": round(cms, 5),
-                                   "bytes_per_launch": int(cbytes)}}
+                                   "bytes_per_launch": int(cbytes)},
+                "at_512": {"k_vol_y GB/s": round(ybytes / yms / 1e6, 1), "k_vol_y avg_ms": round(yms, 4),
+                           "copy GB/s": round(c5bytes / c5ms / 1e6, 1), "bytes_per_launch": int(ybytes),
+                           "note": "1 GiB volume, beyond the 256 MB last-level cache: HBM rates"}}
+    log(f"512^3: y pass {ybytes / yms / 1e6:.0f} GB/s, copy {c5bytes / c5ms / 1e6:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -145,18 +181,22 @@ def main() -> int:
         tc = (time.perf_counter() - t0) / nvol
         err = float(np.abs(ri_b.cpu().numpy() - ref).max())
         cpu = {"value": round(1.0 / tc, 4), "unit": "volumes/s", "cores": threads, "kind": "port",
+               "host_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
                "sample": f"{nvol} full volumes of the same workload (60 x 1024^2 pairs -> 256^3, "
-                         f"10 iterations) through the FP64 oracle, {tc:.1f} s each",
+                         f"10 iterations) through the FP64 oracle on {threads} threads, {tc:.1f} s each",
                "ri_max_err_vs_gpu": err}
         log(f"cpu baseline {tc:.1f} s/volume, GPU-vs-oracle max|dn| {err:.2e}")
 
     config4 = None if args.no_config4 else run_config4(ctx, args)
+    config1 = None if args.no_extra else run_config1(ctx, args)
+    config3 = None if args.no_extra else run_config3(ctx, args)
+    config5 = None if args.no_extra else run_config5(ctx, args)
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "volumes/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded ellipsoid cell with 5 nucleoli, 60-angle NA 1.0 cone)",
+        "data": "synthetic (seeded ellipsoid cell with 5 nucleoli, first-order Rytov fields, 60-angle NA 1.0 cone)",
         "config": {"workload": "one cell: 60 x 1024^2 f32 hologram pairs -> 256^3 complex64 RI, "
                                "10 positivity iterations", "angles": A, "hologram": H, "volume": V,
                    "iters": ITERS, "mode": "batched, hipGraph replay"},
@@ -164,6 +204,7 @@ def main() -> int:
         "speedup_vs_plain": plain_value and round(value / plain_value, 2),
         "ri_max_diff_batched_vs_plain": dn, "wall_ms_per_step": round(wall, 4),
         "roofline": roofline, "cpu_baseline": cpu, "config4": config4,
+        "config1": config1, "config3": config3, "config5": config5,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -249,6 +290,160 @@ def run_config4(ctx, args):
         del c["holo"], c["bg"]
     torch.cuda.empty_cache()
     return res
+
+
+def _oracle_threads():
+    from oracle import binding as orc
+
+    return orc, orc.threads(min(16, len(os.sched_getaffinity(0))))
+
+
+def _time_calls(fn, reps):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def run_config1(ctx, args):
+    """One 1024^2 off-axis hologram and its background (sphere phase, radius 80 px, peak 12 rad, 2nd-order
+    tilt, noise 0.01): sideband retrieval to 256^2 and DCT least-squares unwrapping. Reports holograms/s of
+    the single-hologram call sequence (retrieve + unwrap, the crop located once as the reference does on
+    frame 1), the RMS phase error against the FP64 oracle and the oracle's own rate on the host."""
+    import numpy as np
+    import torch
+
+    import ctdm_amd
+
+    log("step: config1 (one 1024^2 hologram: retrieval + unwrapping)")
+    holo, bg = ctdm_amd.synth_hologram(1, 1024)
+    hd, bd = torch.from_numpy(holo).cuda(), torch.from_numpy(bg).cuda()
+    crop = ctx.find_sideband(bd, M=256)
+    out = {}
+
+    def step():
+        w, la = ctx.retrieve(hd, bd, crop)
+        out["un"] = ctx.unwrap_dct(w)
+        out["w"] = w
+
+    ms = _time_calls(step, 50)
+    orc, threads = _oracle_threads()
+    t0 = time.perf_counter()
+    reps = 0
+    while reps < 3 or time.perf_counter() - t0 < 2.0:
+        r = orc.retrieve(holo, bg, crop.row, crop.col, crop.radius, crop.M)
+        ref = orc.unwrap_dct(r["wrapped"])[0]
+        reps += 1
+    tc = (time.perf_counter() - t0) / reps
+    got = out["un"][0].cpu().numpy().astype(np.float64)
+    d = (got - got.mean()) - (ref - ref.mean())
+    rms = float(np.sqrt(np.mean(d ** 2)))
+    log(f"config1: {ms * 1e3:.1f} us per hologram ({1000.0 / ms:.0f} holograms/s), oracle {tc * 1e3:.1f} ms, "
+        f"rms phase error {rms:.2e} rad")
+    return {"workload": "one 1024^2 hologram pair -> 256^2 wrapped phase, log-amplitude, DCT-unwrapped phase",
+            "crop": list(crop.astuple()), "ms_per_hologram": round(ms, 5), "holograms_per_s": round(1000.0 / ms, 1),
+            "rms_phase_err_vs_oracle_rad": rms,
+            "cpu_baseline": {"value": round(1.0 / tc, 2), "unit": "holograms/s", "cores": threads, "kind": "port",
+                             "sample": f"{reps} holograms through the FP64 oracle"}}
+
+
+def run_config3(ctx, args):
+    """Goldstein branch-cut unwrapping of 64 x 512^2 wrapped maps (8 Gaussians up to 40 rad + uniform noise
+    in +-0.8 pi): maps/s, the whole call's algorithmic bytes (4 B in; residue, cut, 2 pi count and unwrapped
+    phase out: 14 B per pixel) against HBM peak, bit-exactness against the host specification, and the
+    oracle's rate on the host."""
+    import numpy as np
+    import torch
+
+    import ctdm_amd
+
+    log("step: config3 (Goldstein, 64 x 512^2)")
+    B, N = 64, 512
+    w = ctdm_amd.synth_wrapped(3, B, N, N)
+    wd = torch.from_numpy(w).cuda()
+    out = {}
+
+    def step():
+        out["r"] = ctx.goldstein(wd)
+
+    ms = _time_calls(step, 3)
+    orc, threads = _oracle_threads()
+    t0 = time.perf_counter()
+    ref = orc.goldstein(w)
+    tc = time.perf_counter() - t0
+    exact = all(np.array_equal(g.cpu().numpy(), r) for g, r in zip(out["r"], ref))
+    nbytes = 14.0 * B * N * N
+    log(f"config3: {ms:.2f} ms for {B} maps ({B * 1000.0 / ms:.0f} maps/s), oracle {tc * 1e3:.0f} ms, "
+        f"bit-exact {exact}")
+    return {"workload": "64 x 512^2 wrapped maps -> residues, cuts, 2 pi counts, unwrapped phase",
+            "ms_per_call": round(ms, 3), "maps_per_s": round(B * 1000.0 / ms, 1),
+            "algorithmic_bytes": int(nbytes), "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "bit_exact_vs_oracle": exact,
+            "cpu_baseline": {"value": round(B / tc, 1), "unit": "maps/s", "cores": threads, "kind": "port",
+                             "sample": "the same 64 maps through the host specification"}}
+
+
+def run_config5(ctx, args):
+    """120 angles of 1024^2 holograms (field noise 0.02) -> 256^3: projected-gradient fit of the measured
+    Rytov spectra, 50 iterations with the 3-D FFT recomputed every step. Reports the fit's time (step 1, the
+    positivity loop on the averaged measurement), the objective of a step-1/2 fit (first, last, whether it
+    fell at every step), the RI difference against the FP64 oracle's step-1 fit and both fits' RMS error
+    against the phantom."""
+    import numpy as np
+    import torch
+
+    import ctdm_amd
+
+    log("step: config5 synth (120 x 1024^2, field noise 0.02)")
+    A5, V5, iters = 120, 256, 50
+    p = ctdm_amd.synth_params(field_noise=0.02)
+    holo, bg, ang = ctdm_amd.synth_cell(5, A5, p)
+    hd, bd = torch.from_numpy(holo).cuda(), torch.from_numpy(bg).cuda()
+    del holo, bg
+    o = ctdm_amd.optics_for(V5, 1024)
+    crop = ctx.find_sideband(bd[0], M=V5)
+    w, la = ctx.retrieve(hd, bd, crop)
+    del hd, bd
+    spec = ctx.rytov_spectrum(ctx.unwrap_dct(w), la)
+    F, hits = ctx.ewald_scatter(spec, ang, V5, o)
+    ctx.normalize_measured(F, hits, o.dx, with_meas=False)
+    f0 = ctx.fft3d(F, +1)
+    del hits
+    res = {}
+
+    def fit():
+        res["n"] = ctx.fit_pgd(f0.clone(), spec, ang, o, iters, 1.0)
+
+    ms = _time_calls(fit, 3)
+    n_half, J = ctx.fit_pgd_trace(f0.clone(), spec, ang, o, iters, 0.5)
+    J = np.array(J)
+    phantom = ctdm_amd.synth_phantom(5, V5, o.dx, p).real
+    rms = {name: float(np.sqrt(np.mean((v.cpu().numpy().real - phantom) ** 2)))
+           for name, v in (("step_1", res["n"]), ("step_0.5", n_half))}
+    out = {"workload": "120 x 1024^2 pairs (field noise 0.02) -> 256^3, 50 projected-gradient iterations",
+           "fit_ms": round(ms, 3), "fit_ms_per_iteration": round(ms / iters, 4),
+           "objective_step_0.5": {"first": float(J[0]), "last": float(J[-1]),
+                                  "decreased_every_step": bool(np.all(np.diff(J) < 0))},
+           "rms_err_vs_phantom": rms}
+    if not args.no_cpu:
+        orc, threads = _oracle_threads()
+        log(f"step: config5 oracle fit ({threads} threads)")
+        t0 = time.perf_counter()
+        ref = orc.fit_pgd(f0.cpu().numpy(), spec.cpu().numpy(), ang, o.wavelength, o.n_med, o.dx, iters, 1.0)
+        out["ri_max_err_vs_oracle"] = float(np.abs(res["n"].cpu().numpy() - ref).max())
+        out["cpu_baseline"] = {"value": round(time.perf_counter() - t0, 2), "unit": "s per fit", "cores": threads,
+                               "kind": "port", "sample": "the same fit through the FP64 oracle"}
+    log(f"config5: fit {ms:.2f} ms ({iters} iterations); J {J[0]:.3e} -> {J[-1]:.3e}; "
+        f"vs oracle {out.get('ri_max_err_vs_oracle')}; rms vs phantom {rms}")
+    torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":
