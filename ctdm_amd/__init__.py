@@ -116,6 +116,8 @@ SIGNATURES = {
     "ctdm_recon_from_maps": ([_vp, _vp, _vp, C.POINTER(Angle), C.c_int, C.c_int, C.POINTER(Optics), C.c_int, C.c_uint, _vp], C.c_int),
     "ctdm_reconstruct": ([_vp, C.POINTER(Cell), C.c_int, C.POINTER(BatchOpts)], C.c_int),
     "ctdm_autotune": ([_vp, C.POINTER(C.c_int), C.c_int, C.c_char_p, C.c_size_t], C.c_int),
+    "ctdm_get_tuning": ([_vp, C.c_int, C.POINTER(C.c_int), C.c_int], C.c_int),
+    "ctdm_set_tuning": ([_vp, C.c_int, C.POINTER(C.c_int), C.c_int], C.c_int),
     "ctdm_time_pass": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "ctdm_time_loop": ([_vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "ctdm_fit_pgd": ([_vp, _vp, _vp, C.POINTER(Angle), C.c_int, C.c_int, C.POINTER(Optics), C.c_int, C.c_double, _vp, C.c_uint], C.c_int),
@@ -500,6 +502,16 @@ class Context:
         for i, d in enumerate(cells):  # report the crop the pipeline used
             d["crop_used"] = arr[i].crop
         return cells
+
+    def get_tuning(self, V: int):
+        """dict(pencils=[x, y, z+dc, grid], sweep=[x, y]) of volume side V."""
+        v = (C.c_int * 6)()
+        self._call("ctdm_get_tuning", V, v, 6)
+        return dict(pencils=list(v[:4]), sweep=list(v[4:]))
+
+    def set_tuning(self, V: int, pencils=(16, 16, 16, 8), sweep=(0, 0)):
+        v = (C.c_int * 6)(*pencils, *sweep)
+        self._call("ctdm_set_tuning", V, v, 6)
 
     def time_pass(self, pass_kind: int, V: int, pencils: int = 0, reps: int = 20):
         """(average ms per launch, algorithmic bytes per launch) of a fused volume pass."""

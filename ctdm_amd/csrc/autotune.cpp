@@ -2,8 +2,10 @@
 // y, z with data consistency) with 8 and 16 pencils per workgroup on this device, inside the
 // positivity loop's own pass sequence (ctdm_time_loop; each pass reads what the previous one wrote),
 // and keeps the faster choice (for y and z+dc, 8 pencils only when clearly faster) in the context
-// (used by every later launch of that side, including graph captures made afterwards). An explicit call re-times every side; a reconstruction asking for
-// tuning only times sides not tuned yet. Radix plans are fixed per length (fft_dev.hpp); the gridding pass
+// (used by every later launch of that side, including graph captures made afterwards); then the sweep
+// direction of the x and y passes (forward, or one of them walking the z planes backwards so that it
+// starts on the planes the previous pass wrote last). An explicit call re-times every side; a
+// reconstruction asking for tuning only times sides not tuned yet. Radix plans are fixed per length (fft_dev.hpp); the gridding pass
 // keeps 8 pencils (its LDS staging of every angle's samples limits it to two 16-pencil workgroups per CU;
 // it is timed only inside a full reconstruction).
 #include <cstdio>
@@ -45,7 +47,27 @@ std::string autotune(Ctx& c, const int* sizes, int nsizes, bool force)
                           names[k], m[0], m[1], t.pencils[k], bytes[k] / (1e6 * std::min(m[0], m[1])));
             rep << line;
         }
+        // Sweep directions: the loop sequence timed with the x pass, then the y passes, walking the z
+        // planes backwards (outputs are identical; only the order of the tiles changes), kept when the
+        // iteration (x + 2 y + z+dc) is at least 1 % faster than with both forward.
+        static const int combos[3][2] = {{0, 0}, {1, 0}, {0, 1}};
+        int best = 0;
+        for (int v = 0; v < 3; ++v) {
+            t.rev[0] = combos[v][0];
+            t.rev[1] = combos[v][1];
+            double ms[3], bytes[3];
+            time_loop(c, V, kIters, ms, bytes);
+            t.sweep_ms[v] = ms[0] + 2.0 * ms[1] + ms[2];
+            if (t.sweep_ms[v] < (best == 0 ? 0.99 : 1.0) * t.sweep_ms[best]) best = v;
+        }
+        t.rev[0] = combos[best][0];
+        t.rev[1] = combos[best][1];
+        char line[200];
+        std::snprintf(line, sizeof line, "V=%d sweep  fwd %.4f ms  x-rev %.4f ms  y-rev %.4f ms per iteration -> %s\n", V,
+                      t.sweep_ms[0], t.sweep_ms[1], t.sweep_ms[2], best == 0 ? "fwd" : (best == 1 ? "x-rev" : "y-rev"));
+        rep << line;
         t.tuned = true;
+        ++c.tune_epoch;
     }
     return rep.str();
 }

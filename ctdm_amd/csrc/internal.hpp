@@ -58,8 +58,10 @@ struct Ctx;
 // Per-size kernel choices picked by the autotuner (defaults are the untuned picks).
 struct Tune {
     int pencils[4] = {16, 16, 16, 8};  // pencils per workgroup of the x, y, z+dc and gridding passes
+    int rev[2] = {0, 0};               // x and y passes sweep the z planes backwards
     bool tuned = false;
     double ms[4][2] = {};               // measured ms per launch for B = 8, 16 (after tuning)
+    double sweep_ms[3] = {};            // loop iteration (x + 2 y + z+dc) for sweeps (fwd, fwd), (rev, fwd), (fwd, rev)
 };
 
 struct GraphKey {
@@ -86,6 +88,7 @@ struct Ctx {
     std::map<int, float2*> twiddles;   // N -> device table exp(-2*pi*i*m/N), m < N
     std::map<std::string, DevBuf> scratch;
     std::map<int, Tune> tune;
+    uint64_t tune_epoch = 0;  // bumped by every tuning change; part of the graph key (graphs bake in choices)
     std::map<GraphKey, GraphEntry> graphs;
     uint64_t graph_clock = 0;
     std::vector<void*> retired;  // grown buffers still referenced by a cached graph

@@ -546,10 +546,12 @@ struct Tile {
     // share lines and are kept on one XCD (xcd_block).
     static constexpr bool REMAP = BFAST && B * 8 < 128;
     static __device__ int block() { return REMAP ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x; }
-    __device__ Tile()
+    // rev: tiles in reverse order (the volume swept from the last plane back), so that a pass can start
+    // on the planes the previous pass wrote last
+    __device__ Tile(int rev = 0)
     {
         const int tid = threadIdx.x;
-        const int blk = block();
+        const int blk = rev ? (int)gridDim.x - 1 - block() : block();
         b = BFAST ? tid % B : tid / T;
         t = BFAST ? tid / B : tid % T;
         const long long VV = (long long)V * V;
@@ -580,13 +582,13 @@ struct Tile {
 template <int V, int B, int MODE>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_x(float2* __restrict__ W, float2* __restrict__ out,
                                                          const float2* __restrict__ tw, Proj p, float scale,
-                                                         int shift)
+                                                         int shift, int rev)
 {
     __shared__ float2 lds[B * LdsSize<V>::value];
     __shared__ float2 twl[LdsSize<V>::value];
     stage_twiddles<V>(twl, tw);
     const TwPad twp{twl};
-    Tile<V, 0, B> tl;
+    Tile<V, 0, B> tl(rev);
     float2* s = lds + tl.b * LdsSize<V>::value;
     float2* row = W + tl.base;
     // destination row of the final n (octant shift)
@@ -696,14 +698,15 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
 
 // Y pass (plain FFT along y).
 template <int V, int B, int DIR>
-__global__ __launch_bounds__(B* Plan<V>::T) void k_vol_y(float2* __restrict__ W, const float2* __restrict__ tw)
+__global__ __launch_bounds__(B* Plan<V>::T) void k_vol_y(float2* __restrict__ W, const float2* __restrict__ tw,
+                                                         int rev)
 {
     constexpr int LS = LdsStride<V, B>::value;
     __shared__ float2 lds[B * LS];
     __shared__ float2 twl[LdsSize<V>::value];
     stage_twiddles<V>(twl, tw);
     const TwPad twp{twl};
-    Tile<V, 1, B> tl;
+    Tile<V, 1, B> tl(rev);
     float2* s = lds + tl.b * LS;
     float2* pen = W + tl.base;
     auto gl = [&](int y) -> float2 { return pen[(long long)y * tl.stride]; };
@@ -926,25 +929,25 @@ struct VolOps {
     static constexpr int NT = B * Plan<V>::T;
     static dim3 grid() { return dim3((unsigned)((long long)V * V / B)); }
 
-    static void x(Ctx& c, int mode, float2* W, float2* out, const Proj& p, float scale, bool shift)
+    static void x(Ctx& c, int mode, float2* W, float2* out, const Proj& p, float scale, bool shift, int rev = 0)
     {
         const float2* tw = c.tw(V);
         const int sh = shift;
         switch (mode) {
-        case 0: vlaunch(c, &k_vol_x<V, B, 0>, grid(), dim3(NT), W, out, tw, p, scale, sh); break;
-        case 1: vlaunch(c, &k_vol_x<V, B, 1>, grid(), dim3(NT), W, out, tw, p, scale, sh); break;
-        case 2: vlaunch(c, &k_vol_x<V, B, 2>, grid(), dim3(NT), W, out, tw, p, scale, sh); break;
-        default: vlaunch(c, &k_vol_x<V, B, 3>, grid(), dim3(NT), W, out, tw, p, scale, sh); break;
+        case 0: vlaunch(c, &k_vol_x<V, B, 0>, grid(), dim3(NT), W, out, tw, p, scale, sh, rev); break;
+        case 1: vlaunch(c, &k_vol_x<V, B, 1>, grid(), dim3(NT), W, out, tw, p, scale, sh, rev); break;
+        case 2: vlaunch(c, &k_vol_x<V, B, 2>, grid(), dim3(NT), W, out, tw, p, scale, sh, rev); break;
+        default: vlaunch(c, &k_vol_x<V, B, 3>, grid(), dim3(NT), W, out, tw, p, scale, sh, rev); break;
         }
         CTDM_CHECK_LAUNCH("k_vol_x");
     }
-    static void y(Ctx& c, float2* W, int dir)
+    static void y(Ctx& c, float2* W, int dir, int rev = 0)
     {
         const float2* tw = c.tw(V);
         if (dir < 0)
-            vlaunch(c, &k_vol_y<V, B, -1>, grid(), dim3(NT), W, tw);
+            vlaunch(c, &k_vol_y<V, B, -1>, grid(), dim3(NT), W, tw, rev);
         else
-            vlaunch(c, &k_vol_y<V, B, +1>, grid(), dim3(NT), W, tw);
+            vlaunch(c, &k_vol_y<V, B, +1>, grid(), dim3(NT), W, tw, rev);
         CTDM_CHECK_LAUNCH("k_vol_y");
     }
     static void zdc(Ctx& c, float2* W, const DcList& dc, float tau)
@@ -976,15 +979,19 @@ enum PassKind { PASS_X = 0, PASS_Y = 1, PASS_ZDC = 2, PASS_GRID = 3, NPASS = 4, 
 template <int V>
 struct Vol {
     static int pick(Ctx& c, PassKind k) { return c.tuning(V).pencils[k] == 8 ? 8 : 16; }
+    // Sweep direction of the x and y passes (tuned): both walk the z planes in tile order, so running
+    // one of them backwards starts each pass on the planes the previous one wrote last.
     static void x(Ctx& c, int mode, float2* W, float2* out, const Proj& p, float scale, bool shift)
     {
-        if (pick(c, PASS_X) == 8) VolOps<V, 8>::x(c, mode, W, out, p, scale, shift);
-        else VolOps<V, 16>::x(c, mode, W, out, p, scale, shift);
+        const int rev = c.tuning(V).rev[0];
+        if (pick(c, PASS_X) == 8) VolOps<V, 8>::x(c, mode, W, out, p, scale, shift, rev);
+        else VolOps<V, 16>::x(c, mode, W, out, p, scale, shift, rev);
     }
     static void y(Ctx& c, float2* W, int dir)
     {
-        if (pick(c, PASS_Y) == 8) VolOps<V, 8>::y(c, W, dir);
-        else VolOps<V, 16>::y(c, W, dir);
+        const int rev = c.tuning(V).rev[1];
+        if (pick(c, PASS_Y) == 8) VolOps<V, 8>::y(c, W, dir, rev);
+        else VolOps<V, 16>::y(c, W, dir, rev);
     }
     static void zdc(Ctx& c, float2* W, const DcList& dc, float tau = 1.f)
     {

@@ -363,6 +363,7 @@ void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& op
     if (opts.graph) {
         key = graph_key(cells, ncells);
         key.words.push_back((uintptr_t)(opts.lanes == 1 ? 1 : 2));
+        key.words.push_back((uintptr_t)c.tune_epoch);  // a graph holds the kernel choices made when captured
         it = c.graphs.find(key);
     }
     if (it != c.graphs.end()) {  // replay: the graph's own row tables get this call's angles

@@ -295,11 +295,21 @@ ctdm_status ctdm_reconstruct(ctdm_ctx *ctx, ctdm_cell *cells, int ncells,
 /* Times the three fused volume passes (x with projection, y, z with data consistency) of each
  * listed volume side V with 8 and 16 pencils per workgroup on this device, inside the positivity
  * loop's own pass sequence, and caches the choice in the context: the faster one, except that y and
- * z take 8 pencils only when at least 3 % faster (their 8-pencil tiles write half lines). Radix plans
- * and the gridding block are fixed. No reference counterpart. Every listed side is re-timed, also
- * one tuned before. Synchronises. `report` (HOST, may be NULL) receives a text summary. */
+ * z take 8 pencils only when at least 3 % faster (their 8-pencil tiles write half lines). Then times the
+ * loop iteration with the x or the y passes sweeping the z planes backwards and keeps a backward sweep
+ * when it is at least 1 % faster. Radix plans and the gridding block are fixed (ctdm_set_tuning sets the
+ * latter). No reference counterpart. Every listed side is re-timed, also one tuned before. Synchronises.
+ * `report` (HOST, may be NULL) receives a text summary. */
 ctdm_status ctdm_autotune(ctdm_ctx *ctx, const int *sizes, int nsizes, char *report,
                           size_t report_len);
+
+/* The tuning table entry of volume side V (no reference counterpart): values[0..5] = pencils per workgroup
+ * of the x, y, z+dc and gridding passes (8 or 16), then the sweep direction of the x and y passes (0 forward,
+ * 1 backwards over the z planes). Every choice computes the same values; they differ in speed only.
+ * ctdm_get_tuning reads the entry (defaults before tuning); ctdm_set_tuning writes it and marks the side
+ * tuned, for callers that tuned before or want one choice. n must be 6. */
+ctdm_status ctdm_get_tuning(ctdm_ctx *ctx, int V, int *values, int n);
+ctdm_status ctdm_set_tuning(ctdm_ctx *ctx, int V, const int *values, int n);
 
 /* Average duration [ms] of one launch of a fused volume pass (pass 0: x with projection, 1: y,
  * 2: z with data consistency; 4: a plain streaming copy of the V^3 complex64 volume, the bandwidth

@@ -206,12 +206,36 @@ def test_autotune_keeps_results_and_times_passes(ctx):
     holo, bg, ang = _cell(9, A, H, V)
     before, _ = _run(ctx, holo, bg, ang, V, 2, mode=1)
     report = ctx.autotune([V])
-    assert "V=64" in report and report.count("->") == 3
+    assert "V=64" in report and report.count("->") == 4 and "sweep" in report
     after, _ = _run(ctx, holo, bg, ang, V, 2, mode=1)
     assert torch.equal(before, after)  # the pencil grouping does not change any value
     for k in (0, 1, 2):
         ms, nbytes = ctx.time_pass(k, V, 0, 5)
         assert ms > 0 and nbytes >= 2 * 8 * V ** 3
+
+
+@pytest.mark.parametrize("V", [64, 128])
+def test_every_tuning_choice_gives_identical_volumes(ctx, V):
+    """Pencils per workgroup (8 or 16) of each volume pass and of the gridding pass, and the sweep
+    direction of the x and y passes, change only the order and grouping of the work: every combination
+    reconstructs bit-identical volumes, eager and graph-replayed."""
+    import itertools
+
+    A, H = 9, 256
+    holo, bg, ang = _cell(12, A, H, V)
+    saved = ctx.get_tuning(V)
+    ref = None
+    try:
+        for px, py, pz, pg, rx, ry in itertools.product((8, 16), (8, 16), (8, 16), (8, 16), (0, 1), (0, 1)):
+            if (px, py, pz, pg) != (16, 16, 16, 8) and (rx, ry) != (0, 0):
+                continue  # every pencil choice forward, and every sweep with the default pencils
+            ctx.set_tuning(V, (px, py, pz, pg), (rx, ry))
+            got, _ = _run(ctx, holo, bg, ang, V, 3, mode=1, graph=(rx + ry) % 2 == 1)
+            if ref is None:
+                ref = got
+            assert torch.equal(got, ref), (px, py, pz, pg, rx, ry)
+    finally:
+        ctx.set_tuning(V, saved["pencils"], saved["sweep"])
 
 
 @pytest.mark.parametrize("flags", [0, ctdm_amd.EXEC_PLAIN])
