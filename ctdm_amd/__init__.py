@@ -504,14 +504,14 @@ class Context:
         return cells
 
     def get_tuning(self, V: int):
-        """dict(pencils=[x, y, z+dc, grid], sweep=[x, y]) of volume side V."""
-        v = (C.c_int * 6)()
-        self._call("ctdm_get_tuning", V, v, 6)
-        return dict(pencils=list(v[:4]), sweep=list(v[4:]))
+        """dict(pencils=[x, y, z+dc, grid], sweep=[x, y], yplan=0|1) of volume side V."""
+        v = (C.c_int * 7)()
+        self._call("ctdm_get_tuning", V, v, 7)
+        return dict(pencils=list(v[:4]), sweep=list(v[4:6]), yplan=v[6])
 
-    def set_tuning(self, V: int, pencils=(16, 16, 16, 8), sweep=(0, 0)):
-        v = (C.c_int * 6)(*pencils, *sweep)
-        self._call("ctdm_set_tuning", V, v, 6)
+    def set_tuning(self, V: int, pencils=(16, 16, 16, 8), sweep=(0, 0), yplan: int = 0):
+        v = (C.c_int * 7)(*pencils, *sweep, yplan)
+        self._call("ctdm_set_tuning", V, v, 7)
 
     def time_pass(self, pass_kind: int, V: int, pencils: int = 0, reps: int = 20):
         """(average ms per launch, algorithmic bytes per launch) of a fused volume pass."""

@@ -179,8 +179,10 @@ ctdm_status ctdm_create(ctdm_ctx** out, const ctdm_config* cfg, ctdm_stream stre
         return CTDM_EDEVICE;
     }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, ctx->c.device) == hipSuccess)
+    if (hipGetDeviceProperties(&prop, ctx->c.device) == hipSuccess) {
         ctx->c.max_lds = (int)prop.sharedMemPerBlock;
+        ctx->c.arch = prop.gcnArchName;
+    }
     const ctdm_status st = guarded(ctx, [](ctdm::Ctx& c) { ctdm::noalloc_setup(c); });
     if (st != CTDM_OK) {
         ctdm_destroy(ctx);
@@ -409,25 +411,27 @@ ctdm_status ctdm_autotune(ctdm_ctx* ctx, const int* sizes, int nsizes, char* rep
 ctdm_status ctdm_get_tuning(ctdm_ctx* ctx, int V, int* values, int n)
 {
     return guarded(ctx, [&](ctdm::Ctx& c) {
-        CTDM_REQUIRE(values && n == 6, "get_tuning: 6 values");
+        CTDM_REQUIRE(values && n == 7, "get_tuning: 7 values");
         const ctdm::Tune& t = c.tuning(V);
         for (int k = 0; k < 4; ++k) values[k] = t.pencils[k];
         values[4] = t.rev[0];
         values[5] = t.rev[1];
+        values[6] = t.yplan;
     });
 }
 
 ctdm_status ctdm_set_tuning(ctdm_ctx* ctx, int V, const int* values, int n)
 {
     return guarded(ctx, [&](ctdm::Ctx& c) {
-        CTDM_REQUIRE(values && n == 6, "set_tuning: 6 values");
+        CTDM_REQUIRE(values && n == 7, "set_tuning: 7 values");
         CTDM_REQUIRE(V == 32 || V == 64 || V == 128 || V == 192 || V == 256, "set_tuning: volume side has no kernel");
         for (int k = 0; k < 4; ++k) CTDM_REQUIRE(values[k] == 8 || values[k] == 16, "set_tuning: pencils must be 8 or 16");
-        CTDM_REQUIRE((values[4] | values[5]) >> 1 == 0, "set_tuning: sweep directions must be 0 or 1");
+        CTDM_REQUIRE((values[4] | values[5] | values[6]) >> 1 == 0, "set_tuning: sweeps and plan must be 0 or 1");
         ctdm::Tune& t = c.tuning(V);
         for (int k = 0; k < 4; ++k) t.pencils[k] = values[k];
         t.rev[0] = values[4];
         t.rev[1] = values[5];
+        t.yplan = values[6];
         t.tuned = true;
         ++c.tune_epoch;
     });

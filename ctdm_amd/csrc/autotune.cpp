@@ -4,16 +4,37 @@
 // and keeps the faster choice (for y and z+dc, 8 pencils only when clearly faster) in the context
 // (used by every later launch of that side, including graph captures made afterwards); then the sweep
 // direction of the x and y passes (forward, or one of them walking the z planes backwards so that it
-// starts on the planes the previous pass wrote last). An explicit call re-times every side; a
-// reconstruction asking for tuning only times sides not tuned yet. Radix plans are fixed per length (fft_dev.hpp); the gridding pass
-// keeps 8 pencils (its LDS staging of every angle's samples limits it to two 16-pencil workgroups per CU;
-// it is timed only inside a full reconstruction).
+// starts on the planes the previous pass wrote last), the radix plan of the y pass and the pencils per
+// workgroup of the gridding pass. An explicit call re-times every side; a reconstruction asking for
+// tuning only times sides not tuned yet. The choices are also kept per (device architecture, side) for
+// the rest of the process, so contexts created later start from them. The other passes keep one radix
+// plan per length (fft_dev.hpp).
 #include <cstdio>
+#include <mutex>
 #include <sstream>
 
 #include "internal.hpp"
 
 namespace ctdm {
+
+namespace {
+// Choices made by the autotuner in this process, per (device architecture, volume side).
+std::mutex g_tuned_mu;
+std::map<std::pair<std::string, int>, Tune> g_tuned;
+}  // namespace
+
+Tune& Ctx::tuning(int n)
+{
+    auto it = tune.find(n);
+    if (it != tune.end()) return it->second;
+    Tune t;
+    {
+        std::lock_guard<std::mutex> lk(g_tuned_mu);
+        auto g = g_tuned.find({arch, n});
+        if (g != g_tuned.end()) t = g->second;
+    }
+    return tune.emplace(n, t).first->second;
+}
 
 std::string autotune(Ctx& c, const int* sizes, int nsizes, bool force)
 {
@@ -62,12 +83,33 @@ std::string autotune(Ctx& c, const int* sizes, int nsizes, bool force)
         }
         t.rev[0] = combos[best][0];
         t.rev[1] = combos[best][1];
-        char line[200];
+        char line[240];
         std::snprintf(line, sizeof line, "V=%d sweep  fwd %.4f ms  x-rev %.4f ms  y-rev %.4f ms per iteration -> %s\n", V,
                       t.sweep_ms[0], t.sweep_ms[1], t.sweep_ms[2], best == 0 ? "fwd" : (best == 1 ? "x-rev" : "y-rev"));
         rep << line;
+        // Radix plan of the y pass: the alternative factorisation is kept when its y launches are at
+        // least 1 % faster inside the loop sequence.
+        for (int pv = 0; pv < 2; ++pv) {
+            t.yplan = pv;
+            double ms[3], bytes[3];
+            time_loop(c, V, kIters, ms, bytes);
+            t.yplan_ms[pv] = ms[1];
+        }
+        t.yplan = t.yplan_ms[1] < 0.99 * t.yplan_ms[0] ? 1 : 0;
+        std::snprintf(line, sizeof line, "V=%d y plan  base %.4f ms  alternative %.4f ms -> %s\n", V, t.yplan_ms[0],
+                      t.yplan_ms[1], t.yplan ? "alternative" : "base");
+        rep << line;
+        // Gridding pass: 8 or 16 pencils per workgroup for 60 angles (16 only when at least 3 % faster: the
+        // staging of every angle's samples makes its workgroups twice as large in LDS).
+        for (int b = 0; b < 2; ++b) t.grid_ms[b] = time_grid(c, V, b ? 16 : 8, 60, 5);
+        t.pencils[3] = t.grid_ms[1] < 0.97 * t.grid_ms[0] ? 16 : 8;
+        std::snprintf(line, sizeof line, "V=%d grid   B=8 %.4f ms  B=16 %.4f ms  -> B=%d\n", V, t.grid_ms[0],
+                      t.grid_ms[1], t.pencils[3]);
+        rep << line;
         t.tuned = true;
         ++c.tune_epoch;
+        std::lock_guard<std::mutex> lk(g_tuned_mu);
+        g_tuned[{c.arch, V}] = t;
     }
     return rep.str();
 }

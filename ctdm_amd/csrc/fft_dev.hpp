@@ -357,4 +357,64 @@ struct Plan<1024> {
     }
 };
 
+// Alternative radix plan per length with the same thread count, a tuning choice for the volume y pass
+// (the autotuner keeps it only where it measures faster): 32 = 4 x 8, 64 = 4 x 4 x 4, 128 = 16 x 8,
+// 192 = 16 x 12, 256 = 4 x 4 x 16.
+template <int N>
+struct PlanAlt;
+template <>
+struct PlanAlt<32> {
+    static constexpr int T = Plan<32>::T;
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
+    {
+        Stockham<32, T, DIR, 1, 4, 8>::run(t, s, tw, gl, gs);
+    }
+};
+template <>
+struct PlanAlt<64> {
+    static constexpr int T = Plan<64>::T;
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
+    {
+        Stockham<64, T, DIR, 1, 4, 4, 4>::run(t, s, tw, gl, gs);
+    }
+};
+template <>
+struct PlanAlt<128> {
+    static constexpr int T = Plan<128>::T;
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
+    {
+        Stockham<128, T, DIR, 1, 16, 8>::run(t, s, tw, gl, gs);
+    }
+};
+template <>
+struct PlanAlt<192> {
+    static constexpr int T = Plan<192>::T;
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
+    {
+        Stockham<192, T, DIR, 1, 16, 12>::run(t, s, tw, gl, gs);
+    }
+};
+template <>
+struct PlanAlt<256> {
+    static constexpr int T = Plan<256>::T;
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
+    {
+        Stockham<256, T, DIR, 1, 4, 4, 16>::run(t, s, tw, gl, gs);
+    }
+};
+template <>
+struct PlanAlt<512> {
+    static constexpr int T = Plan<512>::T;
+    template <int DIR, class TW, class GL, class GS>
+    __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
+    {
+        Stockham<512, T, DIR, 1, 8, 8, 8>::run(t, s, tw, gl, gs);
+    }
+};
+
 }  // namespace ctdm

@@ -59,9 +59,12 @@ struct Ctx;
 struct Tune {
     int pencils[4] = {16, 16, 16, 8};  // pencils per workgroup of the x, y, z+dc and gridding passes
     int rev[2] = {0, 0};               // x and y passes sweep the z planes backwards
+    int yplan = 0;                     // radix plan of the y pass: 0 Plan<V>, 1 PlanAlt<V>
     bool tuned = false;
     double ms[4][2] = {};               // measured ms per launch for B = 8, 16 (after tuning)
     double sweep_ms[3] = {};            // loop iteration (x + 2 y + z+dc) for sweeps (fwd, fwd), (rev, fwd), (fwd, rev)
+    double yplan_ms[2] = {};            // y pass per launch with either radix plan
+    double grid_ms[2] = {};             // gridding pass per launch with 8 / 16 pencils
 };
 
 struct GraphKey {
@@ -132,7 +135,10 @@ struct Ctx {
     // The same for an event (made on first use): waits for the work recorded before it only.
     hipEvent_t wait_ev = nullptr;
     void spin_event(hipEvent_t e);
-    Tune& tuning(int n) { return tune[n]; }
+    // Tuning entry of side n; a side this context has not tuned starts from the choices the autotuner
+    // made for it on a device of the same architecture earlier in the process, if any.
+    Tune& tuning(int n);
+    std::string arch;  // gcnArchName of the context's device
 };
 
 // ---------------------------------------------------------------- module launchers
@@ -229,6 +235,8 @@ void fit_pgd(Ctx& c, float2* f, const float2* spec, const ctdm_angle* ang, int A
 
 // Average ms of one launch of volume pass `kind` (0 x, 1 y, 2 z+dc) with B pencils per workgroup.
 double time_pass(Ctx& c, int kind, int V, int B, int reps, double* bytes);
+// Average ms of one gridding pass at side V with B pencils per workgroup for A synthetic angles.
+double time_grid(Ctx& c, int V, int B, int A, int reps);
 void time_loop(Ctx& c, int V, int iters, double ms[3], double bytes[3]);
 
 // Pipeline (pipeline.cpp)

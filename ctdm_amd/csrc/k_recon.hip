@@ -697,7 +697,7 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
 }
 
 // Y pass (plain FFT along y).
-template <int V, int B, int DIR>
+template <int V, int B, int DIR, int PV>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_y(float2* __restrict__ W, const float2* __restrict__ tw,
                                                          int rev)
 {
@@ -711,7 +711,10 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_y(float2* __restrict__ W,
     float2* pen = W + tl.base;
     auto gl = [&](int y) -> float2 { return pen[(long long)y * tl.stride]; };
     auto gs = [&](int k, float2 v) { pen[(long long)k * tl.stride] = v; };
-    Plan<V>::template run<DIR>(tl.t, s, twp, gl, gs);
+    if constexpr (PV == 0)
+        Plan<V>::template run<DIR>(tl.t, s, twp, gl, gs);
+    else
+        PlanAlt<V>::template run<DIR>(tl.t, s, twp, gl, gs);
 }
 
 // Compact list from a dense measured spectrum (positivity API): one thread per pencil.
@@ -941,13 +944,20 @@ struct VolOps {
         }
         CTDM_CHECK_LAUNCH("k_vol_x");
     }
-    static void y(Ctx& c, float2* W, int dir, int rev = 0)
+    static void y(Ctx& c, float2* W, int dir, int rev = 0, int plan = 0)
     {
         const float2* tw = c.tw(V);
-        if (dir < 0)
-            vlaunch(c, &k_vol_y<V, B, -1>, grid(), dim3(NT), W, tw, rev);
-        else
-            vlaunch(c, &k_vol_y<V, B, +1>, grid(), dim3(NT), W, tw, rev);
+        if (plan == 0) {
+            if (dir < 0)
+                vlaunch(c, &k_vol_y<V, B, -1, 0>, grid(), dim3(NT), W, tw, rev);
+            else
+                vlaunch(c, &k_vol_y<V, B, +1, 0>, grid(), dim3(NT), W, tw, rev);
+        } else {
+            if (dir < 0)
+                vlaunch(c, &k_vol_y<V, B, -1, 1>, grid(), dim3(NT), W, tw, rev);
+            else
+                vlaunch(c, &k_vol_y<V, B, +1, 1>, grid(), dim3(NT), W, tw, rev);
+        }
         CTDM_CHECK_LAUNCH("k_vol_y");
     }
     static void zdc(Ctx& c, float2* W, const DcList& dc, float tau)
@@ -989,9 +999,9 @@ struct Vol {
     }
     static void y(Ctx& c, float2* W, int dir)
     {
-        const int rev = c.tuning(V).rev[1];
-        if (pick(c, PASS_Y) == 8) VolOps<V, 8>::y(c, W, dir, rev);
-        else VolOps<V, 16>::y(c, W, dir, rev);
+        const int rev = c.tuning(V).rev[1], plan = c.tuning(V).yplan;
+        if (pick(c, PASS_Y) == 8) VolOps<V, 8>::y(c, W, dir, rev, plan);
+        else VolOps<V, 16>::y(c, W, dir, rev, plan);
     }
     static void zdc(Ctx& c, float2* W, const DcList& dc, float tau = 1.f)
     {
@@ -1328,6 +1338,62 @@ void time_loop(Ctx& c, int V, int iters, double ms[3], double bytes[3])
     case 192: time_loop_v<192>(c, iters, ms, bytes); break;
     case 256: time_loop_v<256>(c, iters, ms, bytes); break;
     default: fail(CTDM_EUNSUPPORTED, "time_loop: volume side has no kernel");
+    }
+}
+
+// Average ms of one gridding pass (gridding + hit averaging + list + inverse z-FFT) at side V with B
+// pencils per workgroup, for A angles on an NA 1.0 cone of unit spectra (timing only; the pass's work per
+// pencil does not depend on the spectrum's values).
+template <int V>
+static double time_grid_v(Ctx& c, int B, int A, int reps)
+{
+    const size_t VV = (size_t)V * V;
+    const ctdm_optics o{532e-9, 1.337, 102.4e-6 / V};
+    const EwaldGeom g = ewald_geom(o, V);
+    std::vector<ctdm_angle> ang(A);
+    const double s = 1.0 / 1.337;
+    for (int a = 0; a < A; ++a) {
+        const double az = 2.0 * M_PI * a / A;
+        ang[a] = ctdm_angle{std::asin(s * std::sin(az)), std::asin(s * std::cos(az))};
+    }
+    const double* rows = upload_rows(c, "tg.rows", angle_rows(g, ang.data(), A));
+    float2* spec = (float2*)c.buf("tg.spec", sizeof(float2) * VV * A);
+    fill(c, spec, 2 * (long long)(VV * A), 1.f);
+    float2* W = (float2*)c.buf("vol.W", sizeof(float2) * VV * V);
+    const DcList dc = dc_alloc(c, V, A);
+    auto launch = [&] {
+        if (B == 8)
+            VolOps<V, 8>::grid_pass(c, spec, rows, A, g, false, W, dc);
+        else
+            VolOps<V, 16>::grid_pass(c, spec, rows, A, g, false, W, dc);
+    };
+    launch();
+    hipEvent_t e0, e1;
+    CTDM_HIP(hipEventCreate(&e0));
+    CTDM_HIP(hipEventCreate(&e1));
+    CTDM_HIP(hipEventRecord(e0, c.stream));
+    for (int r = 0; r < reps; ++r) launch();
+    CTDM_HIP(hipEventRecord(e1, c.stream));
+    CTDM_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    CTDM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c.dc_V = 0;  // the list now holds this timing's entries, not a reconstruction's
+    return ms / reps;
+}
+
+double time_grid(Ctx& c, int V, int B, int A, int reps)
+{
+    CTDM_REQUIRE(!c.capturing, "time_grid during capture");
+    CTDM_REQUIRE((B == 8 || B == 16) && A > 0 && A < 32768 && reps > 0, "time_grid: bad arguments");
+    switch (V) {
+    case 32: return time_grid_v<32>(c, B, A, reps);
+    case 64: return time_grid_v<64>(c, B, A, reps);
+    case 128: return time_grid_v<128>(c, B, A, reps);
+    case 192: return time_grid_v<192>(c, B, A, reps);
+    case 256: return time_grid_v<256>(c, B, A, reps);
+    default: fail(CTDM_EUNSUPPORTED, "time_grid: volume side has no kernel");
     }
 }
 

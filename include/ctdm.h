@@ -296,18 +296,22 @@ ctdm_status ctdm_reconstruct(ctdm_ctx *ctx, ctdm_cell *cells, int ncells,
  * listed volume side V with 8 and 16 pencils per workgroup on this device, inside the positivity
  * loop's own pass sequence, and caches the choice in the context: the faster one, except that y and
  * z take 8 pencils only when at least 3 % faster (their 8-pencil tiles write half lines). Then times the
- * loop iteration with the x or the y passes sweeping the z planes backwards and keeps a backward sweep
- * when it is at least 1 % faster. Radix plans and the gridding block are fixed (ctdm_set_tuning sets the
- * latter). No reference counterpart. Every listed side is re-timed, also one tuned before. Synchronises.
- * `report` (HOST, may be NULL) receives a text summary. */
+ * loop iteration with the x or the y passes sweeping the z planes backwards (kept when at least 1 %
+ * faster), the y pass with its alternative radix plan (kept when at least 1 % faster) and the gridding
+ * pass with 8 and 16 pencils for 60 angles (16 when at least 3 % faster). The choices also serve contexts
+ * created later in the process on the same GPU architecture. No reference counterpart. Every listed
+ * side is re-timed, also one tuned before. Synchronises. `report` (HOST, may be NULL) receives a text
+ * summary. */
 ctdm_status ctdm_autotune(ctdm_ctx *ctx, const int *sizes, int nsizes, char *report,
                           size_t report_len);
 
-/* The tuning table entry of volume side V (no reference counterpart): values[0..5] = pencils per workgroup
- * of the x, y, z+dc and gridding passes (8 or 16), then the sweep direction of the x and y passes (0 forward,
- * 1 backwards over the z planes). Every choice computes the same values; they differ in speed only.
- * ctdm_get_tuning reads the entry (defaults before tuning); ctdm_set_tuning writes it and marks the side
- * tuned, for callers that tuned before or want one choice. n must be 6. */
+/* The tuning table entry of volume side V (no reference counterpart): values[0..6] = pencils per workgroup
+ * of the x, y, z+dc and gridding passes (8 or 16), the sweep direction of the x and y passes (0 forward,
+ * 1 backwards over the z planes), and the radix plan of the y pass (0 the base factorisation, 1 the
+ * alternative: 32 = 4x8, 64 = 4x4x4, 128 = 16x8, 192 = 16x12, 256 = 4x4x16). Pencil grouping and sweep
+ * order compute bit-identical values; the two radix plans agree to FP32 rounding. ctdm_get_tuning reads the
+ * entry (defaults, or choices tuned earlier in the process on the same architecture); ctdm_set_tuning
+ * writes it and marks the side tuned. n must be 7. */
 ctdm_status ctdm_get_tuning(ctdm_ctx *ctx, int V, int *values, int n);
 ctdm_status ctdm_set_tuning(ctdm_ctx *ctx, int V, const int *values, int n);
 
