@@ -62,6 +62,27 @@ __device__ __forceinline__ float rhs_at(const float* __restrict__ p, int N, int 
     return (c1 - c2) + (c3 - c4);
 }
 
+// The same from rows staged in LDS: st holds map rows y0 - 1 .. y0 + G (row r of st is map row y0 - 1 + r;
+// rows outside the map are never read, since the borders take no difference there).
+__device__ __forceinline__ float rhs_staged(const float* st, int N, int i, int j, int y0, bool sincos)
+{
+    const float* row = st + (j - y0 + 1) * N;
+    const float c = row[i];
+    if (sincos) {
+        float r = 0.f;
+        if (i < N - 1) r += sinf(row[i + 1] - c);
+        if (i > 0) r += sinf(row[i - 1] - c);
+        if (j < N - 1) r += sinf(row[i + N] - c);
+        if (j > 0) r += sinf(row[i - N] - c);
+        return r;
+    }
+    const float c1 = (i == N - 1) ? 0.f : wrapdiff(row[i + 1] - c);
+    const float c2 = (i == 0) ? 0.f : wrapdiff(c - row[i - 1]);
+    const float c3 = (j == N - 1) ? 0.f : wrapdiff(row[i + N] - c);
+    const float c4 = (j == 0) ? 0.f : wrapdiff(c - row[i - N]);
+    return (c1 - c2) + (c3 - c4);
+}
+
 // Makhoul source index: position n of v holds x[src(n)].
 __device__ __forceinline__ int mk_src(int n, int N) { return n < N / 2 ? 2 * n : 2 * (N - 1 - n) + 1; }
 
@@ -83,13 +104,40 @@ __global__ __launch_bounds__(G* Plan<N>::T) void k_uw_rows_fwd(const float* __re
     const int y = valid ? (int)(row % N) : 0;
     const float* map = in + b * N * N;
     float2* s = lds + g * LdsSize<N>::value;
-    auto gl = [&](int n) -> float2 {
-        if (!valid) return make_float2(0.f, 0.f);
-        const int x = mk_src(n, N);
-        return make_float2(mode == 0 ? rhs_at(map, N, x, y, sincos) : map[x + N * y], 0.f);
-    };
     auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };
-    Plan<N>::template run<-1>(t, s, tw_n, gl, gs);
+    // When a workgroup's G rows are consecutive rows of one map, those rows and their neighbours above
+    // and below are first staged into the (still unused) LDS area with coalesced float4 loads: the
+    // transform's first stage, which evaluates the right-hand side, reads every input before its first
+    // write into the area, so the staged rows and the transform can share it.
+    constexpr bool STAGE = N % G == 0 && (G + 2) * N <= 2 * G * LdsSize<N>::value && N % 4 == 0;
+    if constexpr (STAGE) {
+        float* st = reinterpret_cast<float*>(lds);
+        const long long row0 = (long long)blockIdx.x * G;
+        const bool wg_valid = row0 < (long long)batch * N;  // N % G == 0: all G rows valid or none
+        const int y0 = (int)(row0 % N);
+        const float* m0 = in + (wg_valid ? row0 / N : 0) * N * N;
+        const int r_lo = y0 == 0 ? 1 : 0, r_hi = (y0 + G == N) ? G : G + 1;  // staged rows r_lo .. r_hi
+        if (wg_valid) {
+            const int nv = (r_hi - r_lo + 1) * (N / 4);
+            const float4* src = reinterpret_cast<const float4*>(m0 + (long long)(y0 - 1 + r_lo) * N);
+            float4* dst = reinterpret_cast<float4*>(st + r_lo * N);
+            for (int e = threadIdx.x; e < nv; e += G * T) dst[e] = src[e];
+        }
+        __syncthreads();
+        auto gl = [&](int n) -> float2 {
+            if (!valid) return make_float2(0.f, 0.f);
+            const int x = mk_src(n, N);
+            return make_float2(mode == 0 ? rhs_staged(st, N, x, y, y0, sincos) : st[(y - y0 + 1) * N + x], 0.f);
+        };
+        Plan<N>::template run<-1>(t, s, tw_n, gl, gs);
+    } else {
+        auto gl = [&](int n) -> float2 {
+            if (!valid) return make_float2(0.f, 0.f);
+            const int x = mk_src(n, N);
+            return make_float2(mode == 0 ? rhs_at(map, N, x, y, sincos) : map[x + N * y], 0.f);
+        };
+        Plan<N>::template run<-1>(t, s, tw_n, gl, gs);
+    }
     __syncthreads();
     if (!valid) return;
     const float s0 = sqrtf(1.f / N), s1 = sqrtf(2.f / N);
