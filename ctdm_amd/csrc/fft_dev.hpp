@@ -197,6 +197,67 @@ __device__ __forceinline__ float2 twiddle(const TW& tw, int m)
     return DIR > 0 ? make_float2(w.x, -w.y) : w;
 }
 
+// Input of a transform already in registers: the first stage of a radix-R0 transform of length N by T
+// threads reads element j + r N/R0 (j = t + p T < N/R0) into its slot (p, r). A caller that loads those
+// elements itself (to issue the loads ahead of other work) passes them as a RegFile in place of the load
+// functor.
+template <int P_, int R_>
+struct RegFile {
+    static constexpr int P = P_, R = R_;
+    float2 v[P][R];
+};
+template <class X>
+struct IsRegFile {
+    static constexpr bool value = false;
+};
+template <int P, int R>
+struct IsRegFile<RegFile<P, R>> {
+    static constexpr bool value = true;
+};
+template <int N, int T, int R0>
+using StageRegs = RegFile<(N / R0 + T - 1) / T, R0>;
+
+// Loads, for thread t of a transform, the elements its first stage reads (element j + r N/R0 at
+// p[(j + r N/R0) * stride], j = t + q T) into a RegFile.
+template <int N, int T, int R0, class F>
+__device__ __forceinline__ void load_first_stage(F& f, const float2* __restrict__ p, long long stride, int t)
+{
+    constexpr int S0 = N / R0;
+#pragma unroll
+    for (int q = 0; q < F::P; ++q) {
+        const int j = t + q * T;
+        if (S0 % T == 0 || j < S0) {
+#pragma unroll
+            for (int r = 0; r < F::R; ++r) f.v[q][r] = p[(long long)(j + r * S0) * stride];
+        }
+    }
+}
+
+// A per-N twiddle table copied into LDS in two steps by a workgroup of NT threads: fetch() loads this
+// thread's entries into registers, put() writes them to LDS (padded like stage_twiddles). A kernel that
+// issues its data loads between the two keeps them from queuing behind the wait for the table.
+template <int N, int NT>
+struct TwStager {
+    static constexpr int K = (N + NT - 1) / NT;
+    float2 r[K];
+    __device__ __forceinline__ void fetch(const float2* __restrict__ src)
+    {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = (int)threadIdx.x + k * NT;
+            if (N % NT == 0 || i < N) r[k] = src[i];
+        }
+    }
+    __device__ __forceinline__ void put(float2* dst) const
+    {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = (int)threadIdx.x + k * NT;
+            if (N % NT == 0 || i < N) dst[lds_pad(i)] = r[k];
+        }
+    }
+};
+
 // Cooperative N-point transform by T threads of one transform; threads with t >= T only take part
 // in the barriers (so a workgroup sized for a longer transform can run a shorter one).
 // gload(i) supplies input element i (called in the first stage), gstore(i, v) receives output
@@ -212,6 +273,8 @@ struct Stockham {
         constexpr int P = (S + T - 1) / T;
         constexpr bool FIRST = (L == 1);
         constexpr bool LAST = (sizeof...(Rest) == 0);
+        constexpr bool REG_IN = FIRST && IsRegFile<GLoad>::value;
+        if constexpr (REG_IN) static_assert(GLoad::P == P && GLoad::R == R, "register input must match the stage");
         float2 v[P][R];
 #pragma unroll
         for (int p = 0; p < P; ++p) {
@@ -222,7 +285,9 @@ struct Stockham {
                 const float2* sj = s + (S % 16 == 0 ? lds_pad(j) : j);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    if constexpr (FIRST)
+                    if constexpr (REG_IN)
+                        v[p][r] = gload.v[p][r];
+                    else if constexpr (FIRST)
                         v[p][r] = gload(j + r * S);
                     else if constexpr (S % 16 == 0 || 16 % S == 0)
                         v[p][r] = sj[lds_pad(r * S)];
@@ -287,6 +352,7 @@ struct Plan;
 template <>
 struct Plan<32> {
     static constexpr int T = 4;
+    static constexpr int R0 = 8;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -296,6 +362,7 @@ struct Plan<32> {
 template <>
 struct Plan<64> {
     static constexpr int T = 8;
+    static constexpr int R0 = 8;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -305,6 +372,7 @@ struct Plan<64> {
 template <>
 struct Plan<128> {
     static constexpr int T = 16;
+    static constexpr int R0 = 8;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -314,6 +382,7 @@ struct Plan<128> {
 template <>
 struct Plan<192> {
     static constexpr int T = 16;
+    static constexpr int R0 = 12;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -323,6 +392,7 @@ struct Plan<192> {
 template <>
 struct Plan<256> {
     static constexpr int T = 16;
+    static constexpr int R0 = 16;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -332,6 +402,7 @@ struct Plan<256> {
 template <>
 struct Plan<384> {
     static constexpr int T = 16;
+    static constexpr int R0 = 3;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -341,6 +412,7 @@ struct Plan<384> {
 template <>
 struct Plan<512> {  // three radix-8 stages keep all 64 threads busy (16 x 32 leaves half idle in its second stage)
     static constexpr int T = 64;
+    static constexpr int R0 = 8;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -350,6 +422,7 @@ struct Plan<512> {  // three radix-8 stages keep all 64 threads busy (16 x 32 le
 template <>
 struct Plan<1024> {
     static constexpr int T = 32;
+    static constexpr int R0 = 32;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -365,6 +438,7 @@ struct PlanAlt;
 template <>
 struct PlanAlt<32> {
     static constexpr int T = Plan<32>::T;
+    static constexpr int R0 = 4;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -374,6 +448,7 @@ struct PlanAlt<32> {
 template <>
 struct PlanAlt<64> {
     static constexpr int T = Plan<64>::T;
+    static constexpr int R0 = 4;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -383,6 +458,7 @@ struct PlanAlt<64> {
 template <>
 struct PlanAlt<128> {
     static constexpr int T = Plan<128>::T;
+    static constexpr int R0 = 16;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -392,6 +468,7 @@ struct PlanAlt<128> {
 template <>
 struct PlanAlt<192> {
     static constexpr int T = Plan<192>::T;
+    static constexpr int R0 = 16;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -401,6 +478,7 @@ struct PlanAlt<192> {
 template <>
 struct PlanAlt<256> {
     static constexpr int T = Plan<256>::T;
+    static constexpr int R0 = 4;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {
@@ -410,6 +488,7 @@ struct PlanAlt<256> {
 template <>
 struct PlanAlt<512> {
     static constexpr int T = Plan<512>::T;
+    static constexpr int R0 = 8;  // first radix
     template <int DIR, class TW, class GL, class GS>
     __device__ __forceinline__ static void run(int t, float2* s, const TW& tw, GL& gl, GS& gs)
     {

@@ -584,13 +584,20 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_x(float2* __restrict__ W,
                                                          const float2* __restrict__ tw, Proj p, float scale,
                                                          int shift, int rev)
 {
+    constexpr int T = Plan<V>::T;
     __shared__ float2 lds[B * LdsSize<V>::value];
     __shared__ float2 twl[LdsSize<V>::value];
-    stage_twiddles<V>(twl, tw);
-    const TwPad twp{twl};
+    TwStager<V, B * T> tws;
+    tws.fetch(tw);
     Tile<V, 0, B> tl(rev);
     float2* s = lds + tl.b * LdsSize<V>::value;
     float2* row = W + tl.base;
+    // modes 1 and 2: the row's elements are loaded before the table's LDS copy waits for the table
+    StageRegs<V, T, Plan<V>::R0> rin;
+    if constexpr (MODE == 1 || MODE == 2) load_first_stage<V, T, Plan<V>::R0>(rin, row, 1, tl.t);
+    __builtin_amdgcn_sched_barrier(0);
+    tws.put(twl);
+    const TwPad twp{twl};
     // destination row of the final n (octant shift)
     const int h = shift ? V / 2 : 0;
     const long long orow = ((long long)((tl.px + h) % V) + (long long)V * ((tl.py + h) % V)) * V;
@@ -602,24 +609,26 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_x(float2* __restrict__ W,
     } else if constexpr (MODE == 3) {
         for (int x = tl.t; x < V; x += Plan<V>::T) put_n(x, est_n(row[x], p));
     } else {
-        auto gl = [&](int k) -> float2 { return row[k]; };
         if constexpr (MODE == 1) {
-            auto gs = [&](int x, float2 v) { s[lds_pad(x)] = project(v, p, scale); };
-            Plan<V>::template run<+1>(tl.t, s, twp, gl, gs);
+            auto gs = [&](int k, float2 v) { s[lds_pad(k)] = project(v, p, scale); };
+            Plan<V>::template run<+1>(tl.t, s, twp, rin, gs);
             __syncthreads();
-            auto gl2 = [&](int x) -> float2 { return s[lds_pad(x)]; };
+            auto gl2 = [&](int k) -> float2 { return s[lds_pad(k)]; };
             auto gs2 = [&](int k, float2 v) { row[k] = v; };
             Plan<V>::template run<-1>(tl.t, s, twp, gl2, gs2);
         } else {
-            auto gs = [&](int x, float2 v) { put_n(x, est_n(cscale(v, scale), p)); };
-            Plan<V>::template run<+1>(tl.t, s, twp, gl, gs);
+            auto gs = [&](int k, float2 v) { put_n(k, est_n(cscale(v, scale), p)); };
+            Plan<V>::template run<+1>(tl.t, s, twp, rin, gs);
         }
     }
 }
 
 // Z pass with data consistency: forward z-FFT, replace measured voxels, inverse z-FFT.
-// The pencil's list entries are fetched at kernel entry (DC_PRE per thread, independent of the
-// count) so their latency overlaps the data loads; longer lists finish in a loop.
+// Load order at kernel entry: the twiddle table and the pencil's list count, then the pencil's elements
+// (into the registers of the transform's first stage), then the list entries below the count (DC_PRE
+// per thread; longer lists finish in a loop) and the table's LDS copy. Those wait for the count and the
+// table only, so the data loads are in flight meanwhile, and no entry beyond the count is fetched (a
+// pencil's list averages a third of its capacity).
 // With tau != 1 a measured voxel moves only part of the way: F <- F + tau (m - F), the fit's
 // hit-preconditioned gradient step (see fit_pgd); tau == 1 is the exact replacement.
 // JT (the fit with its objective): before the step, each measured voxel adds h |F - m|^2 / 2 (h its
@@ -634,31 +643,36 @@ __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_zdc(float2* __restrict__ 
     __shared__ float2 lds[B * LS];
     __shared__ float2 twl[LdsSize<V>::value];
     __shared__ double red[JT ? B * T : 1];
-    stage_twiddles<V>(twl, tw);
-    const TwPad twp{twl};
+    TwStager<V, B * T> tws;
+    tws.fetch(tw);
     // list prefetch: lanes walk the entries of one pencil (contiguous in the list)
     const int pb = threadIdx.x / T, k0 = threadIdx.x % T;
     const int blk = Tile<V, 2, B>::block();
     const long long dpencil = (long long)(blk % (V / B)) * B + pb + (long long)V * (blk / (V / B));
     const int n = min(dc.cnt[dpencil], dc.cap);  // a count never exceeds the list capacity
-    short pz[DC_PRE], ph[DC_PRE];
+    Tile<V, 2, B> tl;
+    float2* s = lds + tl.b * LS;
+    float2* pen = W + tl.base;
+    StageRegs<V, T, Plan<V>::R0> x;
+    load_first_stage<V, T, Plan<V>::R0>(x, pen, tl.stride, tl.t);
+    __builtin_amdgcn_sched_barrier(0);  // keeps the data loads ahead of the waits for the count and the table
+    tws.put(twl);
+    const TwPad twp{twl};
+    // Branch-free: lanes past the count read the pencil's first entry (a line the others fetch anyway),
+    // and the entries are used only below the count.
+    int pz[DC_PRE], ph[DC_PRE];
     float2 pv[DC_PRE];
 #pragma unroll
     for (int r = 0; r < DC_PRE; ++r) {
         const int k = k0 + r * T;
-        const bool in = k < dc.cap;
-        const long long e = dpencil * dc.cap + (in ? k : 0);
+        const long long e = dpencil * dc.cap + (k < n ? k : 0);
         pz[r] = dc.idx[e];
         pv[r] = dc.val[e];
         if constexpr (JT) ph[r] = dc.hit[e];
     }
     double jacc = 0.0;
-    Tile<V, 2, B> tl;
-    float2* s = lds + tl.b * LS;
-    float2* pen = W + tl.base;
-    auto gl = [&](int z) -> float2 { return pen[(long long)z * tl.stride]; };
     auto gs = [&](int k, float2 v) { s[lds_pad(k)] = v; };
-    Plan<V>::template run<-1>(tl.t, s, twp, gl, gs);
+    Plan<V>::template run<-1>(tl.t, s, twp, x, gs);
     __syncthreads();
     {
         float2* sp = lds + pb * LS;
@@ -701,20 +715,25 @@ template <int V, int B, int DIR, int PV>
 __global__ __launch_bounds__(B* Plan<V>::T) void k_vol_y(float2* __restrict__ W, const float2* __restrict__ tw,
                                                          int rev)
 {
+    using PL = std::conditional_t<PV == 0, Plan<V>, PlanAlt<V>>;
+    constexpr int T = PL::T;
     constexpr int LS = LdsStride<V, B>::value;
     __shared__ float2 lds[B * LS];
     __shared__ float2 twl[LdsSize<V>::value];
-    stage_twiddles<V>(twl, tw);
-    const TwPad twp{twl};
+    // twiddle table, then the pencil's elements, then the table's LDS copy: the data loads are issued
+    // before the first wait for the table (the first stage's barrier publishes it)
+    TwStager<V, B * T> tws;
+    tws.fetch(tw);
     Tile<V, 1, B> tl(rev);
     float2* s = lds + tl.b * LS;
     float2* pen = W + tl.base;
-    auto gl = [&](int y) -> float2 { return pen[(long long)y * tl.stride]; };
+    StageRegs<V, T, PL::R0> x;
+    load_first_stage<V, T, PL::R0>(x, pen, tl.stride, tl.t);
+    __builtin_amdgcn_sched_barrier(0);
+    tws.put(twl);
+    const TwPad twp{twl};
     auto gs = [&](int k, float2 v) { pen[(long long)k * tl.stride] = v; };
-    if constexpr (PV == 0)
-        Plan<V>::template run<DIR>(tl.t, s, twp, gl, gs);
-    else
-        PlanAlt<V>::template run<DIR>(tl.t, s, twp, gl, gs);
+    PL::template run<DIR>(tl.t, s, twp, x, gs);
 }
 
 // Compact list from a dense measured spectrum (positivity API): one thread per pencil.
