@@ -189,6 +189,11 @@ void unwrap_dct_batched(Ctx& c, const float* wrapped, float* unwrapped, int batc
 void unwrap_dct_plain_one(Ctx& c, const float* wrapped, float* unwrapped, int M, unsigned flags);
 void unwrap_dct_adjoint(Ctx& c, const float* wrapped, const float* g_out, float* g_in, int batch,
                         int M, unsigned flags);
+// unwrap_dct_batched followed by the row pass of rytov_spectrum in one sequence: the last unwrapping
+// pass also transforms each row of u = logamp + i phi into P (batch M x M complex); `unwrapped` may be
+// null. Flags without CTDM_UNWRAP_REFCOMPAT_FAST and CTDM_UNWRAP_REFCOMPAT_IDCT.
+void unwrap_dct_rytov_rows(Ctx& c, const float* wrapped, float* unwrapped, const float* logamp, int batch, int M,
+                           unsigned flags, float2* P);
 // Allocation-free entry point: device scratch address per device and the unwrap tables (context set-up),
 // then solves with the fixed scratch array (no allocation, no table building).
 void noalloc_setup(Ctx& c);
@@ -210,6 +215,8 @@ std::vector<AngleRow> angle_rows(const EwaldGeom& g, const ctdm_angle* ang, int 
 
 void rytov_spectrum(Ctx& c, const float* unwrapped, const float* logamp, int A, int M,
                     float2* spec);
+// The column pass of rytov_spectrum alone, from the row spectra P that unwrap_dct_rytov_rows wrote.
+void rytov_spectrum_cols(Ctx& c, const float2* P, int A, int M, float2* spec);
 void rytov_spectrum_plain_one(Ctx& c, const float* phase, const float* amp, int M, float2* u_sp);
 void ewald_scatter(Ctx& c, const float2* spec, const ctdm_angle* ang, int A, int M, int V,
                    const ctdm_optics& o, float2* F, float* hits);

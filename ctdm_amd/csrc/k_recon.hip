@@ -179,6 +179,29 @@ __global__ __launch_bounds__(B* Plan<M>::T) void k_rytov_cols(const float2* __re
 }
 
 template <int M>
+static void rytov_cols_launch(Ctx& c, const float2* P, int A, float2* spec)
+{
+    constexpr int B = (256 / Plan<M>::T) < M ? (256 / Plan<M>::T) : M;  // columns per workgroup, <= M
+    hipLaunchKernelGGL((k_rytov_cols<M, B>), dim3(A * (M / B)), dim3(B * Plan<M>::T), 0, c.stream, P, A,
+                       c.tw(M), spec);
+    CTDM_CHECK_LAUNCH("k_rytov_cols");
+}
+
+void rytov_spectrum_cols(Ctx& c, const float2* P, int A, int M, float2* spec)
+{
+    CTDM_REQUIRE(P && spec, "rytov_spectrum_cols: null pointer");
+    if (A == 0) return;
+    switch (M) {
+    case 32: rytov_cols_launch<32>(c, P, A, spec); break;
+    case 64: rytov_cols_launch<64>(c, P, A, spec); break;
+    case 128: rytov_cols_launch<128>(c, P, A, spec); break;
+    case 192: rytov_cols_launch<192>(c, P, A, spec); break;
+    case 256: rytov_cols_launch<256>(c, P, A, spec); break;
+    default: fail(CTDM_EUNSUPPORTED, "rytov: size has no kernel");
+    }
+}
+
+template <int M>
 static void rytov_launch(Ctx& c, const float* phi, const float* la, int A, float2* spec)
 {
     constexpr int G = 256 / Plan<M>::T;

@@ -85,8 +85,8 @@ __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
     const int row0 = blockIdx.x * G;  // over nimg * H rows
     const int img_u = row0 / H;
     if (img_u >= nimg) return;
-    stage_twiddles<NH>(twl, tw_half);
-    const TwPad twp{twl};
+    TwStager<NH, G * T> tws;  // table loads first, its LDS copy after the row's loads are issued
+    tws.fetch(tw_half);
     const int g = threadIdx.x / T, t = threadIdx.x % T;
     const long long img = img_u;
     const int y = row0 - img_u * H + g;
@@ -108,9 +108,13 @@ __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
         if (k < 0) k += W;
         wq[j] = tw_full[k];
     }
-    auto gl = [&](int i) -> float2 { return src2[i]; };
+    StageRegs<NH, T, Plan<NH>::R0> x;
+    load_first_stage<NH, T, Plan<NH>::R0>(x, src2, 1, t);
+    __builtin_amdgcn_sched_barrier(0);
+    tws.put(twl);
+    const TwPad twp{twl};
     auto gs = [&](int i, float2 v) { s[lds_pad(i)] = v; };
-    Plan<NH>::template run<-1>(t, s, twp, gl, gs);
+    Plan<NH>::template run<-1>(t, s, twp, x, gs);
     __syncthreads();
     float2* dst = out + ((long long)img * H + y) * nout;
     auto emit = [&](int q, float2 w) {

@@ -301,6 +301,49 @@ __global__ __launch_bounds__(G* Plan<LN>::T) void k_uw_rows_inv(const float* __r
     }
 }
 
+// Rows of the last unwrapping pass fused with the row pass of the Rytov spectrum (k_rytov_rows of the
+// reconstruction module, same transform and twiddles): each row's DCT-III result, after the congruence
+// step, goes to the unwrapped map (when wanted) and stays in the row's LDS area as floats (the DCT-III's
+// last stage writes only after its barrier), where the forward row transform of u = logamp + i phi
+// reads it. Row spectra: P[r][k].
+template <int N, int G>
+__global__ __launch_bounds__(G* Plan<N>::T) void k_uw_rows_inv_rytov(const float* __restrict__ in, int batch,
+                                                                      unsigned flags,
+                                                                      const float* __restrict__ wrapped,
+                                                                      const float* __restrict__ la,
+                                                                      const float2* __restrict__ tw_n,
+                                                                      const float2* __restrict__ tw_4n,
+                                                                      float* __restrict__ out,
+                                                                      float2* __restrict__ P)
+{
+    constexpr int TL = Plan<N>::T;
+    static_assert(2 * LdsSize<N>::value >= N, "the phase row must fit the complex area");
+    __shared__ float2 lds[G * LdsSize<N>::value];
+    const int g = threadIdx.x / TL, t = threadIdx.x % TL;
+    const long long row = (long long)blockIdx.x * G + g;
+    const bool valid = row < (long long)batch * N;
+    const long long r = valid ? row : 0;
+    float2* s = lds + g * LdsSize<N>::value;
+    float* ph = reinterpret_cast<float*>(s);
+    const float* X = in + r * N;
+    const bool cong = (flags & F_CONGRUENT) && wrapped;
+    auto put = [&](int x, float v) {
+        if (cong) {
+            const float w = wrapped[r * N + x];
+            v = w + 2.f * kPI * rintf((v - w) / (2.f * kPI));
+        }
+        ph[x] = v;
+        if (valid && out) out[r * N + x] = v;
+    };
+    Dct3<N>::exact(t, X, s, tw_n, tw_4n, put);
+    __syncthreads();
+    auto gl = [&](int x) -> float2 { return make_float2(la[r * N + x], ph[x]); };
+    auto gs = [&](int k, float2 v) {
+        if (valid) P[r * N + k] = v;
+    };
+    Plan<N>::template run<-1>(t, s, tw_n, gl, gs);
+}
+
 // Adjoint helper: (weighted) zero-flux Laplacian, g_in = sum_nbr w_pn (v_n - v_p); weights
 // cos(phi_n - phi_p) for SINCOS, 1 otherwise.
 __global__ void k_uw_stencil(const float* __restrict__ v, const float* __restrict__ phi, int batch,
@@ -555,6 +598,39 @@ void unwrap_dct_batched(Ctx& c, const float* wrapped, float* unwrapped, int batc
         return;
     }
     solve_any(c, wrapped, batch, M, flags, 0, wrapped, unwrapped);
+}
+
+template <int N>
+static void uw_rytov(Ctx& c, const float* wrapped, float* unwrapped, const float* logamp, int batch, unsigned flags,
+                     float2* P)
+{
+    constexpr int G = 256 / Plan<N>::T;
+    float* D1 = (float*)c.buf("uw.D1", sizeof(float) * (size_t)batch * N * N);
+    float* D2 = (float*)c.buf("uw.D2", sizeof(float) * (size_t)batch * N * N);
+    run_rows_fwd<N>(c, wrapped, batch, 0, flags & F_SINCOS, D1);
+    run_cols<N, N>(c, D1, batch, flags, D2, false);
+    hipLaunchKernelGGL((k_uw_rows_inv_rytov<N, G>), dim3(grid1d((long long)batch * N, G)), dim3(G * Plan<N>::T), 0,
+                       c.stream, D2, batch, flags, wrapped, logamp, c.tw(N), c.tw(4 * N), unwrapped, P);
+    CTDM_CHECK_LAUNCH("k_uw_rows_inv_rytov");
+}
+
+void unwrap_dct_rytov_rows(Ctx& c, const float* wrapped, float* unwrapped, const float* logamp, int batch, int M,
+                           unsigned flags, float2* P)
+{
+    CTDM_REQUIRE(wrapped && logamp && P, "unwrap_dct_rytov_rows: null pointer");
+    CTDM_REQUIRE(batch >= 0 && M > 0, "unwrap_dct_rytov_rows: bad size");
+    CTDM_REQUIRE(wrapped != unwrapped, "unwrap_dct_rytov_rows: in-place call not supported");
+    CTDM_REQUIRE((flags & ~F_ALL) == 0 && !(flags & (F_REF_FAST | F_REF_IDCT)),
+                 "unwrap_dct_rytov_rows: unsupported flag");
+    if (batch == 0) return;
+    switch (M) {
+    case 32: uw_rytov<32>(c, wrapped, unwrapped, logamp, batch, flags, P); break;
+    case 64: uw_rytov<64>(c, wrapped, unwrapped, logamp, batch, flags, P); break;
+    case 128: uw_rytov<128>(c, wrapped, unwrapped, logamp, batch, flags, P); break;
+    case 192: uw_rytov<192>(c, wrapped, unwrapped, logamp, batch, flags, P); break;
+    case 256: uw_rytov<256>(c, wrapped, unwrapped, logamp, batch, flags, P); break;
+    default: fail(CTDM_EUNSUPPORTED, "unwrap size " + std::to_string(M) + " has no kernel");
+    }
 }
 
 void unwrap_dct_adjoint(Ctx& c, const float* wrapped, const float* g_out, float* g_in, int batch,

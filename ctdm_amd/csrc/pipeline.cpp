@@ -218,23 +218,31 @@ static void enqueue_batched(Ctx& c, ctdm_cell& cl, int i, const double* rows)
     const size_t n = (size_t)A * V * V;
     float* wr = cl.wrapped_opt ? cl.wrapped_opt : (float*)c.buf("bt.wr", sizeof(float) * n);
     float* la = cl.logamp_opt ? cl.logamp_opt : (float*)c.buf("bt.la", sizeof(float) * n);
-    float* un = cl.unwrapped_opt ? cl.unwrapped_opt : (float*)c.buf("bt.un", sizeof(float) * n);
     float2* spec = (float2*)c.buf("bt.spec", sizeof(float2) * n);
     if (cl.recon_flags & CTDM_RETRIEVE_ROWS)
         retrieve_rows(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.crop.col, wr, la);
     else
         retrieve_batched_dev(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.V, crop_slot(c, i), wr, la, nullptr);
-    unwrap_dct_batched(c, wr, un, A, V, cl.unwrap_flags);
-    const float* un_r = un;
-    const float* la_r = la;
-    if (cl.recon_flags & CTDM_MAPS_MEDIAN) {  // BatchRecon's median + checkMaps on the maps it receives
-        float* un2 = (float*)c.buf("bt.unm", sizeof(float) * n);
-        float* la2 = (float*)c.buf("bt.lam", sizeof(float) * n);
-        prepare_maps(c, un, la, un2, la2, A, V, CTDM_MAPS_MEDIAN);
-        un_r = un2;
-        la_r = la2;
+    if (!(cl.recon_flags & CTDM_MAPS_MEDIAN) &&
+        !(cl.unwrap_flags & (CTDM_UNWRAP_REFCOMPAT_FAST | CTDM_UNWRAP_REFCOMPAT_IDCT))) {
+        // the unwrapping's last pass also runs the Rytov row transform; the map goes out only when asked for
+        float2* P = (float2*)c.buf("ry.P", sizeof(float2) * n);
+        unwrap_dct_rytov_rows(c, wr, cl.unwrapped_opt, la, A, V, cl.unwrap_flags, P);
+        rytov_spectrum_cols(c, P, A, V, spec);
+    } else {
+        float* un = cl.unwrapped_opt ? cl.unwrapped_opt : (float*)c.buf("bt.un", sizeof(float) * n);
+        unwrap_dct_batched(c, wr, un, A, V, cl.unwrap_flags);
+        const float* un_r = un;
+        const float* la_r = la;
+        if (cl.recon_flags & CTDM_MAPS_MEDIAN) {  // BatchRecon's median + checkMaps on the maps it receives
+            float* un2 = (float*)c.buf("bt.unm", sizeof(float) * n);
+            float* la2 = (float*)c.buf("bt.lam", sizeof(float) * n);
+            prepare_maps(c, un, la, un2, la2, A, V, CTDM_MAPS_MEDIAN);
+            un_r = un2;
+            la_r = la2;
+        }
+        rytov_spectrum(c, un_r, la_r, A, V, spec);
     }
-    rytov_spectrum(c, un_r, la_r, A, V, spec);
     volume_batched(c, spec, rows, A, V, cl.optics, cl.iters, cl.recon_flags, (float2*)cl.ri);
 }
 
