@@ -182,6 +182,7 @@ ctdm_status ctdm_create(ctdm_ctx** out, const ctdm_config* cfg, ctdm_stream stre
     if (hipGetDeviceProperties(&prop, ctx->c.device) == hipSuccess) {
         ctx->c.max_lds = (int)prop.sharedMemPerBlock;
         ctx->c.arch = prop.gcnArchName;
+        ctx->c.cus = prop.multiProcessorCount;
     }
     const ctdm_status st = guarded(ctx, [](ctdm::Ctx& c) { ctdm::noalloc_setup(c); });
     if (st != CTDM_OK) {

@@ -107,6 +107,7 @@ struct Ctx {
     bool capturing = false;
     float* noalloc_scratch = nullptr;  // this device's address of the allocation-free entry point's array
     int max_lds = 65536;
+    int cus = 0;  // compute units of the device (0: unknown)
     int dc_V = 0;    // volume side the cached data-consistency list ("dc.*") was built for
     int dc_cap = 0;  // its entries per pencil
 

@@ -10,9 +10,12 @@
 //      wavefront per map, charges at 2 bits per pixel, in LDS when they fit, else in global memory)
 //      walks the residues in order and takes the first listed hit still present, and k_gs_draw draws
 //      the chosen cuts in parallel.
-//   3. k_gs_bfs       one workgroup per map: level-synchronous breadth-first flood fill over
-//                     non-cut pixels; after a level is discovered each new pixel takes k from its
-//                     first neighbour of the previous level in the order right, left, below, above.
+//   3. flood fill: level-synchronous breadth-first search over non-cut pixels from a fixed start; each
+//                     new pixel takes k from its first neighbour of the previous level in the order
+//                     right, left, below, above. Maps whose masks fit in LDS: k_gs_bfs_par (one
+//                     workgroup per map, one barrier per level) records each pixel's parent, and
+//                     k_gs_ptr_init / k_gs_ptr_jump (whole device) sum the steps along the parent chains
+//                     by pointer jumping. Larger maps: k_gs_bfs, with k along the levels.
 //   4. k_gs_finish    one thread per pixel: interior cut pixels next to a reached pixel take k from
 //                     it (same order); unwrapped = phi + 2 pi k, or 0 where unreached.
 // Float operations that decide integers are written as single rounded operations and this file is
@@ -528,31 +531,42 @@ __global__ __launch_bounds__(1024) void k_gs_bfs(const float* __restrict__ phi, 
     }
 }
 
-// Whole-map flood fill for maps whose bitmasks fit in LDS (cut, visited, and membership of the newest
-// level: 3 bits per pixel). Same levels and parents as k_gs_bfs. The frontier queues live in LDS too
-// (Q entries each; entries beyond Q spill to the global queues, so any frontier size is handled); k
-// stays in global memory. Per level d: (A) expand the current frontier, marking new pixels visited
-// and members of level d+1, while the level-d marks leave the level mask; (B) each new pixel takes k
-// from its first neighbour (right, left, below, above) of level d: a visited neighbour outside level
-// d+1 is of level d, since neighbouring levels differ by at most one. The queue counters rotate over
-// three slots, so a level needs two barriers.
+// Bit p of a bit mask.
 __device__ __forceinline__ bool tbit(const unsigned* b, long long p) { return (b[p >> 5] >> (p & 31)) & 1u; }
 
-__global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ phi, const uint8_t* __restrict__ cuts,
-                                                     int H, int W, int Q, int* __restrict__ kk, int* __restrict__ qa,
-                                                     int* __restrict__ qb)
+// ---------------------------------------------------------------- 3b. flood fill in two stages
+// For maps whose masks fit in LDS. Stage 1 (k_gs_bfs_par, one workgroup per map) is the level-synchronous
+// search of k_gs_bfs with the same levels and parents, except that each new pixel records only which
+// neighbour is its parent (the first of the previous level in the order right, left, below, above) as a
+// byte. Stage 2 (k_gs_ptr_init, k_gs_ptr_jump; the whole device) gives every reached pixel the pair
+// (parent, -step2pi(phi - phi_parent)) and contracts the parent chains by pointer jumping, (ptr, sum) <-
+// (ptr[ptr], sum + sum[ptr]), until each pointer is the start pixel, whose pair is (itself, 0). Then sum
+// is the pixel's 2 pi count: the same integers k_gs_bfs adds level by level, in another order (integer
+// sums do not depend on it). Unreached pixels keep ptr = -1. Parent codes: 0 unreached, 1..4 right, left,
+// below, above, 5 the start pixel.
+//
+// The search keeps, per pixel, 2 bits in LDS: 0 unvisited, else 1 + (level mod 3). Neighbouring pixels'
+// levels differ by at most one, so among the neighbours of a pixel of level d + 1 the code 1 + (d mod 3)
+// identifies level d exactly, and a set code means visited. A level is then one pass and one barrier: a
+// thread expanding a pixel of level d claims an unvisited neighbour q by setting q's code with an atomic
+// OR (the returned word tells whether q was still unvisited; racing claimants write the same code), and
+// the claimant reads q's parent from the codes of level d, which no one changes during the pass.
+__device__ __forceinline__ int code2(const unsigned* c, int p) { return (int)((c[p >> 4] >> ((p & 15) * 2)) & 3u); }
+
+__global__ __launch_bounds__(1024) void k_gs_bfs_par(const uint8_t* __restrict__ cuts, int H, int W, int Q,
+                                                     uint8_t* __restrict__ par, int* __restrict__ qa,
+                                                     int* __restrict__ qb, int* __restrict__ maxdepth)
 {
     extern __shared__ unsigned sm[];
     __shared__ int cnt[3];
     const long long n = (long long)H * W;
-    const int nb = (int)((n + 31) / 32);
+    const int nb = (int)((n + 31) / 32);   // cut bit words
+    const int nc2 = (int)((n + 15) / 16);  // 2-bit code words
     unsigned* cutb = sm;
-    unsigned* vis = sm + nb;
-    unsigned* lnext = sm + 2 * nb;
-    int* lq[2] = {reinterpret_cast<int*>(sm + 3 * nb), reinterpret_cast<int*>(sm + 3 * nb) + Q};
-    const float* ph = phi + blockIdx.x * n;
+    unsigned* code = sm + nb;
+    int* lq[2] = {reinterpret_cast<int*>(sm + nb + nc2), reinterpret_cast<int*>(sm + nb + nc2) + Q};
     const uint8_t* cut = cuts + blockIdx.x * n;
-    int* k = kk + blockIdx.x * n;
+    uint8_t* pc = par + blockIdx.x * n;  // all 0 (unreached) on entry
     int* gq[2] = {qa + blockIdx.x * n, qb + blockIdx.x * n};
     auto qget = [&](int which, int i) -> int { return i < Q ? lq[which][i] : ld_wg(gq[which] + i); };
     auto qput = [&](int which, int i, int v) {
@@ -568,10 +582,8 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ p
             if (p < n && cut[p]) v |= 1u << b;
         }
         cutb[w] = v;
-        vis[w] = 0u;
-        lnext[w] = 0u;
     }
-    for (long long p = threadIdx.x; p < n; p += blockDim.x) st_wg(k + p, INT_MIN);
+    for (int w = threadIdx.x; w < nc2; w += blockDim.x) code[w] = 0u;
     __syncthreads();
     if (threadIdx.x == 0) {
         int sx = W / 2, sy = min(20, H - 1);
@@ -582,9 +594,8 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ p
         cnt[0] = 0;
         if (sx >= 0 && sy < H) {
             const int s0 = sx + W * sy;
-            vis[s0 >> 5] |= 1u << (s0 & 31);
-            lnext[s0 >> 5] |= 1u << (s0 & 31);
-            st_wg(k + s0, 0);
+            code[s0 >> 4] |= 1u << ((s0 & 15) * 2);  // level 0: code 1
+            pc[s0] = 5;
             qput(0, 0, s0);
             cnt[0] = 1;
         }
@@ -593,49 +604,85 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lds(const float* __restrict__ p
     }
     __syncthreads();
     const int dx4[4] = {1, -1, 0, 0}, dy4[4] = {0, 0, 1, -1};
-    for (int d = 0;; ++d) {
+    int d = 0;
+    for (;; ++d) {
         const int c0 = d % 3, c1 = (d + 1) % 3, c2 = (d + 2) % 3;
         const int nc = cnt[c0];
         if (nc == 0) break;
         const int cur = d & 1, nxt = cur ^ 1;
-        if (threadIdx.x == 0) cnt[c2] = 0;  // the level after next appends there (read after two barriers)
-        for (int e = threadIdx.x; e < nc; e += blockDim.x) {  // (A)
-            const int p = qget(cur, e);
-            atomicAnd(lnext + (p >> 5), ~(1u << (p & 31)));  // level d leaves the level mask
-            const int x = p % W, y = p / W;
-            for (int t = 0; t < 4; ++t) {
-                const int nx = x + dx4[t], ny = y + dy4[t];
-                if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
-                const int q = nx + W * ny;
-                if (tbit(cutb, q)) continue;
-                const unsigned m = 1u << (q & 31);
-                if (atomicOr(vis + (q >> 5), m) & m) continue;
-                atomicOr(lnext + (q >> 5), m);
-                qput(nxt, reserve_slot(&cnt[c1]), q);
-            }
-        }
-        __syncthreads();
-        const int nn = cnt[c1];
-        for (int e = threadIdx.x; e < nn; e += blockDim.x) {  // (B)
-            const int q = qget(nxt, e);
-            const int x = q % W, y = q / W;
-            for (int t = 0; t < 4; ++t) {
-                const int nx = x + dx4[t], ny = y + dy4[t];
-                if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
-                const int pp = nx + W * ny;
-                if (tbit(vis, pp) && !tbit(lnext, pp)) {
-                    st_wg(k + q, ld_wg(k + pp) - gs_step2pi(__fsub_rn(ph[q], ph[pp])));
+        const unsigned cd = 1u + (unsigned)(d % 3), cn = 1u + (unsigned)((d + 1) % 3);  // codes of d, d + 1
+        if (threadIdx.x == 0) cnt[c2] = 0;  // the next level appends there after this pass's barrier
+        for (int e = threadIdx.x; e < 4 * nc; e += blockDim.x) {  // one (pixel, direction) per thread
+            const int p = qget(cur, e >> 2), t = e & 3;
+            const int nx = p % W + dx4[t], ny = p / W + dy4[t];
+            if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
+            const int q = nx + W * ny;
+            if (tbit(cutb, q) || code2(code, q)) continue;
+            const unsigned sh = (unsigned)(q & 15) * 2;
+            if ((atomicOr(code + (q >> 4), cn << sh) >> sh) & 3u) continue;  // claimed by another thread
+            for (int u = 0; u < 4; ++u) {  // the parent: first neighbour of q on level d
+                const int rx = nx + dx4[u], ry = ny + dy4[u];
+                if (rx < 0 || rx >= W || ry < 0 || ry >= H) continue;
+                if ((unsigned)code2(code, rx + W * ry) == cd) {
+                    pc[q] = (uint8_t)(1 + u);
                     break;
                 }
             }
+            qput(nxt, reserve_slot(&cnt[c1]), q);
         }
         __syncthreads();
+    }
+    if (threadIdx.x == 0 && d > 1) atomicMax(maxdepth, d - 1);  // longest parent chain of the map
+}
+
+// Pair (ptr, sum) per pixel, map-local ptr; the start pixel points to itself.
+__global__ void k_gs_ptr_init(const float* __restrict__ phi, const uint8_t* __restrict__ par, int batch, int H,
+                              int W, int2* __restrict__ pr)
+{
+    const long long n = (long long)H * W;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * batch) return;
+    const long long b = g / n;
+    const int p = (int)(g % n);
+    const int code = par[g];
+    int2 v = make_int2(-1, 0);
+    if (code == 5) {
+        v = make_int2(p, 0);
+    } else if (code >= 1 && code <= 4) {
+        const int pp = p + (code == 1 ? 1 : code == 2 ? -1 : code == 3 ? W : -W);
+        v = make_int2(pp, -gs_step2pi(__fsub_rn(phi[g], phi[b * n + pp])));
+    }
+    pr[g] = v;
+}
+
+// One pointer-jumping round, in place. Pairs are read and written as single 64-bit accesses, so every
+// read sees a consistent pair: an update keeps k(p) = sum(p) + k(ptr(p)) whichever round's pair of the
+// parent it reads, and a newer one only shortens the chain. Round r is needed only while 2^r is below
+// the longest chain (maxdepth, from the search); later rounds return at once.
+__global__ __launch_bounds__(256) void k_gs_ptr_jump(int2* __restrict__ pr, long long tot, long long n, int round,
+                                                     const int* __restrict__ maxdepth)
+{
+    if ((1LL << round) >= (long long)*maxdepth) return;
+    unsigned long long* a = reinterpret_cast<unsigned long long*>(pr);
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < tot;
+         g += (long long)gridDim.x * blockDim.x) {
+        const unsigned long long mine = __hip_atomic_load(a + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int P = (int)(unsigned)mine;
+        if (P < 0) continue;
+        const long long base = g - g % n;
+        const unsigned long long up = __hip_atomic_load(a + base + P, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int sum = (int)(mine >> 32) + (int)(up >> 32);
+        const unsigned long long nv = (up & 0xffffffffull) | ((unsigned long long)(unsigned)sum << 32);
+        if (nv != mine) __hip_atomic_store(a + g, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 // ---------------------------------------------------------------- 4. cut fill + outputs
+// k of a pixel: from the fill's int32 counts, or (PTR) from the contracted pairs (ptr < 0: unreached).
+template <bool PTR>
 __global__ void k_gs_finish(const float* __restrict__ phi, const uint8_t* __restrict__ cuts, const int* __restrict__ kk,
-                            int batch, int H, int W, int* __restrict__ kout, float* __restrict__ unwrapped)
+                            const int2* __restrict__ pr, int batch, int H, int W, int* __restrict__ kout,
+                            float* __restrict__ unwrapped)
 {
     const long long n = (long long)H * W;
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -643,16 +690,24 @@ __global__ void k_gs_finish(const float* __restrict__ phi, const uint8_t* __rest
     const long long b = g / n;
     const int p = (int)(g % n), x = p % W, y = p / W;
     const uint8_t* cut = cuts + b * n;
-    const int* k = kk + b * n;
     const float* ph = phi + b * n;
-    int kv = k[p];
+    auto kat = [&](int q) -> int {
+        if constexpr (PTR) {
+            const int2 v = pr[b * n + q];
+            return v.x < 0 ? INT_MIN : v.y;
+        } else {
+            return kk[b * n + q];
+        }
+    };
+    int kv = kat(p);
     if (cut[p] && x > 0 && x < W - 1 && y > 0 && y < H - 1 &&
         !(cut[p + 1] && cut[p - 1] && cut[p + W] && cut[p - W])) {
         const int nb[4] = {p + 1, p - 1, p + W, p - W};
         for (int t = 0; t < 4; ++t) {
             const int q = nb[t];
-            if (!cut[q] && k[q] != INT_MIN) {  // reached by the fill
-                kv = k[q] - gs_step2pi(__fsub_rn(ph[p], ph[q]));
+            const int kq = kat(q);
+            if (!cut[q] && kq != INT_MIN) {  // reached by the fill
+                kv = kq - gs_step2pi(__fsub_rn(ph[p], ph[q]));
                 break;
             }
         }
@@ -677,7 +732,6 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
     const long long n = (long long)H * W, tot = n * batch;
     int8_t* r = res ? res : (int8_t*)c.buf("gs.res", tot);
     uint8_t* ct = cuts ? cuts : (uint8_t*)c.buf("gs.cut", tot);
-    int* kk = (int*)c.buf("gs.k", sizeof(int) * tot);
     int* qa = (int*)c.buf("gs.qa", sizeof(int) * tot);
     int* qb = (int*)c.buf("gs.qb", sizeof(int) * tot);
     hipLaunchKernelGGL(k_gs_residues, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, batch, H, W, r);
@@ -708,18 +762,37 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
     hipLaunchKernelGGL(k_gs_draw, dim3((unsigned)((n + 255) / 256), (unsigned)batch), dim3(256), 0, c.stream,
                        pairs, npairs, H, W, ct);
     CTDM_CHECK_LAUNCH("k_gs_draw");
-    const size_t masks = sizeof(unsigned) * (size_t)(3 * nb);
+    const size_t masks = sizeof(unsigned) * (size_t)(nb + (n + 15) / 16);  // cut bits and 2-bit level codes
     if (masks + 64 + 2 * sizeof(int) * 256 <= lds_max) {  // the queues get what LDS has left (>= 256 each)
         const int Q = (int)std::min<long long>(n, (long long)((lds_max - 64 - masks) / (2 * sizeof(int))));
         const size_t bfs_lds = masks + 2 * sizeof(int) * (size_t)Q;
-        hipLaunchKernelGGL(k_gs_bfs_lds, dim3(batch), dim3(1024), bfs_lds, c.stream, wrapped, ct, H, W, Q, kk, qa, qb);
+        // two-stage fill: parent directions by the search, 2 pi counts by pointer jumping
+        uint8_t* par = (uint8_t*)c.buf("gs.par", (size_t)tot);
+        int2* pr = (int2*)c.buf("gs.pr", sizeof(int2) * (size_t)tot);
+        int* maxd = (int*)c.buf("gs.maxd", sizeof(int));
+        hipLaunchKernelGGL(k_gs_fill_u8, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, par, tot);
+        hipLaunchKernelGGL(k_gs_fill_u8, dim3(1), dim3(256), 0, c.stream, (uint8_t*)maxd, (long long)sizeof(int));
+        hipLaunchKernelGGL(k_gs_bfs_par, dim3(batch), dim3(1024), bfs_lds, c.stream, ct, H, W, Q, par, qa, qb, maxd);
+        CTDM_CHECK_LAUNCH("k_gs_bfs_par");
+        hipLaunchKernelGGL(k_gs_ptr_init, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, par, batch, H, W,
+                           pr);
+        // enough rounds for any chain in a map (2^rounds >= n); rounds past the longest chain return at once
+        int rounds = 0;
+        while ((1LL << rounds) < n) ++rounds;
+        const unsigned jg = (unsigned)std::min<long long>(grid1d(tot, 256), (long long)std::max(c.cus, 1) * 8);
+        for (int r = 0; r < rounds; ++r)
+            hipLaunchKernelGGL(k_gs_ptr_jump, dim3(jg), dim3(256), 0, c.stream, pr, tot, n, r, (const int*)maxd);
+        CTDM_CHECK_LAUNCH("k_gs_ptr_jump");
+        hipLaunchKernelGGL(k_gs_finish<true>, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, ct, nullptr,
+                           pr, batch, H, W, k2pi, unwrapped);
     } else {
+        int* kk = (int*)c.buf("gs.k", sizeof(int) * tot);
         int* lev = (int*)c.buf("gs.level", sizeof(int) * tot);
         hipLaunchKernelGGL(k_gs_bfs, dim3(batch), dim3(1024), 0, c.stream, wrapped, ct, H, W, lev, kk, qa, qb);
+        CTDM_CHECK_LAUNCH("k_gs_bfs");
+        hipLaunchKernelGGL(k_gs_finish<false>, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, ct, kk,
+                           nullptr, batch, H, W, k2pi, unwrapped);
     }
-    CTDM_CHECK_LAUNCH("k_gs_bfs");
-    hipLaunchKernelGGL(k_gs_finish, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, ct, kk, batch, H, W,
-                       k2pi, unwrapped);
     CTDM_CHECK_LAUNCH("k_gs_finish");
 }
 
