@@ -188,6 +188,7 @@ def main() -> int:
         log(f"cpu baseline {tc:.1f} s/volume, GPU-vs-oracle max|dn| {err:.2e}")
 
     config4 = None if args.no_config4 else run_config4(ctx, args)
+    ingest = None if args.no_extra else run_ingest(ctx, args, holo, bg, ang, optics)
     config1 = None if args.no_extra else run_config1(ctx, args)
     config3 = None if args.no_extra else run_config3(ctx, args)
     config5 = None if args.no_extra else run_config5(ctx, args)
@@ -204,7 +205,7 @@ def main() -> int:
         "speedup_vs_plain": plain_value and round(value / plain_value, 2),
         "ri_max_diff_batched_vs_plain": dn, "wall_ms_per_step": round(wall, 4),
         "roofline": roofline, "cpu_baseline": cpu, "config4": config4,
-        "config1": config1, "config3": config3, "config5": config5,
+        "config1": config1, "config3": config3, "config5": config5, "ingest": ingest,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -310,6 +311,68 @@ def _time_calls(fn, reps):
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def run_ingest(ctx, args, holo, bg, ang, optics):
+    """The headline cell with its frames as 8- and 16-bit camera pixels (the reference reads 8-bit BMP
+    frames, HilbertTransform.cu:419-420, 660-661) instead of float32: the same frames quantised, passed
+    once as integers and once as float32 of the same values (bit-identical volumes), batched and
+    graph-replayed like the headline; and the host-to-device upload of one cell's frames from pinned
+    memory in each format (the PCIe-inclusive cost the device-resident headline leaves out)."""
+    import numpy as np
+    import torch
+
+    import ctdm_amd
+
+    log("step: ingest (8- and 16-bit frames of the headline cell)")
+    out = {"workload": "the headline cell (60 x 1024^2 pairs -> 256^3, 10 iterations), frames quantised"}
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    for name, dtype, top in (("u8", np.uint8, 255), ("u16", np.uint16, 4095)):
+        scale = top / max(float(holo.max()), float(bg.max()))
+        hq = np.rint(np.clip(holo * scale, 0, top)).astype(dtype)
+        bq = np.rint(np.clip(bg * scale, 0, top)).astype(dtype)
+        res = {}
+        for fmt, h, b in ((name, hq, bq), ("f32", hq.astype(np.float32), bq.astype(np.float32))):
+            cell = dict(holo=torch.from_numpy(h).cuda(), bg=torch.from_numpy(b).cuda(), angles=ang, V=V,
+                        iters=ITERS, optics=optics, ri=torch.empty((V, V, V), dtype=torch.complex64, device="cuda"))
+            ctx.reconstruct([cell], mode=1, graph=True)
+            for _ in range(args.warmup):
+                ctx.reconstruct([cell], mode=1, graph=True)
+            torch.cuda.synchronize()
+            e0, e1 = ev(), ev()
+            e0.record()
+            for _ in range(args.steps):
+                ctx.reconstruct([cell], mode=1, graph=True)
+            e1.record()
+            torch.cuda.synchronize()
+            res[fmt] = (e0.elapsed_time(e1) / args.steps, cell["ri"])
+            # upload of the cell's frames from pinned host memory
+            ph = torch.from_numpy(h).pin_memory()
+            pb = torch.from_numpy(b).pin_memory()
+            dh, db = torch.empty_like(ph, device="cuda"), torch.empty_like(pb, device="cuda")
+            dh.copy_(ph, non_blocking=True)
+            torch.cuda.synchronize()
+            e0, e1 = ev(), ev()
+            e0.record()
+            for _ in range(3):
+                dh.copy_(ph, non_blocking=True)
+                db.copy_(pb, non_blocking=True)
+            e1.record()
+            torch.cuda.synchronize()
+            res[fmt] += (e0.elapsed_time(e1) / 3, 2 * ph.numel() * ph.element_size())
+            del ph, pb, dh, db, cell
+        ms_i, ri_i, up_i, nb_i = res[name]
+        ms_f, ri_f, up_f, nb_f = res["f32"]
+        same = bool(torch.equal(ri_i, ri_f))
+        log(f"ingest {name}: {ms_i:.3f} ms/vol (f32 of the same values {ms_f:.3f}), bit-identical {same}; "
+            f"upload {up_i:.2f} ms for {nb_i / 1e6:.0f} MB (f32 {up_f:.2f} ms)")
+        out[name] = {"ms_per_volume": round(ms_i, 4), "volumes_per_s": round(1000.0 / ms_i, 2),
+                     "f32_same_values_ms_per_volume": round(ms_f, 4), "ri_bit_identical_to_f32": same,
+                     "upload_ms": round(up_i, 3), "upload_bytes": nb_i, "f32_upload_ms": round(up_f, 3),
+                     "f32_upload_bytes": nb_f,
+                     "volumes_per_s_upload_then_compute": round(1000.0 / (ms_i + up_i), 2),
+                     "f32_volumes_per_s_upload_then_compute": round(1000.0 / (ms_f + up_f), 2)}
+    return out
 
 
 def run_config1(ctx, args):

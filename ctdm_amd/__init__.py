@@ -35,7 +35,11 @@ POS_NO_SHIFT = 0x2
 MAPS_MEDIAN = 0x4
 MAPS_AMPLITUDE = 0x8
 RETRIEVE_ROWS = 0x10
+HOLO_U8 = 0x20
+HOLO_U16 = 0x40
 EXEC_PLAIN = 0x100
+
+PIXEL_F32, PIXEL_U8, PIXEL_U16 = 0, 1, 2
 
 
 class CtdmError(RuntimeError):
@@ -101,6 +105,12 @@ SIGNATURES = {
     "ctdm_retrieve_rows": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp], C.c_int),
     "ctdm_angle_from_image": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                C.POINTER(C.c_double)], C.c_int),
+    "ctdm_find_sideband_px": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Crop)], C.c_int),
+    "ctdm_retrieve_px": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Crop), _vp, _vp, _vp], C.c_int),
+    "ctdm_find_sideband_rows_px": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "ctdm_retrieve_rows_px": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp], C.c_int),
+    "ctdm_angle_from_image_px": ([_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.POINTER(C.c_double)], C.c_int),
     "ctdm_unwrap_dct": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_uint], C.c_int),
     "ctdm_unwrap_dct_noalloc": ([_vp, _vp, _vp, C.c_int, C.c_int, C.c_uint], C.c_int),
     "ctdm_unwrap_dct_adjoint": ([_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_uint], C.c_int),
@@ -163,6 +173,22 @@ def _ptr(t):
     if not t.is_contiguous():
         raise ValueError("libctdm takes contiguous tensors")
     return t.data_ptr()
+
+
+def pixel_format(t) -> int:
+    """PIXEL_F32 / PIXEL_U8 / PIXEL_U16 of a hologram tensor or array (float32, uint8 or uint16)."""
+    name = str(t.dtype).replace("torch.", "")
+    fmt = {"float32": PIXEL_F32, "uint8": PIXEL_U8, "uint16": PIXEL_U16}.get(name)
+    if fmt is None:
+        raise ValueError(f"holograms must be float32, uint8 or uint16, not {t.dtype}")
+    return fmt
+
+
+def _pair_format(holo, bg) -> int:
+    px = pixel_format(holo)
+    if pixel_format(bg) != px:
+        raise ValueError("holograms and backgrounds must share one pixel format")
+    return px
 
 
 def angles_array(angles) -> C.Array:
@@ -323,7 +349,11 @@ class Context:
     def find_sideband(self, bg, M: int = 256, row_start: int = 0) -> Crop:
         H, W = bg.shape[-2:]
         crop = Crop()
-        self._call("ctdm_find_sideband", _ptr(bg), H, W, M, row_start, C.byref(crop))
+        px = pixel_format(bg)
+        if px == PIXEL_F32:
+            self._call("ctdm_find_sideband", _ptr(bg), H, W, M, row_start, C.byref(crop))
+        else:
+            self._call("ctdm_find_sideband_px", _ptr(bg), px, H, W, M, row_start, C.byref(crop))
         return crop
 
     def angle_from_image(self, img, pixel_um: float, mag: float, wavelength_nm: float, n_med: float):
@@ -331,13 +361,21 @@ class Context:
         im = img if img.dim() == 3 else img[None]
         B, H, W = im.shape
         out = (C.c_double * B)()
-        self._call("ctdm_angle_from_image", _ptr(im), B, H, W, pixel_um, mag, wavelength_nm, n_med, out)
+        px = pixel_format(im)
+        if px == PIXEL_F32:
+            self._call("ctdm_angle_from_image", _ptr(im), B, H, W, pixel_um, mag, wavelength_nm, n_med, out)
+        else:
+            self._call("ctdm_angle_from_image_px", _ptr(im), px, B, H, W, pixel_um, mag, wavelength_nm, n_med, out)
         return list(out) if img.dim() == 3 else out[0]
 
     def find_sideband_rows(self, bg) -> int:
         H, W = bg.shape[-2:]
         c = C.c_int()
-        self._call("ctdm_find_sideband_rows", _ptr(bg), H, W, C.byref(c))
+        px = pixel_format(bg)
+        if px == PIXEL_F32:
+            self._call("ctdm_find_sideband_rows", _ptr(bg), H, W, C.byref(c))
+        else:
+            self._call("ctdm_find_sideband_rows_px", _ptr(bg), px, H, W, C.byref(c))
         return c.value
 
     def retrieve_rows(self, holo, bg, center: int):
@@ -348,7 +386,12 @@ class Context:
         B, H, W = holo.shape
         wrapped = torch.empty((B, W // 4, H // 4), dtype=torch.float32, device=holo.device)
         logamp = torch.empty_like(wrapped)
-        self._call("ctdm_retrieve_rows", _ptr(holo), _ptr(bg), B, H, W, center, _ptr(wrapped), _ptr(logamp))
+        px = _pair_format(holo, bg)
+        if px == PIXEL_F32:
+            self._call("ctdm_retrieve_rows", _ptr(holo), _ptr(bg), B, H, W, center, _ptr(wrapped), _ptr(logamp))
+        else:
+            self._call("ctdm_retrieve_rows_px", _ptr(holo), _ptr(bg), px, B, H, W, center, _ptr(wrapped),
+                       _ptr(logamp))
         return wrapped, logamp
 
     def retrieve(self, holo, bg, crop: Crop, field: bool = False):
@@ -360,8 +403,13 @@ class Context:
         wrapped = torch.empty((B, M, M), dtype=torch.float32, device=holo.device)
         logamp = torch.empty_like(wrapped)
         fld = torch.empty((B, M, M), dtype=torch.complex64, device=holo.device) if field else None
-        self._call("ctdm_retrieve", _ptr(holo), _ptr(bg), B, H, W, C.byref(crop), _ptr(wrapped),
-                   _ptr(logamp), _ptr(fld))
+        px = _pair_format(holo, bg)
+        if px == PIXEL_F32:
+            self._call("ctdm_retrieve", _ptr(holo), _ptr(bg), B, H, W, C.byref(crop), _ptr(wrapped),
+                       _ptr(logamp), _ptr(fld))
+        else:
+            self._call("ctdm_retrieve_px", _ptr(holo), _ptr(bg), px, B, H, W, C.byref(crop), _ptr(wrapped),
+                       _ptr(logamp), _ptr(fld))
         return (wrapped, logamp, fld) if field else (wrapped, logamp)
 
     # -- unwrapping
@@ -483,8 +531,9 @@ class Context:
     # -- pipeline
     def reconstruct(self, cells: list[dict], mode: int = 1, graph: bool = False,
                     autotune: bool = False, lanes: int = 0):
-        """cells: dicts with holo, bg (device, A x H x W), angles (A x 2), V, iters, optics,
-        ri (device V^3 complex64) and optional crop / flags / wrapped / unwrapped / logamp.
+        """cells: dicts with holo, bg (device, A x H x W; float32, uint8 or uint16 pixels), angles
+        (A x 2), V, iters, optics, ri (device V^3 complex64) and optional crop / flags / wrapped /
+        unwrapped / logamp.
         lanes: batched multi-cell sequences run on two concurrent lanes unless lanes == 1."""
         arr = (Cell * len(cells))()
         keep = []
@@ -493,8 +542,9 @@ class Context:
             ang = angles_array(d["angles"])
             keep.append(ang)
             crop = d.get("crop") or Crop(0, 0, -1, d["V"])
+            holo_flag = {PIXEL_F32: 0, PIXEL_U8: HOLO_U8, PIXEL_U16: HOLO_U16}[_pair_format(d["holo"], d["bg"])]
             arr[i] = Cell(_ptr(d["holo"]), _ptr(d["bg"]), ang, A, H, W, d["V"], d["iters"],
-                          d.get("unwrap_flags", 0), d.get("recon_flags", 0), d["optics"], crop,
+                          d.get("unwrap_flags", 0), d.get("recon_flags", 0) | holo_flag, d["optics"], crop,
                           _ptr(d["ri"]), _ptr(d.get("wrapped")), _ptr(d.get("unwrapped")),
                           _ptr(d.get("logamp")))
         opts = BatchOpts(mode, int(graph), int(autotune), int(lanes), (C.c_int * 4)())

@@ -237,44 +237,87 @@ const char* ctdm_last_error(const ctdm_ctx* ctx)
     return ctx->c.last_error.c_str();
 }
 
+namespace {
+
+// Pixel format of an image argument; an unknown value is a caller error.
+ctdm::Pixels pixels(const void* p, ctdm_pixel px)
+{
+    CTDM_REQUIRE(px == CTDM_PIXEL_F32 || px == CTDM_PIXEL_U8 || px == CTDM_PIXEL_U16, "unknown pixel format");
+    return ctdm::Pixels{p, (ctdm::Px)px};
+}
+
+}  // namespace
+
+ctdm_status ctdm_find_sideband_px(ctdm_ctx* ctx, const void* bg, ctdm_pixel px, int H, int W, int M, int row_start,
+                                  ctdm_crop* out)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) { ctdm::find_sideband(c, pixels(bg, px), H, W, M, row_start, out); });
+}
+
 ctdm_status ctdm_find_sideband(ctdm_ctx* ctx, const float* bg, int H, int W, int M, int row_start,
                                ctdm_crop* out)
 {
-    return guarded(ctx, [&](ctdm::Ctx& c) { ctdm::find_sideband(c, bg, H, W, M, row_start, out); });
+    return ctdm_find_sideband_px(ctx, bg, CTDM_PIXEL_F32, H, W, M, row_start, out);
+}
+
+ctdm_status ctdm_retrieve_px(ctdm_ctx* ctx, const void* holo, const void* bg, ctdm_pixel px, int batch, int H, int W,
+                             const ctdm_crop* crop, float* wrapped, float* logamp, ctdm_complex* field)
+{
+    return guarded(ctx, [&](ctdm::Ctx& c) {
+        CTDM_REQUIRE(crop, "retrieve: null crop");
+        ctdm::retrieve_batched(c, pixels(holo, px), pixels(bg, px), batch, H, W, *crop, wrapped, logamp,
+                               (float2*)field);
+    });
 }
 
 ctdm_status ctdm_retrieve(ctdm_ctx* ctx, const float* holo, const float* bg, int batch, int H, int W,
                           const ctdm_crop* crop, float* wrapped, float* logamp, ctdm_complex* field)
 {
+    return ctdm_retrieve_px(ctx, holo, bg, CTDM_PIXEL_F32, batch, H, W, crop, wrapped, logamp, field);
+}
+
+ctdm_status ctdm_find_sideband_rows_px(ctdm_ctx* ctx, const void* bg, ctdm_pixel px, int H, int W, int* center)
+{
     return guarded(ctx, [&](ctdm::Ctx& c) {
-        CTDM_REQUIRE(crop, "retrieve: null crop");
-        ctdm::retrieve_batched(c, holo, bg, batch, H, W, *crop, wrapped, logamp, (float2*)field);
+        CTDM_REQUIRE(center, "find_sideband_rows: null output");
+        *center = ctdm::find_sideband_rows(c, pixels(bg, px), H, W);
     });
 }
 
 ctdm_status ctdm_find_sideband_rows(ctdm_ctx* ctx, const float* bg, int H, int W, int* center)
 {
+    return ctdm_find_sideband_rows_px(ctx, bg, CTDM_PIXEL_F32, H, W, center);
+}
+
+ctdm_status ctdm_retrieve_rows_px(ctdm_ctx* ctx, const void* holo, const void* bg, ctdm_pixel px, int batch, int H,
+                                  int W, int center, float* wrapped, float* logamp)
+{
     return guarded(ctx, [&](ctdm::Ctx& c) {
-        CTDM_REQUIRE(center, "find_sideband_rows: null output");
-        *center = ctdm::find_sideband_rows(c, bg, H, W);
+        ctdm::retrieve_rows(c, pixels(holo, px), pixels(bg, px), batch, H, W, center, wrapped, logamp);
     });
 }
 
 ctdm_status ctdm_retrieve_rows(ctdm_ctx* ctx, const float* holo, const float* bg, int batch, int H, int W, int center,
                                float* wrapped, float* logamp)
 {
+    return ctdm_retrieve_rows_px(ctx, holo, bg, CTDM_PIXEL_F32, batch, H, W, center, wrapped, logamp);
+}
+
+ctdm_status ctdm_angle_from_image_px(ctdm_ctx* ctx, const void* img, ctdm_pixel px, int batch, int H, int W,
+                                     double pixel_um, double mag, double wavelength_nm, double n_med, double* angles)
+{
     return guarded(ctx, [&](ctdm::Ctx& c) {
-        ctdm::retrieve_rows(c, holo, bg, batch, H, W, center, wrapped, logamp);
+        CTDM_REQUIRE(batch >= 0, "angle_from_image: negative batch");
+        const ctdm::Pixels im = pixels(img, px);
+        if (batch > 0) ctdm::angle_from_image(c, im, batch, H, W, pixel_um, mag, wavelength_nm, n_med, angles);
     });
 }
 
 ctdm_status ctdm_angle_from_image(ctdm_ctx* ctx, const float* img, int batch, int H, int W, double pixel_um,
                                   double mag, double wavelength_nm, double n_med, double* angles)
 {
-    return guarded(ctx, [&](ctdm::Ctx& c) {
-        CTDM_REQUIRE(batch >= 0, "angle_from_image: negative batch");
-        if (batch > 0) ctdm::angle_from_image(c, img, batch, H, W, pixel_um, mag, wavelength_nm, n_med, angles);
-    });
+    return ctdm_angle_from_image_px(ctx, img, CTDM_PIXEL_F32, batch, H, W, pixel_um, mag, wavelength_nm, n_med,
+                                    angles);
 }
 
 ctdm_status ctdm_unwrap_dct(ctdm_ctx* ctx, const float* wrapped, float* unwrapped, int batch, int M,

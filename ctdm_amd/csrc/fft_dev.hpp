@@ -217,10 +217,16 @@ struct IsRegFile<RegFile<P, R>> {
 template <int N, int T, int R0>
 using StageRegs = RegFile<(N / R0 + T - 1) / T, R0>;
 
+// Complex value of one element of a transform's input: a complex sample, or a pair of consecutive
+// real pixels (8- or 16-bit integers widened exactly) read as x[2n] + i x[2n+1].
+__device__ __forceinline__ float2 as_complex(float2 v) { return v; }
+__device__ __forceinline__ float2 as_complex(uchar2 v) { return make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ float2 as_complex(ushort2 v) { return make_float2((float)v.x, (float)v.y); }
+
 // Loads, for thread t of a transform, the elements its first stage reads (element j + r N/R0 at
 // p[(j + r N/R0) * stride], j = t + q T) into a RegFile.
-template <int N, int T, int R0, class F>
-__device__ __forceinline__ void load_first_stage(F& f, const float2* __restrict__ p, long long stride, int t)
+template <int N, int T, int R0, class F, class E>
+__device__ __forceinline__ void load_first_stage(F& f, const E* __restrict__ p, long long stride, int t)
 {
     constexpr int S0 = N / R0;
 #pragma unroll
@@ -228,7 +234,7 @@ __device__ __forceinline__ void load_first_stage(F& f, const float2* __restrict_
         const int j = t + q * T;
         if (S0 % T == 0 || j < S0) {
 #pragma unroll
-            for (int r = 0; r < F::R; ++r) f.v[q][r] = p[(long long)(j + r * S0) * stride];
+            for (int r = 0; r < F::R; ++r) f.v[q][r] = as_complex(p[(long long)(j + r * S0) * stride]);
         }
     }
 }

@@ -144,6 +144,20 @@ struct Ctx {
 
 // ---------------------------------------------------------------- module launchers
 // Retrieval (k_retrieve.hip)
+// Hologram pixels as the caller stores them: 32-bit floats, or 8- or 16-bit unsigned integers (camera
+// frames; the reference reads 8-bit BMP frames and widens them on the host, HilbertTransform.cu:419-420,
+// 660-661, 2694). The kernels that read holograms widen integer pixels exactly as they load them, so an
+// integer frame gives the same results as the same values passed as floats.
+enum class Px : int { F32 = 0, U8 = 1, U16 = 2 };
+struct Pixels {
+    const void* p = nullptr;
+    Px px = Px::F32;
+    size_t size() const { return px == Px::F32 ? 4 : (px == Px::U16 ? 2 : 1); }
+    // The pixels n pixels further on (image a of a stack: offset(a * H * W)).
+    Pixels offset(size_t n) const { return Pixels{p ? (const char*)p + n * size() : nullptr, px}; }
+    explicit operator bool() const { return p != nullptr; }
+};
+inline Pixels f32_pixels(const float* p) { return Pixels{p, Px::F32}; }
 // Crop of the batched retrieval as its kernels read it from device memory, so that one launch sequence
 // (and one captured graph) serves any carrier position: the crop itself and the passband it implies
 // (band [lo, lo + n) of offsets, the same for rows and columns; k_lo = col - W/2 + lo; r0 = row - H/2).
@@ -155,26 +169,26 @@ struct CropDev {
 void set_crop(Ctx& c, CropDev* slot, const ctdm_crop& cr, int H, int W);
 // Queues the sideband search of one background image without waiting: the crop goes to `slot` (if not
 // null) and the 64-bit search key to `key_dev` (device-visible pinned host memory).
-void enqueue_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, CropDev* slot,
+void enqueue_sideband(Ctx& c, Pixels bg, int H, int W, int M, int row_start, CropDev* slot,
                       unsigned long long* key_dev);
 // The crop a search key stands for (obtainRadius's row, column and radius).
 ctdm_crop decode_sideband(unsigned long long key, int H, int W, int M);
 void validate_crop(int H, int W, const ctdm_crop& cr);
-void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, ctdm_crop* out);
-int find_sideband_rows(Ctx& c, const float* bg, int H, int W);
-void angle_from_image(Ctx& c, const float* img, int batch, int H, int W, double pixel_um, double mag,
+void find_sideband(Ctx& c, Pixels bg, int H, int W, int M, int row_start, ctdm_crop* out);
+int find_sideband_rows(Ctx& c, Pixels bg, int H, int W);
+void angle_from_image(Ctx& c, Pixels img, int batch, int H, int W, double pixel_um, double mag,
                       double wavelength_nm, double n_med, double* angles);
-void retrieve_rows(Ctx& c, const float* holo, const float* bg, int batch, int H, int W, int center, float* wrapped,
+void retrieve_rows(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, int center, float* wrapped,
                    float* logamp);
-void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int H, int W,
-                      const ctdm_crop& crop, float* wrapped, float* logamp, float2* field);
+void retrieve_batched(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, const ctdm_crop& crop,
+                      float* wrapped, float* logamp, float2* field);
 // The same with the crop read from device memory (crop size M fixed on the host).
-void retrieve_batched_dev(Ctx& c, const float* holo, const float* bg, int batch, int H, int W, int M,
-                          const CropDev* cd, float* wrapped, float* logamp, float2* field);
+void retrieve_batched_dev(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, int M, const CropDev* cd,
+                          float* wrapped, float* logamp, float2* field);
 // Reference-structured retrieval of one hologram pair (full 2-D FFTs, shift, crop, inverse,
 // scale+conjugate, calcWrapPhase semantics: phase in (-2pi, 2pi), amplitude ratio).
-void retrieve_plain_one(Ctx& c, const float* holo, const float* bg, int H, int W,
-                        const ctdm_crop& crop, float* phase, float* amp);
+void retrieve_plain_one(Ctx& c, Pixels holo, Pixels bg, int H, int W, const ctdm_crop& crop, float* phase,
+                        float* amp);
 
 // Generic FFT passes (k_fft.hip)
 // Batched 1-D transforms of length n over pencils of a strided array (see k_fft.hip).

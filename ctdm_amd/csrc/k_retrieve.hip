@@ -24,6 +24,34 @@
 
 namespace ctdm {
 
+// Two consecutive pixels of a hologram row, read as one complex input element of the row transform.
+template <class PX>
+struct PxPair;
+template <>
+struct PxPair<float> {
+    using type = float2;
+};
+template <>
+struct PxPair<uint8_t> {
+    using type = uchar2;
+};
+template <>
+struct PxPair<uint16_t> {
+    using type = ushort2;
+};
+
+// Calls f with a null pointer of the pixel type of `px` (the kernels are instantiated per type).
+template <class F>
+static void with_pixel_type(Px px, F&& f)
+{
+    switch (px) {
+    case Px::F32: f((const float*)nullptr); break;
+    case Px::U8: f((const uint8_t*)nullptr); break;
+    case Px::U16: f((const uint16_t*)nullptr); break;
+    default: fail(CTDM_EINVAL, "unknown hologram pixel format");
+    }
+}
+
 struct Band {
     int lo;   // first kept offset (p' or q')
     int n;    // number of kept offsets
@@ -64,10 +92,10 @@ __global__ void k_set_crop(CropDev* __restrict__ slot, int row, int col, int rad
 // Each row of W reals is read as W/2 complex z[n] = x[2n] + i x[2n+1]; Z = FFT_{W/2}(z);
 // X[k] = (Z[k] + conj Z[-k]) / 2 - i/2 * exp(-2 pi i k / W) * (Z[k] - conj Z[-k]).
 // Output: out[(img*H + y)*nout + q] = X[(k_lo + q) mod W], q < nout. Images img < n0 come from
-// src0, the others from src1 (sample holograms first, then backgrounds).
-template <int W, int G>
+// src0, the others from src1 (sample holograms first, then backgrounds). PX: the pixel type.
+template <int W, int G, class PX>
 __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
-    const float* __restrict__ src0, int n0, const float* __restrict__ src1, int nimg, int H,
+    const PX* __restrict__ src0, int n0, const PX* __restrict__ src1, int nimg, int H,
     const float2* __restrict__ tw_half, const float2* __restrict__ tw_full, int k_lo, int nout,
     const CropDev* __restrict__ cd, float2* __restrict__ out)
 {
@@ -90,9 +118,9 @@ __global__ __launch_bounds__(G* Plan<W / 2>::T) void k_row_real(
     const int g = threadIdx.x / T, t = threadIdx.x % T;
     const long long img = img_u;
     const int y = row0 - img_u * H + g;
-    const float* src = (img < n0 ? src0 + img * (long long)H * W : src1 + (img - n0) * (long long)H * W) +
-                       (long long)y * W;
-    const float2* src2 = reinterpret_cast<const float2*>(src);
+    const PX* src = (img < n0 ? src0 + img * (long long)H * W : src1 + (img - n0) * (long long)H * W) +
+                    (long long)y * W;
+    const auto* src2 = reinterpret_cast<const typename PxPair<PX>::type*>(src);
     float2* s = lds + g * LdsSize<NH>::value;
     // this thread's first QP split twiddles exp(-2 pi i k / W), fetched before the transform to hide
     // their latency (a band of up to QP * T outputs: the whole passband of the batched retrieval);
@@ -393,29 +421,35 @@ __global__ __launch_bounds__(256) void k_publish_max(const unsigned long long* _
 
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
-template <int W>
-static void launch_row_real(Ctx& c, const float* src0, int n0, const float* src1, int nimg, int H,
-                            int k_lo, int nout, const CropDev* cd, float2* out)
+template <int W, class PX>
+static void launch_row_real(Ctx& c, const PX* src0, int n0, const PX* src1, int nimg, int H, int k_lo, int nout,
+                            const CropDev* cd, float2* out)
 {
     constexpr int G = 256 / Plan<W / 2>::T;
     const long long rows = (long long)nimg * H;
     CTDM_REQUIRE(H % G == 0 && rows <= INT_MAX,
                  "row transform: image height must be a multiple of the row group");
-    hipLaunchKernelGGL((k_row_real<W, G>), dim3(grid1d(rows, G)), dim3(G * Plan<W / 2>::T), 0,
+    hipLaunchKernelGGL((k_row_real<W, G, PX>), dim3(grid1d(rows, G)), dim3(G * Plan<W / 2>::T), 0,
                        c.stream, src0, n0, src1, nimg, H, c.tw(W / 2), c.tw(W), k_lo, nout, cd, out);
     CTDM_CHECK_LAUNCH("k_row_real");
 }
 
-// cd != nullptr: k_lo and nout are read from the device crop instead.
-static void row_real(Ctx& c, const float* src0, int n0, const float* src1, int nimg, int H, int W,
-                     int k_lo, int nout, const CropDev* cd, float2* out)
+// cd != nullptr: k_lo and nout are read from the device crop instead. src0 and src1 share one pixel type.
+static void row_real(Ctx& c, Pixels src0, int n0, Pixels src1, int nimg, int H, int W, int k_lo, int nout,
+                     const CropDev* cd, float2* out)
 {
-    switch (W) {
-    case 256: launch_row_real<256>(c, src0, n0, src1, nimg, H, k_lo, nout, cd, out); break;
-    case 512: launch_row_real<512>(c, src0, n0, src1, nimg, H, k_lo, nout, cd, out); break;
-    case 1024: launch_row_real<1024>(c, src0, n0, src1, nimg, H, k_lo, nout, cd, out); break;
-    default: fail(CTDM_EUNSUPPORTED, "hologram width " + std::to_string(W) + " has no kernel");
-    }
+    CTDM_REQUIRE(src0.px == src1.px, "row transform: mixed pixel formats");
+    with_pixel_type(src0.px, [&](auto tag) {
+        using PX = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
+        const PX* a = static_cast<const PX*>(src0.p);
+        const PX* b = static_cast<const PX*>(src1.p);
+        switch (W) {
+        case 256: launch_row_real<256>(c, a, n0, b, nimg, H, k_lo, nout, cd, out); break;
+        case 512: launch_row_real<512>(c, a, n0, b, nimg, H, k_lo, nout, cd, out); break;
+        case 1024: launch_row_real<1024>(c, a, n0, b, nimg, H, k_lo, nout, cd, out); break;
+        default: fail(CTDM_EUNSUPPORTED, "hologram width " + std::to_string(W) + " has no kernel");
+        }
+    });
 }
 
 template <int H, int M>
@@ -465,9 +499,8 @@ void validate_crop(int H, int W, const ctdm_crop& cr)
 
 // The three kernels for a band of width n (host crop) or of the device crop cd (buffers and grids then
 // sized for the widest band, M).
-static void retrieve_core(Ctx& c, const float* holo, const float* bg, int batch, int H, int W, int M,
-                          const Band& pb, int k_lo, int r0, const CropDev* cd, float* wrapped, float* logamp,
-                          float2* field)
+static void retrieve_core(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, int M, const Band& pb,
+                          int k_lo, int r0, const CropDev* cd, float* wrapped, float* logamp, float2* field)
 {
     const int nimg = 2 * batch;
     const int nmax = cd ? M : pb.n;
@@ -490,8 +523,8 @@ static void retrieve_core(Ctx& c, const float* holo, const float* bg, int batch,
     }
 }
 
-void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int H, int W,
-                      const ctdm_crop& crop, float* wrapped, float* logamp, float2* field)
+void retrieve_batched(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, const ctdm_crop& crop,
+                      float* wrapped, float* logamp, float2* field)
 {
     CTDM_REQUIRE(holo && bg && wrapped && logamp, "retrieve: null pointer");
     CTDM_REQUIRE(batch >= 0, "retrieve: negative batch");
@@ -502,8 +535,8 @@ void retrieve_batched(Ctx& c, const float* holo, const float* bg, int batch, int
                   wrapped, logamp, field);
 }
 
-void retrieve_batched_dev(Ctx& c, const float* holo, const float* bg, int batch, int H, int W, int M,
-                          const CropDev* cd, float* wrapped, float* logamp, float2* field)
+void retrieve_batched_dev(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, int M, const CropDev* cd,
+                          float* wrapped, float* logamp, float2* field)
 {
     CTDM_REQUIRE(holo && bg && wrapped && logamp && cd, "retrieve: null pointer");
     CTDM_REQUIRE(batch >= 0, "retrieve: negative batch");
@@ -531,7 +564,7 @@ static int launch_col_argmax(Ctx& c, const float2* spec, int W, int row_start, u
     return nblk;
 }
 
-void enqueue_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, CropDev* slot,
+void enqueue_sideband(Ctx& c, Pixels bg, int H, int W, int M, int row_start, CropDev* slot,
                       unsigned long long* key_dev)
 {
     CTDM_REQUIRE(bg && key_dev, "find_sideband: null pointer");
@@ -564,7 +597,7 @@ ctdm_crop decode_sideband(unsigned long long key, int H, int W, int M)
     return out;
 }
 
-void find_sideband(Ctx& c, const float* bg, int H, int W, int M, int row_start, ctdm_crop* out)
+void find_sideband(Ctx& c, Pixels bg, int H, int W, int M, int row_start, ctdm_crop* out)
 {
     CTDM_REQUIRE(bg && out, "find_sideband: null pointer");
     unsigned long long* hb = c.host_words(1);
@@ -605,7 +638,7 @@ __global__ void k_argmax_log10(const float2* __restrict__ spec, int H, int W, lo
     if ((threadIdx.x & 63) == 0) atomicMax(best, local);
 }
 
-void angle_from_image(Ctx& c, const float* img, int batch, int H, int W, double pixel_um, double mag,
+void angle_from_image(Ctx& c, Pixels img, int batch, int H, int W, double pixel_um, double mag,
                       double wavelength_nm, double n_med, double* angles)
 {
     CTDM_REQUIRE(img && angles, "angle_from_image: null pointer");
@@ -617,7 +650,7 @@ void angle_from_image(Ctx& c, const float* img, int batch, int H, int W, double 
     const long long lo = (long long)(N * 0.53), hi = (long long)(N * 0.79);
     std::vector<unsigned long long> h(batch);
     for (int b = 0; b < batch; ++b) {
-        const float* im = img + (size_t)b * N;
+        const Pixels im = img.offset((size_t)b * N);
         row_real(c, im, 1, im, 1, H, W, 0, W, nullptr, spec);
         fft_cols(c, spec, H, W, W, 1, 0, -1, 1.f, false);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c.stream, best + b);
@@ -650,17 +683,18 @@ void angle_from_image(Ctx& c, const float* img, int batch, int H, int W, double 
 // at position d mod M.
 
 // bilinear_interpolation_kernel's value at output (i, row j) of an H x W image, the reference's
-// expression in float.
-__device__ __forceinline__ float resample_b(const float* __restrict__ img, int H, int W, int i, int iy, float b)
+// expression in float (integer pixels widened first).
+template <class PX>
+__device__ __forceinline__ float resample_b(const PX* __restrict__ img, int H, int W, int i, int iy, float b)
 {
     const float xr = (float)(W - 1) / (float)W;
     const int ix = (int)(xr * (float)i);
     const float a = xr * (float)i - (float)ix;
-    const float* r0 = img + (long long)iy * W;
-    const float d00 = r0[ix];
-    const float d10 = ix + 1 < W ? r0[ix + 1] : 0.f;
-    const float d01 = iy + 1 < H ? r0[W + ix] : 0.f;
-    const float d11 = (ix + 1 < W && iy + 1 < H) ? r0[W + ix + 1] : 0.f;
+    const PX* r0 = img + (long long)iy * W;
+    const float d00 = (float)r0[ix];
+    const float d10 = ix + 1 < W ? (float)r0[ix + 1] : 0.f;
+    const float d01 = iy + 1 < H ? (float)r0[W + ix] : 0.f;
+    const float d11 = (ix + 1 < W && iy + 1 < H) ? (float)r0[W + ix + 1] : 0.f;
     const float t1 = a * d10 + (-d00 * a + d00);
     const float t2 = a * d11 + (-d01 * a + d01);
     return b * t2 + (-t1 * b + t1);
@@ -679,9 +713,9 @@ __device__ __forceinline__ float2 real_bin(const float2* s, int k, const float2*
     return cadd(e, cmul(tw_full[k], o));
 }
 
-template <int W, int G>
+template <int W, int G, class PX>
 __global__ __launch_bounds__(2 * G * Plan<W / 2>::T) void k_rows_b(
-    const float* __restrict__ holo, const float* __restrict__ bg, int H, int center,
+    const PX* __restrict__ holo, const PX* __restrict__ bg, int H, int center,
     const float2* __restrict__ tw_half, const float2* __restrict__ tw_full, const float2* __restrict__ tw_m,
     float* __restrict__ wrapped, float* __restrict__ logamp)
 {
@@ -694,7 +728,7 @@ __global__ __launch_bounds__(2 * G * Plan<W / 2>::T) void k_rows_b(
     const int tiles = N2 / G;
     const int pair = blockIdx.x / tiles;
     const int yq = (blockIdx.x % tiles) * G + r;  // resampled row y'
-    const float* img = (which ? bg : holo) + (long long)pair * H * W;
+    const PX* img = (which ? bg : holo) + (long long)pair * H * W;
     const float yr = (float)(H - 1) / (float)N2;
     const int iy = (int)(yr * (float)yq);
     const float bfr = yr * (float)yq - (float)iy;
@@ -748,8 +782,8 @@ __global__ __launch_bounds__(2 * G * Plan<W / 2>::T) void k_rows_b(
 
 // HistogramFT (HilbertTransform.cu:1728-1748) input: |X| of every shifted column of every resampled row
 // of one background, mag[y'][s].
-template <int W, int G>
-__global__ __launch_bounds__(G * Plan<W / 2>::T) void k_rows_mag(const float* __restrict__ img, int H,
+template <int W, int G, class PX>
+__global__ __launch_bounds__(G * Plan<W / 2>::T) void k_rows_mag(const PX* __restrict__ img, int H,
                                                                  const float2* __restrict__ tw_half,
                                                                  const float2* __restrict__ tw_full,
                                                                  float* __restrict__ mag)
@@ -802,24 +836,28 @@ static void check_rows_geometry(int H, int W)
     CTDM_REQUIRE(W == 256 || W == 512 || W == 1024, "row-wise retrieval: holograms of 256, 512 or 1024");
 }
 
-int find_sideband_rows(Ctx& c, const float* bg, int H, int W)
+int find_sideband_rows(Ctx& c, Pixels bg, int H, int W)
 {
     CTDM_REQUIRE(bg, "find_sideband_rows: null pointer");
     check_rows_geometry(H, W);
     const int N2 = H / 4;
     float* mag = (float*)c.buf("sbr.mag", sizeof(float) * (size_t)N2 * W);
     unsigned long long* best = (unsigned long long*)c.buf("sbr.best", sizeof(unsigned long long));
-    auto launch = [&](auto wtag) {
-        constexpr int Wc = decltype(wtag)::value;
-        constexpr int G = 256 / Plan<Wc / 2>::T;
-        hipLaunchKernelGGL((k_rows_mag<Wc, G>), dim3((N2 + G - 1) / G), dim3(G * Plan<Wc / 2>::T), 0, c.stream, bg,
-                           H, c.tw(Wc / 2), c.tw(Wc), mag);
-    };
-    switch (W) {
-    case 256: launch(std::integral_constant<int, 256>{}); break;
-    case 512: launch(std::integral_constant<int, 512>{}); break;
-    default: launch(std::integral_constant<int, 1024>{}); break;
-    }
+    with_pixel_type(bg.px, [&](auto ptag) {
+        using PX = std::remove_const_t<std::remove_pointer_t<decltype(ptag)>>;
+        const PX* img = static_cast<const PX*>(bg.p);
+        auto launch = [&](auto wtag) {
+            constexpr int Wc = decltype(wtag)::value;
+            constexpr int G = 256 / Plan<Wc / 2>::T;
+            hipLaunchKernelGGL((k_rows_mag<Wc, G, PX>), dim3((N2 + G - 1) / G), dim3(G * Plan<Wc / 2>::T), 0,
+                               c.stream, img, H, c.tw(Wc / 2), c.tw(Wc), mag);
+        };
+        switch (W) {
+        case 256: launch(std::integral_constant<int, 256>{}); break;
+        case 512: launch(std::integral_constant<int, 512>{}); break;
+        default: launch(std::integral_constant<int, 1024>{}); break;
+        }
+    });
     CTDM_CHECK_LAUNCH("k_rows_mag");
     const int lo = (int)(W * 0.6);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c.stream, best);
@@ -833,35 +871,42 @@ int find_sideband_rows(Ctx& c, const float* bg, int H, int W)
     return idx;
 }
 
-void retrieve_rows(Ctx& c, const float* holo, const float* bg, int batch, int H, int W, int center, float* wrapped,
+void retrieve_rows(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, int center, float* wrapped,
                    float* logamp)
 {
     CTDM_REQUIRE(holo && bg && wrapped && logamp, "retrieve_rows: null pointer");
+    CTDM_REQUIRE(holo.px == bg.px, "retrieve_rows: mixed pixel formats");
     CTDM_REQUIRE(batch >= 0, "retrieve_rows: negative batch");
     check_rows_geometry(H, W);
     CTDM_REQUIRE(center - W / 8 >= 0 && center + W / 8 <= W,
                  "retrieve_rows: the W/4 band around the sideband column leaves the spectrum");
     if (batch == 0) return;
-    auto launch = [&](auto wtag) {
-        constexpr int Wc = decltype(wtag)::value;
-        constexpr int G = 128 / Plan<Wc / 2>::T;
-        const int N2 = H / 4;
-        hipLaunchKernelGGL((k_rows_b<Wc, G>), dim3(batch * (N2 / G)), dim3(2 * G * Plan<Wc / 2>::T), 0, c.stream,
-                           holo, bg, H, center, c.tw(Wc / 2), c.tw(Wc), c.tw(Wc / 4), wrapped, logamp);
-    };
-    switch (W) {
-    case 256: launch(std::integral_constant<int, 256>{}); break;
-    case 512: launch(std::integral_constant<int, 512>{}); break;
-    default: launch(std::integral_constant<int, 1024>{}); break;
-    }
+    with_pixel_type(holo.px, [&](auto ptag) {
+        using PX = std::remove_const_t<std::remove_pointer_t<decltype(ptag)>>;
+        const PX* sp = static_cast<const PX*>(holo.p);
+        const PX* bk = static_cast<const PX*>(bg.p);
+        auto launch = [&](auto wtag) {
+            constexpr int Wc = decltype(wtag)::value;
+            constexpr int G = 128 / Plan<Wc / 2>::T;
+            const int N2 = H / 4;
+            hipLaunchKernelGGL((k_rows_b<Wc, G, PX>), dim3(batch * (N2 / G)), dim3(2 * G * Plan<Wc / 2>::T), 0,
+                               c.stream, sp, bk, H, center, c.tw(Wc / 2), c.tw(Wc), c.tw(Wc / 4), wrapped, logamp);
+        };
+        switch (W) {
+        case 256: launch(std::integral_constant<int, 256>{}); break;
+        case 512: launch(std::integral_constant<int, 512>{}); break;
+        default: launch(std::integral_constant<int, 1024>{}); break;
+        }
+    });
     CTDM_CHECK_LAUNCH("k_rows_b");
 }
 
 // ---------------------------------------------------------------- plain (reference-structured) path
-__global__ void k_real_to_complex(const float* __restrict__ in, float2* __restrict__ out, long long n)
+template <class PX>
+__global__ void k_real_to_complex(const PX* __restrict__ in, float2* __restrict__ out, long long n)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < n) out[g] = make_float2(in[g], 0.f);
+    if (g < n) out[g] = make_float2((float)in[g], 0.f);
 }
 
 // get1stOrder_new (HilbertTransform.cu:3566-3586), M-generalised.
@@ -892,18 +937,24 @@ __global__ void k_calc_wrap_phase(float* __restrict__ phase, float* __restrict__
     amp[i] = a;
 }
 
-void retrieve_plain_one(Ctx& c, const float* holo, const float* bg, int H, int W,
-                        const ctdm_crop& crop, float* phase, float* amp)
+void retrieve_plain_one(Ctx& c, Pixels holo, Pixels bg, int H, int W, const ctdm_crop& crop, float* phase,
+                        float* amp)
 {
     validate_crop(H, W, crop);
+    CTDM_REQUIRE(holo.px == bg.px, "retrieval: mixed pixel formats");
     const int M = crop.M;
     const long long n = (long long)H * W;
     float2* SP = (float2*)c.buf("plain.SP", sizeof(float2) * n);
     float2* BG = (float2*)c.buf("plain.BG", sizeof(float2) * n);
     float2* SP2 = (float2*)c.buf("plain.SP2", sizeof(float2) * M * M);
     float2* BG2 = (float2*)c.buf("plain.BG2", sizeof(float2) * M * M);
-    hipLaunchKernelGGL(k_real_to_complex, dim3(grid1d(n, 256)), dim3(256), 0, c.stream, holo, SP, n);
-    hipLaunchKernelGGL(k_real_to_complex, dim3(grid1d(n, 256)), dim3(256), 0, c.stream, bg, BG, n);
+    with_pixel_type(holo.px, [&](auto ptag) {  // the reference widens its 8-bit frames on the host
+        using PX = std::remove_const_t<std::remove_pointer_t<decltype(ptag)>>;
+        hipLaunchKernelGGL((k_real_to_complex<PX>), dim3(grid1d(n, 256)), dim3(256), 0, c.stream,
+                           static_cast<const PX*>(holo.p), SP, n);
+        hipLaunchKernelGGL((k_real_to_complex<PX>), dim3(grid1d(n, 256)), dim3(256), 0, c.stream,
+                           static_cast<const PX*>(bg.p), BG, n);
+    });
     CTDM_CHECK_LAUNCH("k_real_to_complex");
     for (float2* X : {SP, BG}) {  // cufftExecC2C 2-D forward (HilbertTransform.cu:1564-1565)
         fft_rows(c, X, W, H, -1, 1.f, false);

@@ -21,9 +21,21 @@ void scatter_plain_one(Ctx& c, const float2* u_sp, const AngleRow& r, int V, con
                        float* hits);
 void zero_volume(Ctx& c, float2* F, float* hits, int V);
 
+// Pixel format of a cell's holograms and backgrounds: 8- or 16-bit integers with CTDM_HOLO_U8 or
+// CTDM_HOLO_U16, float32 otherwise.
+static Px cell_px(const ctdm_cell& cl)
+{
+    const unsigned f = cl.recon_flags & (CTDM_HOLO_U8 | CTDM_HOLO_U16);
+    return f == CTDM_HOLO_U8 ? Px::U8 : (f == CTDM_HOLO_U16 ? Px::U16 : Px::F32);
+}
+static Pixels holo_of(const ctdm_cell& cl) { return Pixels{cl.holo, cell_px(cl)}; }
+static Pixels bg_of(const ctdm_cell& cl) { return Pixels{cl.bg, cell_px(cl)}; }
+
 static void validate_cell(const ctdm_cell& cl)
 {
     CTDM_REQUIRE(cl.holo && cl.bg && cl.angles && cl.ri, "cell: null pointer");
+    CTDM_REQUIRE((cl.recon_flags & (CTDM_HOLO_U8 | CTDM_HOLO_U16)) != (CTDM_HOLO_U8 | CTDM_HOLO_U16),
+                 "cell: CTDM_HOLO_U8 and CTDM_HOLO_U16 both set");
     CTDM_REQUIRE(cl.A > 0 && cl.H > 0 && cl.W > 0, "cell: empty");
     CTDM_REQUIRE(cl.V == 32 || cl.V == 64 || cl.V == 128 || cl.V == 192 || cl.V == 256,
                  "cell: volume side has no kernel");
@@ -53,11 +65,11 @@ static ctdm_crop cell_crop(Ctx& c, const ctdm_cell& cl)
 {
     if (cl.crop.radius >= 0) return cl.crop;
     if (cl.recon_flags & CTDM_RETRIEVE_ROWS) {  // sideband column located on frame 1 (HilbertTransform.cu:1634)
-        const int center = find_sideband_rows(c, cl.bg, cl.H, cl.W);
+        const int center = find_sideband_rows(c, bg_of(cl), cl.H, cl.W);
         return ctdm_crop{0, center, cl.W / 8, cl.V};
     }
     ctdm_crop cr{};
-    find_sideband(c, cl.bg, cl.H, cl.W, cl.V, 0, &cr);  // frame 1 of the reference (HilbertTransform.cu:1572)
+    find_sideband(c, bg_of(cl), cl.H, cl.W, cl.V, 0, &cr);  // frame 1 of the reference (HilbertTransform.cu:1572)
     return cr;
 }
 
@@ -93,15 +105,14 @@ static void run_plain(Ctx& c, ctdm_cell& cl)
     float* unw = cl.unwrapped_opt ? cl.unwrapped_opt : (float*)c.buf("pl.unw", sizeof(float) * A * img);
     const bool by_rows = cl.recon_flags & CTDM_RETRIEVE_ROWS;
     for (int a = 0; a < A; ++a) {
+        const Pixels sp = holo_of(cl).offset((size_t)a * H * W), bk = bg_of(cl).offset((size_t)a * H * W);
         if (by_rows) {  // one angle per call; the map is ln A, turned into A for the reference's copy kernel
-            retrieve_rows(c, cl.holo + (size_t)a * H * W, cl.bg + (size_t)a * H * W, 1, H, W, cl.crop.col,
-                          phase + a * img, amp + a * img);
+            retrieve_rows(c, sp, bk, 1, H, W, cl.crop.col, phase + a * img, amp + a * img);
             hipLaunchKernelGGL(k_exp, dim3(grid1d((long long)img, 256)), dim3(256), 0, c.stream, amp + a * img,
                                amp + a * img, (long long)img);
             CTDM_CHECK_LAUNCH("k_exp");
         } else {
-            retrieve_plain_one(c, cl.holo + (size_t)a * H * W, cl.bg + (size_t)a * H * W, H, W, cl.crop,
-                               phase + a * img, amp + a * img);
+            retrieve_plain_one(c, sp, bk, H, W, cl.crop, phase + a * img, amp + a * img);
         }
         CTDM_HIP(hipStreamSynchronize(c.stream));
         unwrap_dct_plain_one(c, phase + a * img, unw + a * img, V, cl.unwrap_flags);
@@ -220,9 +231,9 @@ static void enqueue_batched(Ctx& c, ctdm_cell& cl, int i, const double* rows)
     float* la = cl.logamp_opt ? cl.logamp_opt : (float*)c.buf("bt.la", sizeof(float) * n);
     float2* spec = (float2*)c.buf("bt.spec", sizeof(float2) * n);
     if (cl.recon_flags & CTDM_RETRIEVE_ROWS)
-        retrieve_rows(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.crop.col, wr, la);
+        retrieve_rows(c, holo_of(cl), bg_of(cl), A, cl.H, cl.W, cl.crop.col, wr, la);
     else
-        retrieve_batched_dev(c, cl.holo, cl.bg, A, cl.H, cl.W, cl.V, crop_slot(c, i), wr, la, nullptr);
+        retrieve_batched_dev(c, holo_of(cl), bg_of(cl), A, cl.H, cl.W, cl.V, crop_slot(c, i), wr, la, nullptr);
     if (!(cl.recon_flags & CTDM_MAPS_MEDIAN) &&
         !(cl.unwrap_flags & (CTDM_UNWRAP_REFCOMPAT_FAST | CTDM_UNWRAP_REFCOMPAT_IDCT))) {
         // the unwrapping's last pass also runs the Rytov row transform; the map goes out only when asked for
@@ -338,7 +349,7 @@ void reconstruct(Ctx& c, ctdm_cell* cells, int ncells, const ctdm_batch_opts& op
             validate_crop(cl.H, cl.W, cl.crop);
             set_crop(c, crop_slot(c, i), cl.crop, cl.H, cl.W);
         } else {
-            enqueue_sideband(c, cl.bg, cl.H, cl.W, cl.V, 0, crop_slot(c, i), keys_dev + k++);
+            enqueue_sideband(c, bg_of(cl), cl.H, cl.W, cl.V, 0, crop_slot(c, i), keys_dev + k++);
         }
     }
     if (!located.empty()) {

@@ -89,7 +89,20 @@ typedef struct ctdm_angle { double ang_x, ang_y; } ctdm_angle;
 #define CTDM_RETRIEVE_ROWS  0x10u  /* ctdm_reconstruct: row-wise retrieval (extractQPI_AlgorithmB, the
                                       reference's default) instead of the 2-D crop; crop.col holds the
                                       sideband column (crop.radius < 0: located from bg[0]); V = W / 4 */
+#define CTDM_HOLO_U8        0x20u  /* ctdm_cell: holo and bg hold 8-bit unsigned pixels (the reference's BMP
+                                      frames, HilbertTransform.cu:419-420, 660-661) instead of float32 */
+#define CTDM_HOLO_U16       0x40u  /* ctdm_cell: holo and bg hold 16-bit unsigned pixels */
 #define CTDM_EXEC_PLAIN     0x100u /* run the reference-structured, unfused kernel sequence */
+
+/* Pixel format of hologram, background and angle-calibration images (the *_px entry points). Integer
+ * pixels are widened to float exactly as they are loaded, so results equal those for the same values
+ * passed as float32; the reference reads 8-bit BMP frames (read_bmp, HilbertTransform.cu:2694) and
+ * widens them on the host (:660-661). */
+typedef enum ctdm_pixel {
+    CTDM_PIXEL_F32 = 0,
+    CTDM_PIXEL_U8 = 1,
+    CTDM_PIXEL_U16 = 2
+} ctdm_pixel;
 
 /* ---------------------------------------------------------------- context */
 ctdm_status ctdm_create(ctdm_ctx **ctx, const ctdm_config *cfg, ctdm_stream stream);
@@ -145,6 +158,19 @@ ctdm_status ctdm_retrieve_rows(ctdm_ctx *ctx, const float *holo, const float *bg
  * images. */
 ctdm_status ctdm_angle_from_image(ctdm_ctx *ctx, const float *img, int batch, int H, int W, double pixel_um,
                                   double mag, double wavelength_nm, double n_med, double *angles);
+
+/* The same five entry points for images of any ctdm_pixel format (holo, bg and img point to pixels of
+ * that format; the float32 entry points above are these with CTDM_PIXEL_F32). */
+ctdm_status ctdm_find_sideband_px(ctdm_ctx *ctx, const void *bg, ctdm_pixel px, int H, int W, int M,
+                                  int row_start, ctdm_crop *out);
+ctdm_status ctdm_retrieve_px(ctdm_ctx *ctx, const void *holo, const void *bg, ctdm_pixel px, int batch, int H,
+                             int W, const ctdm_crop *crop, float *wrapped, float *logamp, ctdm_complex *field_opt);
+ctdm_status ctdm_find_sideband_rows_px(ctdm_ctx *ctx, const void *bg, ctdm_pixel px, int H, int W, int *center);
+ctdm_status ctdm_retrieve_rows_px(ctdm_ctx *ctx, const void *holo, const void *bg, ctdm_pixel px, int batch,
+                                  int H, int W, int center, float *wrapped, float *logamp);
+ctdm_status ctdm_angle_from_image_px(ctdm_ctx *ctx, const void *img, ctdm_pixel px, int batch, int H, int W,
+                                     double pixel_um, double mag, double wavelength_nm, double n_med,
+                                     double *angles);
 
 /* ---------------------------------------------------------------- unwrapping (L3) */
 /* Replaces DCT_UWLS_Unwrapped (DCT_Unwrapping.cu:10-59; RHS option from FastUnwrapping.cu:5-117):
@@ -252,14 +278,16 @@ ctdm_status ctdm_recon_from_maps(ctdm_ctx *ctx, const float *unwrapped, const fl
 /* One cell: A hologram pairs -> one V^3 RI volume (reference: per-frame loop
  * HilbertTransform.cu:785-1165 followed by BatchRecon, HilbertTransform.cu:1825-2047). */
 typedef struct ctdm_cell {
-    const float *holo;          /* A x H x W sample holograms (device) */
-    const float *bg;            /* A x H x W backgrounds (device) */
+    const void *holo;           /* A x H x W sample holograms (device): float32 pixels, or 8- / 16-bit
+                                   unsigned pixels with CTDM_HOLO_U8 / CTDM_HOLO_U16 in recon_flags */
+    const void *bg;             /* A x H x W backgrounds (device), same pixel format */
     const ctdm_angle *angles;   /* A angles (HOST) */
     int A, H, W;
     int V;                      /* volume side = crop side M */
     int iters;                  /* positivity iterations (reference IterTime + 1) */
     unsigned unwrap_flags;      /* CTDM_UNWRAP_* */
-    unsigned recon_flags;       /* CTDM_DC_* | CTDM_POS_NO_SHIFT | CTDM_MAPS_MEDIAN | CTDM_RETRIEVE_ROWS */
+    unsigned recon_flags;       /* CTDM_DC_* | CTDM_POS_NO_SHIFT | CTDM_MAPS_MEDIAN | CTDM_RETRIEVE_ROWS |
+                                   CTDM_HOLO_U8 or CTDM_HOLO_U16 */
     ctdm_optics optics;
     ctdm_crop crop;             /* crop.M == V; crop.radius < 0: locate from bg[0] (reference frame 1);
                                    ctdm_reconstruct writes back the crop it used */
