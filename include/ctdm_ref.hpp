@@ -43,38 +43,52 @@ private:
     ctdm_ctx* ctx_ = nullptr;
 };
 
+// Pixel format of a frame pointer: float32, or the 8-bit pixels the reference reads from its BMP frames
+// (unsigned char, HilbertTransform.cu:419-420, 2694) and 16-bit camera pixels, which the kernels widen as
+// they load them (the reference widens on the host, :660-661).
+inline ctdm_pixel pixel_of(const float*) { return CTDM_PIXEL_F32; }
+inline ctdm_pixel pixel_of(const unsigned char*) { return CTDM_PIXEL_U8; }
+inline ctdm_pixel pixel_of(const unsigned short*) { return CTDM_PIXEL_U16; }
+
 // extractQPI_AlgorithmA(SP, BG, cuSP_FFT, cuBG_FFT, Nx, Ny, frame) (HilbertTransform.cu:1546)
 // followed by calcWrapPhase (HilbertTransform.cu:3618): one Nx x Ny hologram pair in, the Nx/4
 // (or crop.M, when set) wrapped phase and log-amplitude out. On frame 1 the sideband is located from
 // BG and stored in `crop` for the following frames, as the reference does (HilbertTransform.cu:1572).
-inline void extractQPI_AlgorithmA(const Context& c, const float* SP, const float* BG, int Nx, int Ny, int frame,
+// P: float, unsigned char or unsigned short pixels.
+template <class P>
+inline void extractQPI_AlgorithmA(const Context& c, const P* SP, const P* BG, int Nx, int Ny, int frame,
                                   ctdm_crop& crop, float* wrapped, float* logamp)
 {
+    const ctdm_pixel px = pixel_of(SP);
     if (frame == 1 || crop.radius < 0) {
         const int M = crop.M > 0 ? crop.M : Nx / 4;
-        c.check(ctdm_find_sideband(c.get(), BG, Ny, Nx, M, 0, &crop));
+        c.check(ctdm_find_sideband_px(c.get(), BG, px, Ny, Nx, M, 0, &crop));
     }
-    c.check(ctdm_retrieve(c.get(), SP, BG, 1, Ny, Nx, &crop, wrapped, logamp, nullptr));
+    c.check(ctdm_retrieve_px(c.get(), SP, BG, px, 1, Ny, Nx, &crop, wrapped, logamp, nullptr));
 }
 
 // extractQPI_AlgorithmB(SP, BG, cuSP_FFT, cuBG_FFT, Nx, Ny, frame) (HilbertTransform.cu:1587) followed by
 // calcWrapPhase: the reference's default retrieval. On frame 1 the sideband column is located from BG
 // and kept in `center` for the following frames (sequence1DFFT, HilbertTransform.cu:1634). Outputs are
 // (Nx/4) x (Ny/4) maps in the reference's transposed layout.
-inline void extractQPI_AlgorithmB(const Context& c, const float* SP, const float* BG, int Nx, int Ny, int frame,
+template <class P>
+inline void extractQPI_AlgorithmB(const Context& c, const P* SP, const P* BG, int Nx, int Ny, int frame,
                                   int& center, float* wrapped, float* logamp)
 {
-    if (frame == 1 || center < 0) c.check(ctdm_find_sideband_rows(c.get(), BG, Ny, Nx, &center));
-    c.check(ctdm_retrieve_rows(c.get(), SP, BG, 1, Ny, Nx, center, wrapped, logamp));
+    const ctdm_pixel px = pixel_of(SP);
+    if (frame == 1 || center < 0) c.check(ctdm_find_sideband_rows_px(c.get(), BG, px, Ny, Nx, &center));
+    c.check(ctdm_retrieve_rows_px(c.get(), SP, BG, px, 1, Ny, Nx, center, wrapped, logamp));
 }
 
 // AngCal_GPU(float *src, int sizeX, int sizeY, int frame, int totalFrame) (HilbertTransform.cu:2920) with
 // the optics the reference reads from globals (dx = camera pixel [um], Mag, wavelength [nm], Nmed).
-inline float AngCal_GPU(const Context& c, const float* src, int sizeX, int sizeY, double dx_um, double mag,
+template <class P>
+inline float AngCal_GPU(const Context& c, const P* src, int sizeX, int sizeY, double dx_um, double mag,
                         double wavelength_nm, double n_med)
 {
     double a = 0;
-    c.check(ctdm_angle_from_image(c.get(), src, 1, sizeY, sizeX, dx_um, mag, wavelength_nm, n_med, &a));
+    c.check(ctdm_angle_from_image_px(c.get(), src, pixel_of(src), 1, sizeY, sizeX, dx_um, mag, wavelength_nm, n_med,
+                                      &a));
     return (float)a;
 }
 

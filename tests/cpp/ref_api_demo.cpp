@@ -1,12 +1,14 @@
 // Drives libctdm through the reference-named C++ layer (include/ctdm_ref.hpp) the way the reference's
 // per-frame loop and BatchRecon call do, and checks the result against the one-call batched
 // pipeline (ctdm_reconstruct) with the same median step. BatchRecon runs twice: on ln A maps, and on
-// amplitude maps as the reference passes them (AmpStack). Prints "OK max|dn| = ..." and exits 0 on
-// agreement.
+// amplitude maps as the reference passes them (AmpStack). The per-frame retrieval also runs on 8-bit
+// frames (unsigned char, as the reference reads its BMP files) and must give the maps of float frames of
+// the same values bit for bit. Prints "OK max|dn| = ..." and exits 0 on agreement.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "ctdm_ref.hpp"
@@ -28,7 +30,8 @@ int main()
     std::vector<float> holo((size_t)A * H * H), bg((size_t)A * H * H);
     std::vector<ctdm_angle> ang(A);
     if (ctdm_synth_cell(7, A, &p, holo.data(), bg.data(), ang.data()) != CTDM_OK) return 2;
-    float *dh, *db, *dw, *dla, *dun, *damp;
+    float *dh, *db, *dw, *dla, *dun, *damp, *dhf, *dbf, *dw8, *dwf;
+    unsigned char *dh8, *db8;
     ctdm_complex *ri1, *ri2, *ri3;
     const size_t nh = (size_t)A * H * H, nm = (size_t)A * V * V, nv = (size_t)V * V * V;
     HIPCHECK(hipMalloc(&dh, nh * 4));
@@ -42,6 +45,27 @@ int main()
     HIPCHECK(hipMalloc(&ri3, nv * 8));
     HIPCHECK(hipMemcpy(dh, holo.data(), nh * 4, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(db, bg.data(), nh * 4, hipMemcpyHostToDevice));
+    // 8-bit frames of the same cell, and float frames holding the same integer values
+    float top = 0.f;
+    for (size_t i = 0; i < nh; ++i) top = std::fmax(top, std::fmax(holo[i], bg[i]));
+    std::vector<unsigned char> h8(nh), b8(nh);
+    std::vector<float> hf(nh), bf(nh);
+    for (size_t i = 0; i < nh; ++i) {
+        h8[i] = (unsigned char)std::lrint(255.f * holo[i] / top);
+        b8[i] = (unsigned char)std::lrint(255.f * bg[i] / top);
+        hf[i] = h8[i];
+        bf[i] = b8[i];
+    }
+    HIPCHECK(hipMalloc(&dh8, nh));
+    HIPCHECK(hipMalloc(&db8, nh));
+    HIPCHECK(hipMalloc(&dhf, nh * 4));
+    HIPCHECK(hipMalloc(&dbf, nh * 4));
+    HIPCHECK(hipMalloc(&dw8, nm * 4));
+    HIPCHECK(hipMalloc(&dwf, nm * 4));
+    HIPCHECK(hipMemcpy(dh8, h8.data(), nh, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(db8, b8.data(), nh, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(dhf, hf.data(), nh * 4, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(dbf, bf.data(), nh * 4, hipMemcpyHostToDevice));
     try {
         ctdm_ref::Context c;
         ctdm_crop crop{0, 0, -1, V};
@@ -81,7 +105,17 @@ int main()
         ctdm_batch_opts opts{};
         opts.mode = 1;
         c.check(ctdm_reconstruct(c.get(), &cell, 1, &opts));
+        ctdm_crop c8{0, 0, -1, V}, cf{0, 0, -1, V};
+        for (int f = 0; f < A; ++f) {
+            const size_t o = (size_t)f * H * H, m = (size_t)f * V * V;
+            ctdm_ref::extractQPI_AlgorithmA(c, dh8 + o, db8 + o, H, H, f + 1, c8, dw8 + m, dla + m);
+            ctdm_ref::extractQPI_AlgorithmA(c, dhf + o, dbf + o, H, H, f + 1, cf, dwf + m, dla + m);
+        }
         c.sync();
+        if (c8.row != cf.row || c8.col != cf.col || c8.radius != cf.radius) {
+            std::printf("MISMATCH: 8-bit frames located another sideband\n");
+            return 1;
+        }
     } catch (const ctdm_ref::Error& e) {
         std::printf("ctdm error %d: %s\n", (int)e.status, e.what());
         return 1;
@@ -97,11 +131,15 @@ int main()
         lo = std::fmin(lo, a[i].x);
         hi = std::fmax(hi, a[i].x);
     }
-    const bool ok = d <= 1e-6 && d3 <= 1e-6;
-    std::printf("%s max|dn| = %.3e (ln A maps), %.3e (amplitude maps), RI %.4f..%.4f\n", ok ? "OK" : "MISMATCH", d,
-                d3, lo, hi);
+    std::vector<float> w8(nm), wf(nm);
+    HIPCHECK(hipMemcpy(w8.data(), dw8, nm * 4, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(wf.data(), dwf, nm * 4, hipMemcpyDeviceToHost));
+    const bool same8 = std::memcmp(w8.data(), wf.data(), nm * 4) == 0;
+    const bool ok = d <= 1e-6 && d3 <= 1e-6 && same8;
+    std::printf("%s max|dn| = %.3e (ln A maps), %.3e (amplitude maps), RI %.4f..%.4f; 8-bit frames %s\n",
+                ok ? "OK" : "MISMATCH", d, d3, lo, hi, same8 ? "bit-identical" : "DIFFER");
     for (void* q : {(void*)dh, (void*)db, (void*)dw, (void*)dla, (void*)dun, (void*)damp, (void*)ri1, (void*)ri2,
-                    (void*)ri3})
+                    (void*)ri3, (void*)dh8, (void*)db8, (void*)dhf, (void*)dbf, (void*)dw8, (void*)dwf})
         hipFree(q);
     return ok ? 0 : 1;
 }
