@@ -209,7 +209,11 @@ def test_autotune_keeps_results_and_times_passes(ctx):
     assert "V=64" in report and report.count("->") == 6 and "sweep" in report and "y plan" in report
     assert "grid" in report
     after, _ = _run(ctx, holo, bg, ang, V, 2, mode=1)
-    assert torch.equal(before, after)  # the pencil grouping does not change any value
+    print("tuned:", ctx.get_tuning(V))
+    if ctx.get_tuning(V)["yplan"] == 0:
+        assert torch.equal(before, after)  # pencil grouping and sweep order do not change any value
+    else:  # the tuner took the y pass's other radix plan (a timing choice): it rounds differently
+        assert (before - after).abs().max().item() <= 1e-6
     for k in (0, 1, 2):
         ms, nbytes = ctx.time_pass(k, V, 0, 5)
         assert ms > 0 and nbytes >= 2 * 8 * V ** 3
