@@ -439,6 +439,10 @@ static void row_real(Ctx& c, Pixels src0, int n0, Pixels src1, int nimg, int H, 
                      const CropDev* cd, float2* out)
 {
     CTDM_REQUIRE(src0.px == src1.px, "row transform: mixed pixel formats");
+    // pixels are read in pairs (one complex element each), so images start at a multiple of two pixels
+    const size_t pair = 2 * src0.size();
+    CTDM_REQUIRE((uintptr_t)src0.p % pair == 0 && (uintptr_t)src1.p % pair == 0,
+                 "hologram pointers must be aligned to two pixels");
     with_pixel_type(src0.px, [&](auto tag) {
         using PX = std::remove_const_t<std::remove_pointer_t<decltype(tag)>>;
         const PX* a = static_cast<const PX*>(src0.p);

@@ -97,7 +97,8 @@ typedef struct ctdm_angle { double ang_x, ang_y; } ctdm_angle;
 /* Pixel format of hologram, background and angle-calibration images (the *_px entry points). Integer
  * pixels are widened to float exactly as they are loaded, so results equal those for the same values
  * passed as float32; the reference reads 8-bit BMP frames (read_bmp, HilbertTransform.cu:2694) and
- * widens them on the host (:660-661). */
+ * widens them on the host (:660-661). The 2-D retrieval reads pixels in pairs: its image pointers must
+ * be aligned to two pixels (CTDM_EINVAL otherwise). */
 typedef enum ctdm_pixel {
     CTDM_PIXEL_F32 = 0,
     CTDM_PIXEL_U8 = 1,

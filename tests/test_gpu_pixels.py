@@ -167,3 +167,10 @@ def test_pixel_format_errors(ctx):
                               c_int["wrapped"].data_ptr(), None)
     assert st == ctdm_amd.EINVAL
     assert b"pixel format" in lib.ctdm_last_error(ctx.h)
+    # 8-bit frames are read in pixel pairs: an image starting at an odd byte address is refused before
+    # any work is queued
+    st = lib.ctdm_retrieve_px(ctx.h, c_int["holo"].data_ptr() + 1, c_int["bg"].data_ptr(), ctdm_amd.PIXEL_U8, 1,
+                              H, H, C.byref(ctdm_amd.Crop(192, 192, 21, 64)), c_int["wrapped"].data_ptr(),
+                              c_int["wrapped"].data_ptr(), None)
+    assert st == ctdm_amd.EINVAL
+    assert b"aligned to two pixels" in lib.ctdm_last_error(ctx.h)
