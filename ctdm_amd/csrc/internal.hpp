@@ -178,13 +178,15 @@ void find_sideband(Ctx& c, Pixels bg, int H, int W, int M, int row_start, ctdm_c
 int find_sideband_rows(Ctx& c, Pixels bg, int H, int W);
 void angle_from_image(Ctx& c, Pixels img, int batch, int H, int W, double pixel_um, double mag,
                       double wavelength_nm, double n_med, double* angles);
+// refph (may be null): calcWrapPhase's own value atan2(BG) - atan2(SP) in (-2 pi, 2 pi), the input of
+// FastUnwrapping's min-max rescaling in the reference (HilbertTransform.cu:3630; FastUnwrapping.cu:7).
 void retrieve_rows(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, int center, float* wrapped,
-                   float* logamp);
+                   float* logamp, float* refph = nullptr);
 void retrieve_batched(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, const ctdm_crop& crop,
                       float* wrapped, float* logamp, float2* field);
 // The same with the crop read from device memory (crop size M fixed on the host).
 void retrieve_batched_dev(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, int M, const CropDev* cd,
-                          float* wrapped, float* logamp, float2* field);
+                          float* wrapped, float* logamp, float2* field, float* refph = nullptr);
 // Reference-structured retrieval of one hologram pair (full 2-D FFTs, shift, crop, inverse,
 // scale+conjugate, calcWrapPhase semantics: phase in (-2pi, 2pi), amplitude ratio).
 void retrieve_plain_one(Ctx& c, Pixels holo, Pixels bg, int H, int W, const ctdm_crop& crop, float* phase,

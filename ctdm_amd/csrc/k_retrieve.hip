@@ -284,6 +284,16 @@ __global__ __launch_bounds__(C* Plan<H>::T) void k_col_band(
     Plan<M>::template run<+1>(t, s, tw_m, gl2, gs2);
 }
 
+// calcWrapPhase's phase (HilbertTransform.cu:3630): atan2(BG) - atan2(SP) of the conjugated fields, i.e.
+// arg SP - arg BG of the fields here, each in (-pi, pi], so the difference lies in (-2 pi, 2 pi) and is
+// not wrapped again; non-finite -> 0. Only FastUnwrapping's min-max rescaling (PhaseNormalize) sees the
+// difference between this value and the wrapped phase.
+__device__ __forceinline__ float calc_wrap_phase(float2 sp, float2 bg)
+{
+    float p = atan2f(sp.y, sp.x) - atan2f(bg.y, bg.x);
+    return isfinite(p) ? p : 0.f;
+}
+
 // ---------------------------------------------------------------- 3. final rows + phase / log-amp
 // in: G[img][y][q] for img < 2*batch (SP items first). Each workgroup does GR rows of one pair:
 // inverse M-point along x of SP and BG rows, then
@@ -293,7 +303,7 @@ template <int M, int GR>
 __global__ __launch_bounds__(2 * GR * Plan<M>::T) void k_row_final(
     const float2* __restrict__ in, int batch, int nin, int p_lo, const CropDev* __restrict__ cd,
     const float2* __restrict__ tw_m, float* __restrict__ wrapped, float* __restrict__ logamp,
-    float2* __restrict__ field)
+    float2* __restrict__ field, float* __restrict__ refph)
 {
     constexpr int T = Plan<M>::T;
     if (cd) {  // crop located on the device
@@ -337,6 +347,7 @@ __global__ __launch_bounds__(2 * GR * Plan<M>::T) void k_row_final(
             if (!isfinite(f.x) || !isfinite(f.y)) f = make_float2(0.f, 0.f);
             field[o] = f;
         }
+        if (refph) refph[o] = calc_wrap_phase(sp, bgv);
     }
 }
 
@@ -469,11 +480,11 @@ static void launch_col_band(Ctx& c, const float2* in, int nimg, int nin, int r0,
 
 template <int M>
 static void launch_row_final(Ctx& c, const float2* in, int batch, int nin, int p_lo, const CropDev* cd,
-                             float* wrapped, float* logamp, float2* field)
+                             float* wrapped, float* logamp, float2* field, float* refph)
 {
     constexpr int GR = 128 / Plan<M>::T;
     hipLaunchKernelGGL((k_row_final<M, GR>), dim3(batch * (M / GR)), dim3(2 * GR * Plan<M>::T), 0,
-                       c.stream, in, batch, nin, p_lo, cd, c.tw(M), wrapped, logamp, field);
+                       c.stream, in, batch, nin, p_lo, cd, c.tw(M), wrapped, logamp, field, refph);
     CTDM_CHECK_LAUNCH("k_row_final");
 }
 
@@ -504,7 +515,8 @@ void validate_crop(int H, int W, const ctdm_crop& cr)
 // The three kernels for a band of width n (host crop) or of the device crop cd (buffers and grids then
 // sized for the widest band, M).
 static void retrieve_core(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, int M, const Band& pb,
-                          int k_lo, int r0, const CropDev* cd, float* wrapped, float* logamp, float2* field)
+                          int k_lo, int r0, const CropDev* cd, float* wrapped, float* logamp, float2* field,
+                          float* refph)
 {
     const int nimg = 2 * batch;
     const int nmax = cd ? M : pb.n;
@@ -518,11 +530,11 @@ static void retrieve_core(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int 
     default: fail(CTDM_EUNSUPPORTED, "hologram height " + std::to_string(H) + " has no kernel");
     }
     switch (M) {
-    case 32: launch_row_final<32>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field); break;
-    case 64: launch_row_final<64>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field); break;
-    case 128: launch_row_final<128>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field); break;
-    case 192: launch_row_final<192>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field); break;
-    case 256: launch_row_final<256>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field); break;
+    case 32: launch_row_final<32>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field, refph); break;
+    case 64: launch_row_final<64>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field, refph); break;
+    case 128: launch_row_final<128>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field, refph); break;
+    case 192: launch_row_final<192>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field, refph); break;
+    case 256: launch_row_final<256>(c, G, batch, pb.n, pb.lo, cd, wrapped, logamp, field, refph); break;
     default: fail(CTDM_EUNSUPPORTED, "crop size has no kernel");
     }
 }
@@ -536,17 +548,17 @@ void retrieve_batched(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, c
     if (batch == 0) return;
     const Band pb = passband(crop.radius, crop.M);
     retrieve_core(c, holo, bg, batch, H, W, crop.M, pb, crop.col - W / 2 + pb.lo, crop.row - H / 2, nullptr,
-                  wrapped, logamp, field);
+                  wrapped, logamp, field, nullptr);
 }
 
 void retrieve_batched_dev(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, int M, const CropDev* cd,
-                          float* wrapped, float* logamp, float2* field)
+                          float* wrapped, float* logamp, float2* field, float* refph)
 {
     CTDM_REQUIRE(holo && bg && wrapped && logamp && cd, "retrieve: null pointer");
     CTDM_REQUIRE(batch >= 0, "retrieve: negative batch");
     CTDM_REQUIRE(H == W && M > 0 && M % 2 == 0 && M <= H, "retrieval: bad sizes");
     if (batch == 0) return;
-    retrieve_core(c, holo, bg, batch, H, W, M, Band{0, M}, 0, 0, cd, wrapped, logamp, field);
+    retrieve_core(c, holo, bg, batch, H, W, M, Band{0, M}, 0, 0, cd, wrapped, logamp, field, refph);
 }
 
 void set_crop(Ctx& c, CropDev* slot, const ctdm_crop& cr, int H, int W)
@@ -721,7 +733,7 @@ template <int W, int G, class PX>
 __global__ __launch_bounds__(2 * G * Plan<W / 2>::T) void k_rows_b(
     const PX* __restrict__ holo, const PX* __restrict__ bg, int H, int center,
     const float2* __restrict__ tw_half, const float2* __restrict__ tw_full, const float2* __restrict__ tw_m,
-    float* __restrict__ wrapped, float* __restrict__ logamp)
+    float* __restrict__ wrapped, float* __restrict__ logamp, float* __restrict__ refph)
 {
     constexpr int NH = W / 2, T = Plan<NH>::T, M = W / 4;
     constexpr int LS = LdsSize<NH>::value;
@@ -781,6 +793,7 @@ __global__ __launch_bounds__(2 * G * Plan<W / 2>::T) void k_rows_b(
         const long long o = (long long)pair * M * N2 + (long long)(y0 + rr) + (long long)N2 * n;
         wrapped[o] = ph;
         logamp[o] = la;
+        if (refph) refph[o] = calc_wrap_phase(sp, bv);
     }
 }
 
@@ -876,7 +889,7 @@ int find_sideband_rows(Ctx& c, Pixels bg, int H, int W)
 }
 
 void retrieve_rows(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, int center, float* wrapped,
-                   float* logamp)
+                   float* logamp, float* refph)
 {
     CTDM_REQUIRE(holo && bg && wrapped && logamp, "retrieve_rows: null pointer");
     CTDM_REQUIRE(holo.px == bg.px, "retrieve_rows: mixed pixel formats");
@@ -894,7 +907,8 @@ void retrieve_rows(Ctx& c, Pixels holo, Pixels bg, int batch, int H, int W, int 
             constexpr int G = 128 / Plan<Wc / 2>::T;
             const int N2 = H / 4;
             hipLaunchKernelGGL((k_rows_b<Wc, G, PX>), dim3(batch * (N2 / G)), dim3(2 * G * Plan<Wc / 2>::T), 0,
-                               c.stream, sp, bk, H, center, c.tw(Wc / 2), c.tw(Wc), c.tw(Wc / 4), wrapped, logamp);
+                               c.stream, sp, bk, H, center, c.tw(Wc / 2), c.tw(Wc), c.tw(Wc / 4), wrapped, logamp,
+                               refph);
         };
         switch (W) {
         case 256: launch(std::integral_constant<int, 256>{}); break;

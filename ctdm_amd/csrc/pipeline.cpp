@@ -104,10 +104,14 @@ static void run_plain(Ctx& c, ctdm_cell& cl)
     float* amp = (float*)c.buf("pl.amp", sizeof(float) * A * img);
     float* unw = cl.unwrapped_opt ? cl.unwrapped_opt : (float*)c.buf("pl.unw", sizeof(float) * A * img);
     const bool by_rows = cl.recon_flags & CTDM_RETRIEVE_ROWS;
+    const bool fast = cl.unwrap_flags & CTDM_UNWRAP_REFCOMPAT_FAST;
     for (int a = 0; a < A; ++a) {
         const Pixels sp = holo_of(cl).offset((size_t)a * H * W), bk = bg_of(cl).offset((size_t)a * H * W);
         if (by_rows) {  // one angle per call; the map is ln A, turned into A for the reference's copy kernel
-            retrieve_rows(c, sp, bk, 1, H, W, cl.crop.col, phase + a * img, amp + a * img);
+            // the phase map holds calcWrapPhase's value, as after retrieve_plain_one (FastUnwrapping's
+            // min-max rescaling depends on it; every other unwrapping is the same for the wrapped map)
+            float* wr1 = fast ? (float*)c.buf("pl.wr1", sizeof(float) * img) : phase + a * img;
+            retrieve_rows(c, sp, bk, 1, H, W, cl.crop.col, wr1, amp + a * img, fast ? phase + a * img : nullptr);
             hipLaunchKernelGGL(k_exp, dim3(grid1d((long long)img, 256)), dim3(256), 0, c.stream, amp + a * img,
                                amp + a * img, (long long)img);
             CTDM_CHECK_LAUNCH("k_exp");
@@ -230,10 +234,14 @@ static void enqueue_batched(Ctx& c, ctdm_cell& cl, int i, const double* rows)
     float* wr = cl.wrapped_opt ? cl.wrapped_opt : (float*)c.buf("bt.wr", sizeof(float) * n);
     float* la = cl.logamp_opt ? cl.logamp_opt : (float*)c.buf("bt.la", sizeof(float) * n);
     float2* spec = (float2*)c.buf("bt.spec", sizeof(float2) * n);
+    // FastUnwrapping's conventions rescale each map by its own extremes, which the reference takes of
+    // calcWrapPhase's value in (-2 pi, 2 pi) (HilbertTransform.cu:3630, FastUnwrapping.cu:7): the retrieval
+    // then also writes that value, and the unwrapping reads it instead of the wrapped map
+    float* ph = (cl.unwrap_flags & CTDM_UNWRAP_REFCOMPAT_FAST) ? (float*)c.buf("bt.ph", sizeof(float) * n) : nullptr;
     if (cl.recon_flags & CTDM_RETRIEVE_ROWS)
-        retrieve_rows(c, holo_of(cl), bg_of(cl), A, cl.H, cl.W, cl.crop.col, wr, la);
+        retrieve_rows(c, holo_of(cl), bg_of(cl), A, cl.H, cl.W, cl.crop.col, wr, la, ph);
     else
-        retrieve_batched_dev(c, holo_of(cl), bg_of(cl), A, cl.H, cl.W, cl.V, crop_slot(c, i), wr, la, nullptr);
+        retrieve_batched_dev(c, holo_of(cl), bg_of(cl), A, cl.H, cl.W, cl.V, crop_slot(c, i), wr, la, nullptr, ph);
     if (!(cl.recon_flags & CTDM_MAPS_MEDIAN) &&
         !(cl.unwrap_flags & (CTDM_UNWRAP_REFCOMPAT_FAST | CTDM_UNWRAP_REFCOMPAT_IDCT))) {
         // the unwrapping's last pass also runs the Rytov row transform; the map goes out only when asked for
@@ -242,7 +250,7 @@ static void enqueue_batched(Ctx& c, ctdm_cell& cl, int i, const double* rows)
         rytov_spectrum_cols(c, P, A, V, spec);
     } else {
         float* un = cl.unwrapped_opt ? cl.unwrapped_opt : (float*)c.buf("bt.un", sizeof(float) * n);
-        unwrap_dct_batched(c, wr, un, A, V, cl.unwrap_flags);
+        unwrap_dct_batched(c, ph ? ph : wr, un, A, V, cl.unwrap_flags);
         const float* un_r = un;
         const float* la_r = la;
         if (cl.recon_flags & CTDM_MAPS_MEDIAN) {  // BatchRecon's median + checkMaps on the maps it receives

@@ -68,7 +68,7 @@ _EXTRA: dict = {
     "orc_fit_pgd": [_dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                     C.c_double, _dp, C.c_uint, _dp],
     "orc_find_sideband_rows": [_fp, C.c_int, C.c_int],
-    "orc_retrieve_rows": [_fp, _fp, C.c_int, C.c_int, C.c_int, _dp, _dp],
+    "orc_retrieve_rows": [_fp, _fp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp],
     "orc_median3x3": [_fp, _fp, C.c_int, C.c_int, C.c_int],
     "orc_check_maps": [C.c_void_p, C.c_void_p, C.c_long],
     "orc_recon_from_maps": [_fp, _fp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
@@ -334,8 +334,9 @@ def retrieve_rows(holo, bg, center):
     M, N2 = W // 4, H // 4
     w = np.zeros((M, N2))
     la = np.zeros((M, N2))
-    lib().orc_retrieve_rows(_p(h, _fp), _p(b, _fp), H, W, center, _p(w, _dp), _p(la, _dp))
-    return {"wrapped": w, "logamp": la}
+    ph = np.zeros((M, N2))
+    lib().orc_retrieve_rows(_p(h, _fp), _p(b, _fp), H, W, center, _p(w, _dp), _p(la, _dp), _p(ph, _dp))
+    return {"wrapped": w, "logamp": la, "phase_ref": ph}
 
 
 def angle_from_image(img, pixel_um, mag, wavelength_nm, n_med):

@@ -332,7 +332,7 @@ int orc_find_sideband_rows(const float* bg, int H, int W)
 // transposed, out[y' + (H/4) n]. wrapped = arg SP - arg BG in (-pi, pi], logamp = ln(|SP| / |BG|).
 // Square holograms (H == W).
 void orc_retrieve_rows(const float* holo, const float* bg, int H, int W, int center, double* wrapped,
-                       double* logamp)
+                       double* logamp, double* phase)
 {
     const int M = W / 4, N2 = H / 4;
     const std::vector<cd> S = rows_spectrum_b(holo, H, W), B = rows_spectrum_b(bg, H, W);
@@ -363,6 +363,7 @@ void orc_retrieve_rows(const float* holo, const float* bg, int H, int W, int cen
             const size_t o = (size_t)j + (size_t)N2 * n;
             if (wrapped) wrapped[o] = wrap_pi(p);
             if (logamp) logamp[o] = l;
+            if (phase) phase[o] = p;  // calcWrapPhase's own value, in (-2 pi, 2 pi) (HilbertTransform.cu:3630)
         }
 }
 
@@ -1003,8 +1004,7 @@ void orc_reconstruct_cell(const float* holo, const float* bg, const double* angl
     for (int a = 0; a < A; ++a) {
         std::vector<double> pr(img), am(img), lg(img), wr(img);
         if (recon_flags & 16u) {  // row-wise retrieval (Algorithm B), sideband column in col
-            orc_retrieve_rows(holo + a * hh, bg + a * hh, H, H, col, wr.data(), lg.data());
-            pr = wr;
+            orc_retrieve_rows(holo + a * hh, bg + a * hh, H, H, col, wr.data(), lg.data(), pr.data());
         } else {
             orc_retrieve(holo + a * hh, bg + a * hh, H, H, row, col, radius, V, pr.data(), am.data(), wr.data(),
                          lg.data(), nullptr);

@@ -1,0 +1,94 @@
+"""Whole cells with every unwrapping variant, against the FP64 oracle's cell reconstruction.
+
+The batched pipeline takes one of two routes after the retrieval: flag sets without the reference's
+IDCT or FastUnwrapping conventions run the unwrapping's last pass fused with the Rytov row transform
+(k_uw_rows_inv_rytov), the others the plain unwrapping followed by the Rytov spectrum. Each variant is
+run batched (eager and graph-replayed) and plain, and must give the oracle's RI within 1e-5 and the
+oracle's unwrapped maps (written by the fused pass when asked for) within FP32 rounding; with the
+congruence step a pixel may round to the neighbouring multiple of 2 pi where the least-squares value
+sits on a half-integer, so those maps are compared outside such pixels.
+
+FastUnwrapping's conventions (CTDM_UNWRAP_REFCOMPAT_FAST) rescale each map by its own minimum and
+maximum (PhaseNormalize, FastUnwrapping.cu:119-142), which the reference takes of calcWrapPhase's value
+atan2(BG) - atan2(SP) in (-2 pi, 2 pi) (HilbertTransform.cu:3630), not of the wrapped phase: every
+path, both retrievals included, unwraps that value then, and the wrapped-map output stays the wrapped
+phase."""
+import numpy as np
+import pytest
+import torch
+
+import ctdm_amd
+
+pytestmark = pytest.mark.gpu
+LAM, NM = 532e-9, 1.337
+
+FLAGS = [
+    ctdm_amd.UNWRAP_SINCOS,                                   # fused route
+    ctdm_amd.UNWRAP_SINCOS | ctdm_amd.UNWRAP_CONGRUENT,       # fused, congruence in the last pass
+    ctdm_amd.UNWRAP_REFCOMPAT_EIG,                            # fused, the reference's eigenvalues
+    ctdm_amd.UNWRAP_CONGRUENT,                                # fused
+    ctdm_amd.UNWRAP_REFCOMPAT,                                # unfused: the reference's IDCT
+    ctdm_amd.UNWRAP_SINCOS | ctdm_amd.UNWRAP_CONGRUENT | ctdm_amd.UNWRAP_REFCOMPAT_FAST,  # unfused
+]
+
+
+def _cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.fixture(scope="module")
+def cell_data():
+    A, H, V = 8, 256, 64
+    holo, bg, ang = ctdm_amd.synth_cell(61, A, ctdm_amd.synth_params(H=H))
+    return A, H, V, holo, bg, ang
+
+
+CASES = [(f, False) for f in FLAGS] + [
+    (0, True), (ctdm_amd.UNWRAP_SINCOS | ctdm_amd.UNWRAP_CONGRUENT, True),
+    (ctdm_amd.UNWRAP_SINCOS | ctdm_amd.UNWRAP_CONGRUENT | ctdm_amd.UNWRAP_REFCOMPAT_FAST, True),
+]
+
+
+@pytest.mark.parametrize("flags,rows", CASES)
+def test_cell_unwrap_variant_matches_oracle(ctx, orc, cell_data, flags, rows):
+    A, H, V, holo, bg, ang = cell_data
+    iters = 2
+    o = ctdm_amd.optics_for(V, H)
+    hd, bd = _cuda(holo), _cuda(bg)
+    recon = ctdm_amd.RETRIEVE_ROWS if rows else 0
+    outs = {}
+    for label, mode, graph in (("batched", 1, False), ("graph", 1, True), ("graph replay", 1, True),
+                               ("plain", 0, False)):
+        maps = {k: torch.empty((A, V, V), dtype=torch.float32, device="cuda")
+                for k in ("wrapped", "unwrapped", "logamp")}
+        ri = torch.empty((V, V, V), dtype=torch.complex64, device="cuda")
+        cell = dict(holo=hd, bg=bd, angles=ang, V=V, iters=iters, optics=o, ri=ri, unwrap_flags=flags,
+                    recon_flags=recon, **maps)
+        ctx.reconstruct([cell], mode=mode, graph=graph)
+        torch.cuda.synchronize()
+        outs[label] = (ri.cpu().numpy(), {k: v.cpu().numpy() for k, v in maps.items()},
+                       cell["crop_used"].astuple())
+    crop = outs["batched"][2]
+    ref, ext = orc.reconstruct_cell(holo, bg, ang, V, crop, LAM, NM, o.dx, iters, unwrap_flags=flags,
+                                    recon_flags=recon, outputs=True)
+    for label, (ri, m, cr) in outs.items():
+        assert cr == crop, label
+        err = float(np.abs(ri - ref).max())
+        dw = _wrap(m["wrapped"].astype(np.float64) - ext["wrapped"])
+        d = m["unwrapped"].astype(np.float64) - ext["unwrapped"]
+        if flags & ctdm_amd.UNWRAP_CONGRUENT:
+            jumps = np.abs(d) > 1.0  # a 2 pi step at a half-integer least-squares value
+            assert jumps.mean() <= 1e-3, (label, jumps.mean())
+            d = np.where(jumps, 0.0, d)
+        rms = float(np.sqrt(np.mean(d ** 2)))
+        print(f"flags {flags:#x} rows {rows} {label}: max|dn| {err:.2e}, unwrapped rms {rms:.2e}, "
+              f"wrapped max {np.abs(dw).max():.1e}")
+        assert err <= 1e-5, label
+        assert rms <= 1e-4, label
+        assert np.abs(dw).max() <= 1e-3 and np.all(np.abs(m["wrapped"]) <= np.float32(np.pi)), label
+    assert np.array_equal(outs["graph"][0], outs["batched"][0])
+    assert np.array_equal(outs["graph replay"][0], outs["batched"][0])
+
+
+def _wrap(a):
+    return np.angle(np.exp(1j * a))

@@ -242,6 +242,20 @@ def test_rows_retrieval_recovers_phase_and_amplitude(orc):
     assert np.abs(r["logamp"][inner] - la_s[inner]).max() < 0.03
 
 
+def test_rows_phase_ref_is_calcwrapphase(orc):
+    """The row-wise retrieval's calcWrapPhase value (the difference of the two atan2, in (-2 pi, 2 pi),
+    HilbertTransform.cu:3630) is congruent to its wrapped phase modulo 2 pi, and takes values outside
+    (-pi, pi] where the two atan2 straddle the cut."""
+    N, k0 = 256, 72
+    sp, bg, _, _ = _rows_holograms(N, k0, phase_amp=9.0)
+    r = orc.retrieve_rows(sp, bg, k0 + N // 2)
+    ph, w = r["phase_ref"], r["wrapped"]
+    k = (ph - w) / (2 * np.pi)
+    assert np.abs(k - np.rint(k)).max() < 1e-9
+    assert np.all(np.abs(ph) < 2 * np.pi) and np.all(np.abs(w) <= np.pi)
+    assert np.any(np.abs(ph) > np.pi)
+
+
 def test_angle_from_image_closed_form(orc):
     """A cosine fringe at bins (kx, ky), ky > 0: the shifted peak in the searched band is (kx + N/2, ky + N/2),
     so fx = (N/2 - 1 - posX) / (N dx) = -(kx + 1) / (N dx) and the angle follows from asin."""
