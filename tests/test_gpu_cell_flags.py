@@ -92,3 +92,63 @@ def test_cell_unwrap_variant_matches_oracle(ctx, orc, cell_data, flags, rows):
 
 def _wrap(a):
     return np.angle(np.exp(1j * a))
+
+
+@pytest.mark.parametrize("recon", [ctdm_amd.DC_REFCOMPAT, ctdm_amd.POS_NO_SHIFT,
+                                   ctdm_amd.DC_REFCOMPAT | ctdm_amd.POS_NO_SHIFT])
+def test_cell_recon_flags_match_oracle(ctx, orc, cell_data, recon):
+    """The reference's measured-voxel test (F2.re != 0 && F2.im != 0, Recon_kernel.cu:406) and the volume
+    without its final octant shift, batched and plain, against the oracle's cell."""
+    A, H, V, holo, bg, ang = cell_data
+    iters = 2
+    o = ctdm_amd.optics_for(V, H)
+    got = {}
+    for mode in (1, 0):
+        ri = torch.empty((V, V, V), dtype=torch.complex64, device="cuda")
+        cell = dict(holo=_cuda(holo), bg=_cuda(bg), angles=ang, V=V, iters=iters, optics=o, ri=ri,
+                    recon_flags=recon)
+        ctx.reconstruct([cell], mode=mode)
+        torch.cuda.synchronize()
+        got[mode] = (ri.cpu().numpy(), cell["crop_used"].astuple())
+    ref = orc.reconstruct_cell(holo, bg, ang, V, got[1][1], LAM, NM, o.dx, iters, recon_flags=recon)
+    for mode, (ri, _) in got.items():
+        err = float(np.abs(ri - ref).max())
+        print(f"recon flags {recon:#x} mode {mode}: max|dn| {err:.2e}")
+        assert err <= 1e-5, mode
+    if recon & ctdm_amd.POS_NO_SHIFT:  # the unshifted volume is the shifted one moved by half a side
+        shifted = orc.reconstruct_cell(holo, bg, ang, V, got[1][1], LAM, NM, o.dx, iters,
+                                       recon_flags=recon & ~ctdm_amd.POS_NO_SHIFT)
+        assert np.abs(np.fft.fftshift(got[1][0]) - shifted).max() <= 1e-5
+
+
+def test_mixed_flag_cells_one_graph(ctx, cell_data):
+    """Cells with different retrievals and unwrapping variants in one two-lane graph (each lane's scratch,
+    the fused and unfused routes and the calcWrapPhase buffer interleaved): every volume equals its cell
+    run alone, on the first call and on replays."""
+    A, H, V, holo, bg, ang = cell_data
+    o = ctdm_amd.optics_for(V, H)
+    variants = [(ctdm_amd.UNWRAP_SINCOS | ctdm_amd.UNWRAP_CONGRUENT, 0),
+                (ctdm_amd.UNWRAP_SINCOS | ctdm_amd.UNWRAP_CONGRUENT | ctdm_amd.UNWRAP_REFCOMPAT_FAST, 0),
+                (ctdm_amd.UNWRAP_SINCOS | ctdm_amd.UNWRAP_CONGRUENT | ctdm_amd.UNWRAP_REFCOMPAT_FAST,
+                 ctdm_amd.RETRIEVE_ROWS),
+                (0, ctdm_amd.RETRIEVE_ROWS),
+                (ctdm_amd.UNWRAP_REFCOMPAT, ctdm_amd.MAPS_MEDIAN)]
+    hd, bd = _cuda(holo), _cuda(bg)
+
+    def cells():
+        return [dict(holo=hd, bg=bd, angles=ang, V=V, iters=2, optics=o, unwrap_flags=u, recon_flags=r,
+                     ri=torch.empty((V, V, V), dtype=torch.complex64, device="cuda")) for u, r in variants]
+
+    alone = []
+    for c in cells():
+        ctx.reconstruct([c], mode=1)
+        torch.cuda.synchronize()
+        alone.append(c["ri"].clone())
+    together = cells()
+    for rep in range(3):
+        for c in together:
+            c["ri"].fill_(float("nan"))
+        ctx.reconstruct(together, mode=1, graph=True)
+        torch.cuda.synchronize()
+        for i, (c, r) in enumerate(zip(together, alone)):
+            assert torch.equal(c["ri"], r), (rep, i)
