@@ -67,6 +67,15 @@ def test_goldstein_large_map_global_path(ctx, orc):
     print(f"2 x 1024^2: GPU {tg * 1e3:.1f} ms, oracle {tc:.2f} s")
 
 
+def test_goldstein_wide_and_tall_maps_global_path(ctx, orc):
+    """Non-square maps past the LDS budget (charge words and flood-fill masks in global memory): rows and
+    columns must not be exchanged anywhere on those paths; a wide map and a tall one in separate calls."""
+    for shape, seed in (((1, 640, 1536), 6), ((1, 1400, 720), 7)):
+        w = ctdm_amd.synth_wrapped(seed, *shape)
+        tg, tc, rr = _check(ctx, orc, w)
+        print(f"{shape}: GPU {tg * 1e3:.1f} ms, oracle {tc:.2f} s, residue density {(rr != 0).mean():.3f}")
+
+
 def test_goldstein_no_residues(ctx, orc):
     """A smooth ramp has no residues: no cuts, every pixel reached, unwrapped equal to the ramp."""
     H, W = 48, 70
