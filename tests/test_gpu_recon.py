@@ -71,6 +71,28 @@ def test_scatter_and_gather_values(ctx, orc):
     assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
 
 
+def test_scatter_accumulates_like_the_reference(ctx, orc):
+    """ctdm_ewald_scatter adds to F and hits (fillEdwardSphere accumulates into the caller's arrays,
+    Recon_kernel.cu:224-227): two calls with two halves of the angles into the same arrays equal one call
+    with all angles (hits exactly; values to FP32 rounding of the summation order), and equal the oracle's
+    accumulation starting from the same non-zero arrays."""
+    V, A = 64, 10
+    o = ctdm_amd.optics_for(V)
+    ang = _angles(A)
+    rng = np.random.default_rng(8)
+    spec = (rng.standard_normal((A, V, V)) + 1j * rng.standard_normal((A, V, V))).astype(np.complex64)
+    F0 = (rng.standard_normal((V, V, V)) + 1j * rng.standard_normal((V, V, V))).astype(np.complex64)
+    h0 = rng.integers(0, 3, size=(V, V, V)).astype(np.float32)
+    F, hits = ctx.ewald_scatter(_cuda(spec[:4]), ang[:4], V, o, F=_cuda(F0), hits=_cuda(h0))
+    F, hits = ctx.ewald_scatter(_cuda(spec[4:]), ang[4:], V, o, F=F, hits=hits)
+    Fa, ha = ctx.ewald_scatter(_cuda(spec), ang, V, o, F=_cuda(F0), hits=_cuda(h0))
+    assert torch.equal(hits, ha)
+    assert (F - Fa).abs().max().item() <= 1e-6 * Fa.abs().max().item()
+    Fr, hr = orc.ewald_scatter(spec, ang, V, LAM, NM, o.dx, F=F0.astype(np.complex128), hits=h0.astype(np.float64))
+    assert np.array_equal(hits.cpu().numpy(), hr)
+    assert _relmax(F.cpu().numpy(), Fr) <= 1e-6
+
+
 @pytest.mark.parametrize("V", [64, 128, 192, 256])
 def test_fft3d_matches_rocfft_and_oracle(ctx, orc, V):
     g = torch.Generator(device="cuda").manual_seed(V)
