@@ -32,6 +32,26 @@ def test_rows_retrieval_matches_oracle(ctx, orc, H):
         assert _rms(dw) <= 1e-4 and _rms(dl) <= 1e-4
 
 
+@pytest.mark.parametrize("k0", [40, 90, 100, 120])
+def test_rows_sideband_search_matches_oracle_across_carriers(ctx, orc, k0):
+    """The HistogramFT search over shifted columns [0.6 W, W) for carriers across that range, including
+    the reference's mirror of a peak within W/8 of the end to W - column (HilbertTransform.cu:1641): of a
+    256-wide image, carrier bins 40 and 90 peak inside the range (the resampling's x scale (W-1)/W moves
+    the peak by up to one bin), bins 100 and 120 within W/8 of the end and come back mirrored."""
+    N = 256
+    x = np.mgrid[0:N, 0:N][1].astype(np.float64)
+    bg = (2 + 2 * np.cos(2 * np.pi * k0 * x / N)).astype(np.float32)
+    got = ctx.find_sideband_rows(torch.from_numpy(bg).cuda())
+    ref = orc.find_sideband_rows(bg)
+    print(f"carrier bin {k0}: column {got} (oracle {ref})")
+    assert got == ref
+    peak = k0 + N // 2
+    if peak >= N - N // 8:
+        assert 0 < got <= N // 8
+    else:
+        assert abs(got - peak) <= 1
+
+
 def test_rows_retrieval_rejects_band_outside_spectrum(ctx):
     holo, bg = ctdm_amd.synth_hologram(1, 256)
     with pytest.raises(RuntimeError):

@@ -217,6 +217,16 @@ def test_rows_sideband_column_is_the_carrier(orc):
         assert orc.find_sideband_rows(bg) == k0 + N // 2
 
 
+def test_rows_sideband_mirrored_near_the_end(orc):
+    """sequence1DFFT mirrors a peak within W/8 of the end to W - column (HilbertTransform.cu:1641): a carrier
+    at bin 120 of 256 peaks at shifted column 247-248 and comes back as 8-9; one at bin 90 (column 218) is
+    kept."""
+    N = 256
+    for k0, lo, hi in ((120, 1, N // 8), (90, 217, 219)):
+        _, bg, _, _ = _rows_holograms(N, k0)
+        assert lo <= orc.find_sideband_rows(bg) <= hi, k0
+
+
 def test_rows_retrieval_identical_pair_is_zero(orc):
     sp, bg, _, _ = _rows_holograms(256, 72)
     r = orc.retrieve_rows(bg, bg, 72 + 128)
