@@ -152,3 +152,31 @@ def test_mixed_flag_cells_one_graph(ctx, cell_data):
         torch.cuda.synchronize()
         for i, (c, r) in enumerate(zip(together, alone)):
             assert torch.equal(c["ri"], r), (rep, i)
+
+
+@pytest.mark.parametrize("carrier", [(0.375, 0.375), (0.25, 0.375), (0.33, 0.2)])
+def test_cell_carriers_off_the_diagonal_and_at_the_edge(ctx, orc, carrier):
+    """Cells whose sideband window reaches the spectrum's edge (carrier 0.375 cycles/pixel: window
+    [384, 512) of a 512 spectrum, passband as wide as the window) or sits off the diagonal: the device
+    crop from the search, the batched pipeline (eager and replayed) and the plain one against the
+    oracle's cell with the same crop."""
+    A, H, V, iters = 8, 512, 128, 2
+    holo, bg, ang = ctdm_amd.synth_cell(71, A, ctdm_amd.synth_params(H=H, carrier=carrier))
+    o = ctdm_amd.optics_for(V, H)
+    hd, bd = _cuda(holo), _cuda(bg)
+    got = {}
+    for label, mode, graph in (("batched", 1, False), ("graph", 1, True), ("replay", 1, True), ("plain", 0, False)):
+        ri = torch.empty((V, V, V), dtype=torch.complex64, device="cuda")
+        cell = dict(holo=hd, bg=bd, angles=ang, V=V, iters=iters, optics=o, ri=ri)
+        ctx.reconstruct([cell], mode=mode, graph=graph)
+        torch.cuda.synchronize()
+        got[label] = (ri.cpu().numpy(), cell["crop_used"].astuple())
+    crop = got["batched"][1]
+    expect = (round(H / 2 + carrier[1] * H), round(H / 2 + carrier[0] * H))
+    assert crop[:2] == expect and crop == orc.find_sideband(bg[0]) + (V,)
+    ref = orc.reconstruct_cell(holo, bg, ang, V, crop, LAM, NM, o.dx, iters)
+    for label, (ri, cr) in got.items():
+        assert cr == crop, label
+        err = float(np.abs(ri - ref).max())
+        print(f"carrier {carrier} {label}: crop {cr}, max|dn| {err:.2e}")
+        assert err <= 1e-5, label
