@@ -189,6 +189,7 @@ def main() -> int:
 
     config4 = None if args.no_config4 else run_config4(ctx, args)
     ingest = None if args.no_extra else run_ingest(ctx, args, holo, bg, ang, optics)
+    rows = None if args.no_extra else run_rows(ctx, args, hd, bd, ang, optics)
     config1 = None if args.no_extra else run_config1(ctx, args)
     config3 = None if args.no_extra else run_config3(ctx, args)
     config5 = None if args.no_extra else run_config5(ctx, args)
@@ -205,7 +206,7 @@ def main() -> int:
         "speedup_vs_plain": plain_value and round(value / plain_value, 2),
         "ri_max_diff_batched_vs_plain": dn, "wall_ms_per_step": round(wall, 4),
         "roofline": roofline, "cpu_baseline": cpu, "config4": config4,
-        "config1": config1, "config3": config3, "config5": config5, "ingest": ingest,
+        "config1": config1, "config3": config3, "config5": config5, "ingest": ingest, "rows_retrieval": rows,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -311,6 +312,36 @@ def _time_calls(fn, reps):
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def run_rows(ctx, args, hd, bd, ang, optics):
+    """The headline cell through the reference's default retrieval (Algorithm B, QPI_Method 0,
+    HilbertTransform.cu:953-964): each hologram resampled to H/4 rows, row transforms, the W/4 band around
+    the sideband column, inverse row transforms; the rest of the cell as in the headline (V = W/4 = 256).
+    Batched and graph-replayed, timed like the headline."""
+    import torch
+
+    import ctdm_amd
+
+    log("step: row-wise retrieval cell (Algorithm B)")
+    ri = torch.empty((V, V, V), dtype=torch.complex64, device="cuda")
+    cell = dict(holo=hd, bg=bd, angles=ang, V=V, iters=ITERS, optics=optics, ri=ri,
+                recon_flags=ctdm_amd.RETRIEVE_ROWS)
+    ctx.reconstruct([cell], mode=1, graph=True)
+    for _ in range(args.warmup):
+        ctx.reconstruct([cell], mode=1, graph=True)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        ctx.reconstruct([cell], mode=1, graph=True)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    log(f"rows cell: {ms:.3f} ms/vol ({1000.0 / ms:.1f} vol/s), column {cell['crop_used'].col}")
+    return {"workload": "the headline cell with row-wise retrieval (Algorithm B), 256^3, 10 iterations",
+            "ms_per_volume": round(ms, 4), "volumes_per_s": round(1000.0 / ms, 2),
+            "sideband_column": cell["crop_used"].col}
 
 
 def run_ingest(ctx, args, holo, bg, ang, optics):
