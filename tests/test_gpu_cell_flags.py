@@ -180,3 +180,23 @@ def test_cell_carriers_off_the_diagonal_and_at_the_edge(ctx, orc, carrier):
         err = float(np.abs(ri - ref).max())
         print(f"carrier {carrier} {label}: crop {cr}, max|dn| {err:.2e}")
         assert err <= 1e-5, label
+
+
+def test_cell_with_many_angles(ctx, orc):
+    """400 angles (13 chunks of the gridding's 32-angle staging, a data-consistency list of up to 400
+    entries per pencil): batched and plain against the oracle's cell."""
+    A, H, V, iters = 400, 256, 64, 2
+    holo, bg, ang = ctdm_amd.synth_cell(81, A, ctdm_amd.synth_params(H=H))
+    o = ctdm_amd.optics_for(V, H)
+    got = {}
+    for mode in (1, 0):
+        ri = torch.empty((V, V, V), dtype=torch.complex64, device="cuda")
+        cell = dict(holo=_cuda(holo), bg=_cuda(bg), angles=ang, V=V, iters=iters, optics=o, ri=ri)
+        ctx.reconstruct([cell], mode=mode)
+        torch.cuda.synchronize()
+        got[mode] = (ri.cpu().numpy(), cell["crop_used"].astuple())
+    ref = orc.reconstruct_cell(holo, bg, ang, V, got[1][1], LAM, NM, o.dx, iters)
+    for mode, (ri, cr) in got.items():
+        err = float(np.abs(ri - ref).max())
+        print(f"{A} angles mode {mode}: max|dn| {err:.2e}")
+        assert cr == got[1][1] and err <= 1e-5
