@@ -345,6 +345,11 @@ class Context:
     def sync(self):
         self._call("ctdm_sync")
 
+    def set_stream(self, stream):
+        """ctdm_set_stream: later work goes to `stream` (a torch.cuda.Stream) after the current one drains."""
+        self._call("ctdm_set_stream", _vp(stream.cuda_stream))
+        self.stream = stream
+
     # -- retrieval
     def find_sideband(self, bg, M: int = 256, row_start: int = 0) -> Crop:
         H, W = bg.shape[-2:]

@@ -205,3 +205,30 @@ def test_scratch_and_graph_cache_stay_bounded(ctx):
     assert grown < 48
     for r in outs[1:]:
         assert torch.equal(r, outs[0])
+
+
+def test_context_on_its_own_streams(ctx):
+    """A context bound to a caller's stream runs there (eager, captured and replayed), and after
+    ctdm_set_stream replays its cached graph on the new stream: every volume equals the one from the
+    session context on torch's default stream, with other work queued on the default stream meanwhile."""
+    cells, _ = _cells([(6, 256, 64), (5, 256, 32)])
+    ctx.reconstruct(cells, mode=1)
+    torch.cuda.synchronize()
+    ref = [c["ri"].clone() for c in cells]
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    own = ctdm_amd.Context(0, stream=s1)
+    try:
+        for rep, s in enumerate((s1, s1, s1, s2, s2)):
+            if rep == 3:
+                own.set_stream(s2)
+            for c in cells:
+                c["ri"].fill_(float("nan"))
+            torch.cuda.synchronize()
+            own.reconstruct(cells, mode=1, graph=rep > 0)
+            busy = torch.randn(1 << 20, device="cuda").sum()  # unrelated work on the default stream
+            s.synchronize()
+            for i, (c, r) in enumerate(zip(cells, ref)):
+                assert torch.equal(c["ri"], r), (rep, i)
+            assert torch.isfinite(busy)
+    finally:
+        own.close()
