@@ -7,7 +7,7 @@
 //   2. branch cuts, deterministic raster order (the reference runs its search for all residues
 //      concurrently, which is racy; the raster order makes it reproducible). Three phases:
 //      k_gs_cand lists every residue's first hits on the initial charges in parallel, k_gs_commit (one
-//      wavefront per map, charges at 2 bits per pixel, in LDS when they fit, else in global memory)
+//      wavefront per map, presence at 1 bit per pixel, in LDS when it fits, else in global memory)
 //      walks the residues in order and takes the first listed hit still present, and k_gs_draw draws
 //      the chosen cuts in parallel.
 //   3. flood fill: level-synchronous breadth-first search over non-cut pixels from a fixed start; each
@@ -67,27 +67,28 @@ __global__ void k_gs_residues(const float* __restrict__ phi, int batch, int H, i
 }
 
 // ---------------------------------------------------------------- 2. branch cuts
-// Packed charges: 2 bits per pixel, 16 pixels per word; 0 = none, 1 = +1, 3 = -1. The words live in
-// LDS when a map's words fit, else in global memory; either way one wavefront reads and changes them
-// (in global memory through relaxed atomics, so its reads see its own clears).
+// Charges still present: 1 bit per pixel, 32 pixels per word. A charge keeps its sign until it is
+// removed, so the sign is read from the residue map. The words live in LDS when a map's words fit
+// (maps up to 1.3 M pixels in 160 KiB), else in global memory; either way one wavefront reads and
+// clears them (in global memory through atomics at device scope, which bypass the vector L1).
 template <bool LDSW>
-struct ChargeWords {
+struct Presence {
     unsigned* w;
     __device__ __forceinline__ unsigned get(int i) const
     {
         if constexpr (LDSW)
             return w[i];
         else
-            return __hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return __hip_atomic_load(w + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __device__ __forceinline__ int at(int pix) const { return (int)((get(pix >> 4) >> ((pix & 15) * 2)) & 3u); }
-    __device__ __forceinline__ void clear(int pix) const  // caller: one lane
+    __device__ __forceinline__ bool at(int pix) const { return (get(pix >> 5) >> (pix & 31)) & 1u; }
+    __device__ __forceinline__ void clear(int pix) const
     {
-        const unsigned m = ~(3u << ((pix & 15) * 2));
+        const unsigned m = ~(1u << (pix & 31));
         if constexpr (LDSW)
-            atomicAnd(w + (pix >> 4), m);
+            atomicAnd(w + (pix >> 5), m);
         else
-            __hip_atomic_fetch_and(w + (pix >> 4), m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_and(w + (pix >> 5), m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 };
 
@@ -202,12 +203,16 @@ __global__ __launch_bounds__(1024) void k_gs_cand(const int8_t* __restrict__ res
 
 __device__ __forceinline__ int sel4(const int4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
 
-// Phase 2: one wavefront per map, residues in raster order, charges 2 bits per pixel (LDS, or global
-// memory for maps whose words exceed LDS). Residues are taken 64 at a time, one per lane: each lane
-// reads, once, whether its residue and its listed hits are still present; the residues are then
-// committed in order from those flags, and after each commit every lane clears the flags of the pixels
-// that commit removed (a few vector compares), so the loop itself waits for no memory. The charge words
-// receive the same clears for later batches and rescans.
+// Phase 2: one wavefront per map, residues in raster order. Residues are taken 64 at a time, one per
+// lane. Each lane reads, once per batch, whether its residue and its listed hits are still present;
+// those flags are kept as wave-wide lane masks (one for the residues, one per list position), so a
+// residue's choice is a few scalar bit tests, and after each commit the masks lose the lanes whose
+// residue or listed hit that commit removed (vector compares of the two removed pixels against every
+// lane's pixels). The batch's commits collect in two registers (lane j: the j-th residue and its
+// choice) and are written, and their charges cleared, once at the end of the batch, so the loop
+// itself issues no memory operation. A residue whose listed charges are all gone clears the batch's
+// charges so far (clearing twice is harmless) and scans again on the current charges (the signs of
+// the charges it meets come from the residue map).
 template <bool LDSW>
 __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res, int H, int W,
                                                   const int* __restrict__ nres, const int2* __restrict__ rec_pm,
@@ -217,18 +222,30 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
     extern __shared__ unsigned lwords[];
     const int lane = threadIdx.x;
     const long long n = (long long)H * W;
-    const int nw = (int)((n + 15) / 16);
-    const ChargeWords<LDSW> ch{LDSW ? lwords : gwords + (long long)blockIdx.x * nw};
+    const int nw = (int)((n + 31) / 32);
+    const Presence<LDSW> pr{LDSW ? lwords : gwords + (long long)blockIdx.x * nw};
     const int8_t* r = res + blockIdx.x * n;
     for (int w = lane; w < nw; w += 64) {
+        const long long p0 = (long long)w * 32;
         unsigned v = 0;
-        for (int k = 0; k < 16; ++k) {
-            const long long p = (long long)w * 16 + k;
-            const int q = p < n ? r[p] : 0;
-            v |= (unsigned)(q == 1 ? 1 : (q == -1 ? 3 : 0)) << (2 * k);
+        if (p0 + 32 <= n && ((reinterpret_cast<uintptr_t>(r + p0) & 15) == 0)) {
+            const uint4 a = *reinterpret_cast<const uint4*>(r + p0);
+            const uint4 b = *reinterpret_cast<const uint4*>(r + p0 + 16);
+            const unsigned q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v |= (unsigned)(((q[k] >> (8 * e)) & 0xffu) != 0) << (4 * k + e);
+        } else {  // last word, or maps not on 16-byte boundaries: byte loads, all issued before use
+            int8_t q[32];
+#pragma unroll
+            for (int k = 0; k < 32; ++k) q[k] = r[min(p0 + k, n - 1)];
+#pragma unroll
+            for (int k = 0; k < 32; ++k) v |= (unsigned)(p0 + k < n && q[k] != 0) << k;
         }
-        ch.w[w] = v;
+        pr.w[w] = v;
     }
+    if constexpr (!LDSW) __threadfence();
     __syncthreads();
     const int total = nres[blockIdx.x];
     const int2* pm = rec_pm + blockIdx.x * n;
@@ -238,165 +255,136 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
     int emitted = 0;
     int2 nxt_pm = lane < total ? pm[lane] : make_int2(0, 0);
     int4 nxt_c = lane < total ? rc[lane] : make_int4(0, 0, 0, 0);
+    // The first batch's records are waited for here, so that no batch waits for the records of the one
+    // after it (loaded at its start) before its own work.
+    __builtin_amdgcn_s_waitcnt(0);
     for (int base = 0; base < total; base += 64) {
         const int2 mypm = nxt_pm;
         const int4 myc = nxt_c;
         const int nidx = base + 64 + lane;  // the next batch's records load while this one commits
         nxt_pm = nidx < total ? pm[nidx] : make_int2(0, 0);
         nxt_c = nidx < total ? rc[nidx] : make_int4(0, 0, 0, 0);
-        const int cnt_batch = min(64, total - base);
-        // presence flags: bit 0 the residue, bit 1 + k its k-th listed hit (border hits never go away)
+        const bool in = lane < min(64, total - base);
         const int cn = mypm.y & 7;
-        int valid = 0;
+        // cpix[k]: the pixel of the k-th listed hit when it is a charge (a commit can remove it), else -1
+        // (presence is read for every lane and position, at pixel 0 where there is nothing to read, so
+        // that the five reads issue together)
         int cpix[GS_K];
-        if (lane < cnt_batch) {
-            if (ch.at(mypm.x)) valid |= 1;
+        bool hv[GS_K];
 #pragma unroll
-            for (int k = 0; k < GS_K; ++k) {
-                const unsigned c = (unsigned)sel4(myc, k);
-                cpix[k] = (int)(c & 0x3fffffffu);
-                if (k < cn) {
-                    if ((c >> 30) != 3u || ch.at(cpix[k])) valid |= 2 << k;
-                }
-                if ((c >> 30) != 3u) cpix[k] = -1;  // not a charge: no clear can touch it
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < GS_K; ++k) cpix[k] = -1;
+        for (int k = 0; k < GS_K; ++k) {
+            const unsigned c = (unsigned)sel4(myc, k);
+            cpix[k] = (c >> 30) == 3u ? (int)(c & 0x3fffffffu) : -1;
+            const bool pres = pr.at(max(cpix[k], 0));
+            hv[k] = in && k < cn && (cpix[k] < 0 || pres);
         }
-        // lanes whose residue is still present, in raster order; residues balanced as partners of an
-        // earlier commit drop out of the mask without an iteration
-        unsigned long long live = __ballot(lane < cnt_batch && (valid & 1));
-        // Choice of lane ll from its listed hits (or itself on the border), valid while its flags hold.
-        auto listed_choice = [&](int ll, int vv, int pp, int mm) -> int {
-            if (mm & GS_META_SELF) return pp;  // kind 0
-            const int k = __ffs(vv >> 1) - 1;  // wave-uniform: four reads and a scalar select
-            const int c0 = __builtin_amdgcn_readlane(myc.x, ll), c1 = __builtin_amdgcn_readlane(myc.y, ll);
-            const int c2 = __builtin_amdgcn_readlane(myc.z, ll), c3 = __builtin_amdgcn_readlane(myc.w, ll);
-            return k == 0 ? c0 : (k == 1 ? c1 : (k == 2 ? c2 : c3));
-        };
+        const bool rpres = pr.at(mypm.x);
+        unsigned long long live = __ballot(in && rpres);
+        unsigned long long h0 = __ballot(hv[0]), h1 = __ballot(hv[1]), h2 = __ballot(hv[2]), h3 = __ballot(hv[3]);
+        int pa = 0, pb = 0, jb = 0;  // this batch's commits: lane j holds the j-th residue and its choice
         while (live) {
             const int l = __ffsll((long long)live) - 1;
-            live &= l == 63 ? 0ull : (~0ull << (l + 1));
-            const int v = __builtin_amdgcn_readlane(valid, l);
+            const unsigned long long bit = 1ull << l;
+            live &= ~bit;
             const int p = __builtin_amdgcn_readlane(mypm.x, l);
             const int meta = __builtin_amdgcn_readlane(mypm.y, l);
-            // Charge words in global memory (large maps): two residues at once when the second's
-            // decision cannot depend on the first's commit (both choose from their lists, the second's
-            // residue is not the first's partner, and the second's partner is neither pixel the first
-            // clears); pairs, clears and flags come out as two single steps would make them. This
-            // halves the clears' memory round trips; with the words in LDS it measured slower.
-            if constexpr (!LDSW) {
-                if (live && ((meta & GS_META_SELF) || (v & 0x1e))) {
-                    const int l2 = __ffsll((long long)live) - 1;
-                    const int v2 = __builtin_amdgcn_readlane(valid, l2);
-                    const int meta2 = __builtin_amdgcn_readlane(mypm.y, l2);
-                    if ((meta2 & GS_META_SELF) || (v2 & 0x1e)) {
-                        const int p2 = __builtin_amdgcn_readlane(mypm.x, l2);
-                        const int ch1 = listed_choice(l, v, p, meta);
-                        const int ch2 = listed_choice(l2, v2, p2, meta2);
-                        const int x1 = ((unsigned)ch1 >> 30) == 3u ? (ch1 & 0x3fffffff) : -2;
-                        const int x2 = ((unsigned)ch2 >> 30) == 3u ? (ch2 & 0x3fffffff) : -2;
-                        if (p2 != x1 && x2 != p && (x2 < 0 || x2 != x1)) {
-                            live &= l2 == 63 ? 0ull : (~0ull << (l2 + 1));
-                            if (lane == 0) {
-                                out[emitted] = make_int2(p, ch1);
-                                out[emitted + 1] = make_int2(p2, ch2);
-                                ch.clear(p);
-                                if (x1 >= 0) ch.clear(x1);
-                                ch.clear(p2);
-                                if (x2 >= 0) ch.clear(x2);
-                            }
-                            emitted += 2;
-                            int drop = (mypm.x == p) | (mypm.x == x1) | (mypm.x == p2) | (mypm.x == x2);
-#pragma unroll
-                            for (int k = 0; k < GS_K; ++k)
-                                drop |= ((cpix[k] == p) | (cpix[k] == x1) | (cpix[k] == p2) | (cpix[k] == x2)) << (k + 1);
-                            valid &= ~drop;
-                            live &= __ballot(valid & 1);
-                            continue;
-                        }
-                    }
-                }
-            }
-            int chosen = 0;  // encoded choice; kinds 2 and 3 set the sign bit, so `found` tracks it
-            bool found = false;
-            if (meta & GS_META_SELF) {
-                chosen = p;  // kind 0
-                found = true;
-            } else if (v & 0x1e) {
-                const int k = __ffs(v >> 1) - 1;  // wave-uniform: four reads and a scalar select
-                const int c0 = __builtin_amdgcn_readlane(myc.x, l), c1 = __builtin_amdgcn_readlane(myc.y, l);
-                const int c2 = __builtin_amdgcn_readlane(myc.z, l), c3 = __builtin_amdgcn_readlane(myc.w, l);
-                chosen = k == 0 ? c0 : (k == 1 ? c1 : (k == 2 ? c2 : c3));
-                found = true;
-            } else if (!(meta & (GS_META_TERM | GS_META_EXHAUSTED))) {
+            // the first listed hit still present, or the residue itself on the border (kind 0); -1 when
+            // there is none (no choice encodes to -1: pixels stay below 2^30 - 1). Scalar selects, no branch.
+            const int c0 = __builtin_amdgcn_readlane(myc.x, l), c1 = __builtin_amdgcn_readlane(myc.y, l);
+            const int c2 = __builtin_amdgcn_readlane(myc.z, l), c3 = __builtin_amdgcn_readlane(myc.w, l);
+            int chosen = (h3 & bit) ? c3 : -1;
+            chosen = (h2 & bit) ? c2 : chosen;
+            chosen = (h1 & bit) ? c1 : chosen;
+            chosen = (h0 & bit) ? c0 : chosen;
+            chosen = (meta & GS_META_SELF) ? p : chosen;
+            bool found = chosen != -1;
+            if (!found) {
+                if (meta & (GS_META_TERM | GS_META_EXHAUSTED)) continue;  // nothing within reach: unbalanced
                 // every listed charge is gone: scan again on the current charges
-                const int cp2 = ch.at(p);
-                const int q = cp2 == 1 ? 1 : -1;
+                if (lane < jb) {
+                    pr.clear(pa);
+                    if (((unsigned)pb >> 30) == 3u) pr.clear(pb & 0x3fffffff);
+                }
+                if constexpr (!LDSW) __builtin_amdgcn_s_waitcnt(0);
+                const int q = r[p];
                 const int i = p % W, j = p / W;
-                for (int rr = 1; rr < R && !found; ++rr) {
-                    const int m1 = max(i - rr, 0), m2 = min(i + rr, W - 1);
-                    const int n1 = max(j - rr, 0), n2 = min(j + rr, H - 1);
-                    const int nrow = n2 - n1 + 1, ncol = m2 - m1 + 1;
-                    const int P = 2 * nrow + 2 * (ncol - 2);
-                    for (int q0 = 0; q0 < P && !found; q0 += 64) {
-                        const int qi = q0 + lane;
-                        int c = 0;
-                        bool hit_here = false;
-                        if (qi < P) {
-                            int mm, nn;
-                            if (qi < nrow) {
-                                mm = m1;
-                                nn = n1 + qi;
-                            } else if (qi < nrow + 2 * (ncol - 2)) {
-                                const int q2 = qi - nrow;
-                                mm = m1 + 1 + q2 / 2;
-                                nn = (q2 & 1) ? n2 : n1;
-                            } else {
-                                mm = m2;
-                                nn = n1 + (qi - nrow - 2 * (ncol - 2));
-                            }
-                            if (mm <= 0 || mm >= W - 1) {
-                                c = (int)((1u << 30) | (unsigned)(mm + W * j));
-                                hit_here = true;
-                            } else if (nn <= 0 || nn >= H - 1) {
-                                c = (int)((2u << 30) | (unsigned)(i + W * nn));
-                                hit_here = true;
-                            } else {
-                                const int pp = mm + W * nn;
-                                const int cc = ch.at(pp);
-                                if (q * (cc == 1 ? 1 : (cc == 3 ? -1 : 0)) == -1) {
-                                    c = (int)((3u << 30) | (unsigned)pp);
-                                    hit_here = true;
-                                }
-                            }
+                // The box rings 1, 2, ... (radius below R) in scan order, 64 positions per round: each
+                // lane takes the next position (ring lr, index lq on that ring), several rings per round
+                // while they are short; the first hit in scan order is the lowest lane with a hit.
+                int rc = 1, off0 = 0;  // next position to scan: ring rc, index off0
+                while (!found && rc < R) {
+                    int lr = R, lq = 0;  // lanes past the last ring scan nothing
+                    int start = 0, rr = rc, off = off0;
+                    while (start < 64 && rr < R) {
+                        const int P = 2 * (min(j + rr, H - 1) - max(j - rr, 0) + 1) +
+                                      2 * (min(i + rr, W - 1) - max(i - rr, 0) - 1);
+                        const int take = min(P - off, 64 - start);
+                        if (lane >= start && lane < start + take) {
+                            lr = rr;
+                            lq = off + lane - start;
                         }
-                        const unsigned long long hit = __ballot(hit_here);
-                        if (hit) {
-                            chosen = __builtin_amdgcn_readlane(c, __ffsll((long long)hit) - 1);
-                            found = true;
+                        start += take;
+                        off += take;
+                        if (off == P) {
+                            ++rr;
+                            off = 0;
                         }
                     }
+                    int c = 0;
+                    bool hit_here = false;
+                    if (lr < R) {
+                        // ring lr's positions: its left column top to bottom, then the columns between
+                        // alternately on its top and bottom rows, then its right column top to bottom
+                        const int m1 = max(i - lr, 0), m2 = min(i + lr, W - 1);
+                        const int n1 = max(j - lr, 0), n2 = min(j + lr, H - 1);
+                        const int nrow = n2 - n1 + 1, nmid = 2 * (m2 - m1 - 1);
+                        const int q2 = lq - nrow;
+                        const bool left = lq < nrow, mid = !left && q2 < nmid;
+                        const int mm = left ? m1 : (mid ? m1 + 1 + (q2 >> 1) : m2);
+                        const int nn = left ? n1 + lq : (mid ? ((q2 & 1) ? n2 : n1) : n1 + (q2 - nmid));
+                        const int pp = mm + W * nn;
+                        const bool present = pr.at(pp) && r[pp] == -q;
+                        if (mm <= 0 || mm >= W - 1) {
+                            c = (int)((1u << 30) | (unsigned)(mm + W * j));
+                            hit_here = true;
+                        } else if (nn <= 0 || nn >= H - 1) {
+                            c = (int)((2u << 30) | (unsigned)(i + W * nn));
+                            hit_here = true;
+                        } else if (present) {
+                            c = (int)((3u << 30) | (unsigned)pp);
+                            hit_here = true;
+                        }
+                    }
+                    const unsigned long long hit = __ballot(hit_here);
+                    if (hit) {
+                        chosen = __builtin_amdgcn_readlane(c, __ffsll((long long)hit) - 1);
+                        found = true;
+                    }
+                    rc = rr;
+                    off0 = off;
                 }
+                if (!found) continue;  // nothing within reach: the residue stays unbalanced
             }
-            if (!found) continue;  // nothing within reach: the residue stays unbalanced
-            const int pix = ((unsigned)chosen >> 30) == 3u ? (chosen & 0x3fffffff) : -1;
-            if (lane == 0) {
-                out[emitted] = make_int2(p, chosen);
-                ch.clear(p);
-                if (pix >= 0) ch.clear(pix);
-            }
-            ++emitted;
-            // the batch's flags follow the two clears (pixels are >= 0; -2 never matches, and
-            // cpix[k] = -1 marks a hit that no clear can touch)
-            const int q2 = pix >= 0 ? pix : -2;
-            int drop = (mypm.x == p) | (mypm.x == q2);
-#pragma unroll
-            for (int k = 0; k < GS_K; ++k) drop |= ((cpix[k] == p) | (cpix[k] == q2)) << (k + 1);
-            valid &= ~drop;
-            live &= __ballot(valid & 1);
+            pa = lane == jb ? p : pa;
+            pb = lane == jb ? chosen : pb;
+            ++jb;
+            // the masks follow the two removed charges (pixels are >= 0; -2 never matches, and
+            // cpix[k] = -1 marks a hit that no commit can remove)
+            const int pix = ((unsigned)chosen >> 30) == 3u ? (chosen & 0x3fffffff) : -2;
+            live &= ~__ballot(mypm.x == pix);
+            h0 &= ~(__ballot(cpix[0] == p) | __ballot(cpix[0] == pix));
+            h1 &= ~(__ballot(cpix[1] == p) | __ballot(cpix[1] == pix));
+            h2 &= ~(__ballot(cpix[2] == p) | __ballot(cpix[2] == pix));
+            h3 &= ~(__ballot(cpix[3] == p) | __ballot(cpix[3] == pix));
         }
+        // the batch's pairs, in commit order, and its charges cleared for the later batches
+        if (lane < jb) {
+            out[emitted + lane] = make_int2(pa, pb);
+            pr.clear(pa);
+            if (((unsigned)pb >> 30) == 3u) pr.clear(pb & 0x3fffffff);
+        }
+        if constexpr (!LDSW) __builtin_amdgcn_s_waitcnt(0);
+        emitted += jb;
     }
     if (lane == 0) npairs[blockIdx.x] = emitted;
 }
@@ -735,7 +723,7 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
     int* qa = (int*)c.buf("gs.qa", sizeof(int) * tot);
     int* qb = (int*)c.buf("gs.qb", sizeof(int) * tot);
     hipLaunchKernelGGL(k_gs_residues, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, batch, H, W, r);
-    const long long nw = (n + 15) / 16, nb = (n + 31) / 32;
+    const long long nw = (n + 31) / 32, nb = (n + 31) / 32;
     const size_t lds_max = (size_t)(c.max_lds > 0 ? c.max_lds : 64 * 1024);
     // branch cuts in three phases: candidates in parallel, raster-order commit, drawing
     const size_t commit_lds = sizeof(unsigned) * (size_t)nw;
@@ -753,7 +741,7 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
     if (commit_lds <= lds_max) {
         hipLaunchKernelGGL(k_gs_commit<true>, dim3(batch), dim3(64), commit_lds, c.stream, r, H, W, nres, rec_pm,
                            rec_c, nullptr, pairs, npairs);
-    } else {  // a map's charge words exceed LDS: they stay in global memory
+    } else {  // a map's presence bits exceed LDS: they stay in global memory
         unsigned* gw = (unsigned*)c.buf("gs.words", sizeof(unsigned) * (size_t)nw * batch);
         hipLaunchKernelGGL(k_gs_commit<false>, dim3(batch), dim3(64), 0, c.stream, r, H, W, nres, rec_pm, rec_c,
                            gw, pairs, npairs);
