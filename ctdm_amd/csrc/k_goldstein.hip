@@ -12,10 +12,11 @@
 //      the chosen cuts in parallel.
 //   3. flood fill: level-synchronous breadth-first search over non-cut pixels from a fixed start; each
 //                     new pixel takes k from its first neighbour of the previous level in the order
-//                     right, left, below, above. Maps whose masks fit in LDS: k_gs_bfs_par (one
-//                     workgroup per map, one barrier per level) records each pixel's parent, and
-//                     k_gs_ptr_init / k_gs_ptr_jump (whole device) sum the steps along the parent chains
-//                     by pointer jumping. Larger maps: k_gs_bfs, with k along the levels.
+//                     right, left, below, above. Maps whose cut bits fit in LDS (up to 1.3 M pixels):
+//                     k_gs_bfs_lev (one workgroup per map, one barrier per level) stores each pixel's
+//                     level, k_gs_par_lev takes the parents from the levels, and k_gs_ptr_init /
+//                     k_gs_ptr_jump (whole device) sum the steps along the parent chains by pointer
+//                     jumping. Larger maps: k_gs_bfs, with k along the levels.
 //   4. k_gs_finish    one thread per pixel: interior cut pixels next to a reached pixel take k from
 //                     it (same order); unwrapped = phi + 2 pi k, or 0 where unreached.
 // Float operations that decide integers are written as single rounded operations and this file is
@@ -67,6 +68,31 @@ __global__ void k_gs_residues(const float* __restrict__ phi, int batch, int H, i
 }
 
 // ---------------------------------------------------------------- 2. branch cuts
+// Word w of the bit array of a byte map's nonzero entries (bit b: byte 32 w + b; bytes past n read as 0).
+// Two 16-byte loads where the word's bytes sit on a 16-byte boundary, else byte loads at clamped
+// addresses, all issued before use.
+__device__ __forceinline__ unsigned nonzero_bits(const uint8_t* __restrict__ m, long long n, int w)
+{
+    const long long p0 = (long long)w * 32;
+    unsigned v = 0;
+    if (p0 + 32 <= n && ((reinterpret_cast<uintptr_t>(m + p0) & 15) == 0)) {
+        const uint4 a = *reinterpret_cast<const uint4*>(m + p0);
+        const uint4 b = *reinterpret_cast<const uint4*>(m + p0 + 16);
+        const unsigned q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v |= (unsigned)(((q[k] >> (8 * e)) & 0xffu) != 0) << (4 * k + e);
+    } else {
+        uint8_t q[32];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) q[k] = m[min(p0 + k, n - 1)];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) v |= (unsigned)(p0 + k < n && q[k] != 0) << k;
+    }
+    return v;
+}
+
 // Charges still present: 1 bit per pixel, 32 pixels per word. A charge keeps its sign until it is
 // removed, so the sign is read from the residue map. The words live in LDS when a map's words fit
 // (maps up to 1.3 M pixels in 160 KiB), else in global memory; either way one wavefront reads and
@@ -225,26 +251,7 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
     const int nw = (int)((n + 31) / 32);
     const Presence<LDSW> pr{LDSW ? lwords : gwords + (long long)blockIdx.x * nw};
     const int8_t* r = res + blockIdx.x * n;
-    for (int w = lane; w < nw; w += 64) {
-        const long long p0 = (long long)w * 32;
-        unsigned v = 0;
-        if (p0 + 32 <= n && ((reinterpret_cast<uintptr_t>(r + p0) & 15) == 0)) {
-            const uint4 a = *reinterpret_cast<const uint4*>(r + p0);
-            const uint4 b = *reinterpret_cast<const uint4*>(r + p0 + 16);
-            const unsigned q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) v |= (unsigned)(((q[k] >> (8 * e)) & 0xffu) != 0) << (4 * k + e);
-        } else {  // last word, or maps not on 16-byte boundaries: byte loads, all issued before use
-            int8_t q[32];
-#pragma unroll
-            for (int k = 0; k < 32; ++k) q[k] = r[min(p0 + k, n - 1)];
-#pragma unroll
-            for (int k = 0; k < 32; ++k) v |= (unsigned)(p0 + k < n && q[k] != 0) << k;
-        }
-        pr.w[w] = v;
-    }
+    for (int w = lane; w < nw; w += 64) pr.w[w] = nonzero_bits(reinterpret_cast<const uint8_t*>(r), n, w);
     if constexpr (!LDSW) __threadfence();
     __syncthreads();
     const int total = nres[blockIdx.x];
@@ -523,9 +530,9 @@ __global__ __launch_bounds__(1024) void k_gs_bfs(const float* __restrict__ phi, 
 __device__ __forceinline__ bool tbit(const unsigned* b, long long p) { return (b[p >> 5] >> (p & 31)) & 1u; }
 
 // ---------------------------------------------------------------- 3b. flood fill in two stages
-// For maps whose masks fit in LDS. Stage 1 (k_gs_bfs_par, one workgroup per map) is the level-synchronous
-// search of k_gs_bfs with the same levels and parents, except that each new pixel records only which
-// neighbour is its parent (the first of the previous level in the order right, left, below, above) as a
+// For maps whose cut bits fit in LDS (up to 1.3 M pixels in 160 KiB). Stage 1 (k_gs_bfs_lev, one workgroup
+// per map, then k_gs_par_lev over every pixel) finds the levels of the search of k_gs_bfs and each reached
+// pixel's parent (the first neighbour of the previous level in the order right, left, below, above) as a
 // byte. Stage 2 (k_gs_ptr_init, k_gs_ptr_jump; the whole device) gives every reached pixel the pair
 // (parent, -step2pi(phi - phi_parent)) and contracts the parent chains by pointer jumping, (ptr, sum) <-
 // (ptr[ptr], sum + sum[ptr]), until each pointer is the start pixel, whose pair is (itself, 0). Then sum
@@ -533,28 +540,23 @@ __device__ __forceinline__ bool tbit(const unsigned* b, long long p) { return (b
 // sums do not depend on it). Unreached pixels keep ptr = -1. Parent codes: 0 unreached, 1..4 right, left,
 // below, above, 5 the start pixel.
 //
-// The search keeps, per pixel, 2 bits in LDS: 0 unvisited, else 1 + (level mod 3). Neighbouring pixels'
-// levels differ by at most one, so among the neighbours of a pixel of level d + 1 the code 1 + (d mod 3)
-// identifies level d exactly, and a set code means visited. A level is then one pass and one barrier: a
-// thread expanding a pixel of level d claims an unvisited neighbour q by setting q's code with an atomic
-// OR (the returned word tells whether q was still unvisited; racing claimants write the same code), and
-// the claimant reads q's parent from the codes of level d, which no one changes during the pass.
-__device__ __forceinline__ int code2(const unsigned* c, int p) { return (int)((c[p >> 4] >> ((p & 15) * 2)) & 3u); }
-
-__global__ __launch_bounds__(1024) void k_gs_bfs_par(const uint8_t* __restrict__ cuts, int H, int W, int Q,
-                                                     uint8_t* __restrict__ par, int* __restrict__ qa,
+// The search keeps one bit per pixel in LDS, set for cut pixels and for visited ones, so that an atomic
+// OR claims a pixel (the returned word tells whether it was still free; a level is one pass and one
+// barrier); each claimed pixel's level goes to global memory (stored, never read during the search). The
+// parents are taken afterwards from the final levels: levels are the shortest path lengths from the start
+// pixel, the same whatever order the search claims pixels in.
+__global__ __launch_bounds__(1024) void k_gs_bfs_lev(const uint8_t* __restrict__ cuts, int H, int W, int Q,
+                                                     int* __restrict__ level, int* __restrict__ qa,
                                                      int* __restrict__ qb, int* __restrict__ maxdepth)
 {
     extern __shared__ unsigned sm[];
-    __shared__ int cnt[3];
+    __shared__ int cnt[2];
     const long long n = (long long)H * W;
-    const int nb = (int)((n + 31) / 32);   // cut bit words
-    const int nc2 = (int)((n + 15) / 16);  // 2-bit code words
-    unsigned* cutb = sm;
-    unsigned* code = sm + nb;
-    int* lq[2] = {reinterpret_cast<int*>(sm + nb + nc2), reinterpret_cast<int*>(sm + nb + nc2) + Q};
+    const int nb = (int)((n + 31) / 32);
+    unsigned* blk = sm;  // cut or visited
+    int* lq[2] = {reinterpret_cast<int*>(sm + nb), reinterpret_cast<int*>(sm + nb) + Q};
     const uint8_t* cut = cuts + blockIdx.x * n;
-    uint8_t* pc = par + blockIdx.x * n;  // all 0 (unreached) on entry
+    int* lev = level + blockIdx.x * n;  // all -1 (unreached) on entry
     int* gq[2] = {qa + blockIdx.x * n, qb + blockIdx.x * n};
     auto qget = [&](int which, int i) -> int { return i < Q ? lq[which][i] : ld_wg(gq[which] + i); };
     auto qput = [&](int which, int i, int v) {
@@ -563,64 +565,80 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_par(const uint8_t* __restrict__
         else
             st_wg(gq[which] + i, v);
     };
-    for (int w = threadIdx.x; w < nb; w += blockDim.x) {
-        unsigned v = 0;
-        for (int b = 0; b < 32; ++b) {
-            const long long p = (long long)w * 32 + b;
-            if (p < n && cut[p]) v |= 1u << b;
-        }
-        cutb[w] = v;
-    }
-    for (int w = threadIdx.x; w < nc2; w += blockDim.x) code[w] = 0u;
+    for (int w = threadIdx.x; w < nb; w += blockDim.x) blk[w] = nonzero_bits(cut, n, w);
     __syncthreads();
     if (threadIdx.x == 0) {
         int sx = W / 2, sy = min(20, H - 1);
-        while (sx >= 0 && sy < H && tbit(cutb, sx + (long long)W * sy)) {
+        while (sx >= 0 && sy < H && tbit(blk, sx + (long long)W * sy)) {
             --sx;
             ++sy;
         }
         cnt[0] = 0;
         if (sx >= 0 && sy < H) {
             const int s0 = sx + W * sy;
-            code[s0 >> 4] |= 1u << ((s0 & 15) * 2);  // level 0: code 1
-            pc[s0] = 5;
+            blk[s0 >> 5] |= 1u << (s0 & 31);
+            lev[s0] = 0;
             qput(0, 0, s0);
             cnt[0] = 1;
         }
         cnt[1] = 0;
-        cnt[2] = 0;
     }
     __syncthreads();
     const int dx4[4] = {1, -1, 0, 0}, dy4[4] = {0, 0, 1, -1};
     int d = 0;
     for (;; ++d) {
-        const int c0 = d % 3, c1 = (d + 1) % 3, c2 = (d + 2) % 3;
-        const int nc = cnt[c0];
-        if (nc == 0) break;
         const int cur = d & 1, nxt = cur ^ 1;
-        const unsigned cd = 1u + (unsigned)(d % 3), cn = 1u + (unsigned)((d + 1) % 3);  // codes of d, d + 1
-        if (threadIdx.x == 0) cnt[c2] = 0;  // the next level appends there after this pass's barrier
+        const int nc = cnt[cur];
+        if (nc == 0) break;
         for (int e = threadIdx.x; e < 4 * nc; e += blockDim.x) {  // one (pixel, direction) per thread
             const int p = qget(cur, e >> 2), t = e & 3;
             const int nx = p % W + dx4[t], ny = p / W + dy4[t];
             if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
             const int q = nx + W * ny;
-            if (tbit(cutb, q) || code2(code, q)) continue;
-            const unsigned sh = (unsigned)(q & 15) * 2;
-            if ((atomicOr(code + (q >> 4), cn << sh) >> sh) & 3u) continue;  // claimed by another thread
-            for (int u = 0; u < 4; ++u) {  // the parent: first neighbour of q on level d
-                const int rx = nx + dx4[u], ry = ny + dy4[u];
-                if (rx < 0 || rx >= W || ry < 0 || ry >= H) continue;
-                if ((unsigned)code2(code, rx + W * ry) == cd) {
-                    pc[q] = (uint8_t)(1 + u);
-                    break;
-                }
-            }
-            qput(nxt, reserve_slot(&cnt[c1]), q);
+            const unsigned bit = 1u << (q & 31);
+            if (blk[q >> 5] & bit) continue;
+            if (atomicOr(blk + (q >> 5), bit) & bit) continue;  // cut, or claimed by another thread
+            lev[q] = d + 1;
+            qput(nxt, reserve_slot(&cnt[nxt]), q);
         }
+        __syncthreads();
+        if (threadIdx.x == 0) cnt[cur] = 0;  // the level after next appends there, after the next barrier
         __syncthreads();
     }
     if (threadIdx.x == 0 && d > 1) atomicMax(maxdepth, d - 1);  // longest parent chain of the map
+}
+
+// Parent codes from the levels of k_gs_bfs_lev: a reached pixel of level d + 1 takes its first neighbour
+// of level d in the order right, left, below, above (cut and unreached pixels have level -1); the start
+// pixel (level 0) gets 5, unreached pixels 0.
+__global__ void k_gs_par_lev(const int* __restrict__ level, int batch, int H, int W, uint8_t* __restrict__ par)
+{
+    const long long n = (long long)H * W;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * batch) return;
+    const int* lev = level + (g / n) * n;
+    const int p = (int)(g % n), x = p % W, y = p / W;
+    const int d = lev[p];
+    uint8_t code = 0;
+    if (d == 0) {
+        code = 5;
+    } else if (d > 0) {
+        if (x + 1 < W && lev[p + 1] == d - 1)
+            code = 1;
+        else if (x > 0 && lev[p - 1] == d - 1)
+            code = 2;
+        else if (y + 1 < H && lev[p + W] == d - 1)
+            code = 3;
+        else if (y > 0 && lev[p - W] == d - 1)
+            code = 4;
+    }
+    par[g] = code;
+}
+
+__global__ void k_gs_fill_i32(int* __restrict__ p, long long n, int v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // Pair (ptr, sum) per pixel, map-local ptr; the start pixel points to itself.
@@ -750,18 +768,20 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
     hipLaunchKernelGGL(k_gs_draw, dim3((unsigned)((n + 255) / 256), (unsigned)batch), dim3(256), 0, c.stream,
                        pairs, npairs, H, W, ct);
     CTDM_CHECK_LAUNCH("k_gs_draw");
-    const size_t masks = sizeof(unsigned) * (size_t)(nb + (n + 15) / 16);  // cut bits and 2-bit level codes
-    if (masks + 64 + 2 * sizeof(int) * 256 <= lds_max) {  // the queues get what LDS has left (>= 256 each)
-        const int Q = (int)std::min<long long>(n, (long long)((lds_max - 64 - masks) / (2 * sizeof(int))));
-        const size_t bfs_lds = masks + 2 * sizeof(int) * (size_t)Q;
-        // two-stage fill: parent directions by the search, 2 pi counts by pointer jumping
+    const size_t bits = sizeof(unsigned) * (size_t)nb;  // cut-or-visited bits
+    if (bits + 64 + 2 * sizeof(int) * 256 <= lds_max) {  // the queues get what LDS has left (>= 256 each)
+        // two-stage fill: levels by the search and parent directions from them, 2 pi counts by pointer jumping
+        const int Q = (int)std::min<long long>(n, (long long)((lds_max - 64 - bits) / (2 * sizeof(int))));
         uint8_t* par = (uint8_t*)c.buf("gs.par", (size_t)tot);
         int2* pr = (int2*)c.buf("gs.pr", sizeof(int2) * (size_t)tot);
         int* maxd = (int*)c.buf("gs.maxd", sizeof(int));
-        hipLaunchKernelGGL(k_gs_fill_u8, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, par, tot);
+        int* lev = (int*)c.buf("gs.level", sizeof(int) * tot);
         hipLaunchKernelGGL(k_gs_fill_u8, dim3(1), dim3(256), 0, c.stream, (uint8_t*)maxd, (long long)sizeof(int));
-        hipLaunchKernelGGL(k_gs_bfs_par, dim3(batch), dim3(1024), bfs_lds, c.stream, ct, H, W, Q, par, qa, qb, maxd);
-        CTDM_CHECK_LAUNCH("k_gs_bfs_par");
+        hipLaunchKernelGGL(k_gs_fill_i32, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, lev, tot, -1);
+        hipLaunchKernelGGL(k_gs_bfs_lev, dim3(batch), dim3(1024), bits + 2 * sizeof(int) * (size_t)Q, c.stream, ct,
+                           H, W, Q, lev, qa, qb, maxd);
+        CTDM_CHECK_LAUNCH("k_gs_bfs_lev");
+        hipLaunchKernelGGL(k_gs_par_lev, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, lev, batch, H, W, par);
         hipLaunchKernelGGL(k_gs_ptr_init, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, par, batch, H, W,
                            pr);
         // enough rounds for any chain in a map (2^rounds >= n); rounds past the longest chain return at once

@@ -60,25 +60,28 @@ def test_goldstein_config3_bit_exact(ctx, orc):
     assert dens > 0.01
 
 
-def test_goldstein_large_map_global_path(ctx, orc):
-    """1024^2 maps: the commit's presence bits (128 KiB) stay in LDS, the flood fill's masks exceed it and
-    take the global-memory variant."""
+def test_goldstein_large_maps_level_path(ctx, orc):
+    """1024^2 maps: the commit's presence bits (128 KiB) and the flood fill's cut bits stay in LDS, the
+    fill's levels go to global memory and the parents are taken from them afterwards."""
     w = ctdm_amd.synth_wrapped(5, 2, 1024, 1024)
     tg, tc, rr = _check(ctx, orc, w)
     print(f"2 x 1024^2: GPU {tg * 1e3:.1f} ms, oracle {tc:.2f} s")
 
 
 def test_goldstein_commit_global_presence(ctx, orc):
-    """A map of 1.57 M pixels: its presence bits (192 KiB) exceed 160 KiB of LDS, so the commit keeps them
-    in global memory (device-scope atomics, cleared once per batch and before each rescan)."""
-    w = ctdm_amd.synth_wrapped(8, 1, 1536, 1024)
-    tg, tc, rr = _check(ctx, orc, w)
-    print(f"1 x 1536 x 1024: GPU {tg * 1e3:.1f} ms, oracle {tc:.2f} s, residue density {(rr != 0).mean():.3f}")
+    """Maps of 1.57 M pixels, a tall one and a wide one: their presence bits and cut bits (192 KiB each)
+    exceed 160 KiB of LDS, so the commit keeps the presence bits in global memory (device-scope atomics,
+    cleared once per batch and before each rescan) and the flood fill runs entirely in global memory."""
+    for shape, seed in (((1, 1536, 1024), 8), ((1, 1024, 1536), 9)):
+        w = ctdm_amd.synth_wrapped(seed, *shape)
+        tg, tc, rr = _check(ctx, orc, w)
+        print(f"{shape}: GPU {tg * 1e3:.1f} ms, oracle {tc:.2f} s, residue density {(rr != 0).mean():.3f}")
 
 
-def test_goldstein_wide_and_tall_maps_global_path(ctx, orc):
-    """Non-square maps past the flood fill's LDS budget (its masks in global memory): rows and columns
-    must not be exchanged anywhere on those paths; a wide map and a tall one in separate calls."""
+def test_goldstein_wide_and_tall_maps_level_path(ctx, orc):
+    """Non-square maps past the LDS budget of the flood fill's 2-bit codes (cut bits in LDS, levels in
+    global memory): rows and columns must not be exchanged anywhere on those paths; a wide map and a tall
+    one in separate calls."""
     for shape, seed in (((1, 640, 1536), 6), ((1, 1400, 720), 7)):
         w = ctdm_amd.synth_wrapped(seed, *shape)
         tg, tc, rr = _check(ctx, orc, w)
