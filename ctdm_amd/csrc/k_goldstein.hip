@@ -189,7 +189,9 @@ __global__ __launch_bounds__(1024) void k_gs_cand(const int8_t* __restrict__ res
     int cand[GS_K] = {0, 0, 0, 0};
     int cnt = 0, meta = 0;
     if (i <= 0 || i >= W - 1 || j <= 0 || j >= H - 1) {
-        meta = GS_META_SELF;
+        meta = GS_META_SELF;  // its one listed hit is itself (kind 0), which no commit removes
+        cand[0] = (int)p;
+        cnt = 1;
     } else {
         const int R = min(901, W / 2);
         bool stop = false;
@@ -294,18 +296,24 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
             const unsigned long long bit = 1ull << l;
             live &= ~bit;
             const int p = __builtin_amdgcn_readlane(mypm.x, l);
-            const int meta = __builtin_amdgcn_readlane(mypm.y, l);
-            // the first listed hit still present, or the residue itself on the border (kind 0); -1 when
+            // the first listed hit still present (a residue on the border lists itself, kind 0); -1 when
             // there is none (no choice encodes to -1: pixels stay below 2^30 - 1). Scalar selects, no branch.
             const int c0 = __builtin_amdgcn_readlane(myc.x, l), c1 = __builtin_amdgcn_readlane(myc.y, l);
             const int c2 = __builtin_amdgcn_readlane(myc.z, l), c3 = __builtin_amdgcn_readlane(myc.w, l);
+            // (instruction order, held by scheduling barriers: the five reads first, so that their
+            // latencies overlap, then the vector compares that need only p, then the scalar selects,
+            // then the compares that need the choice, and the mask updates last)
+            __builtin_amdgcn_sched_barrier(0);
+            const unsigned long long e0 = __ballot(cpix[0] == p), e1 = __ballot(cpix[1] == p);
+            const unsigned long long e2 = __ballot(cpix[2] == p), e3 = __ballot(cpix[3] == p);
+            __builtin_amdgcn_sched_barrier(0);
             int chosen = (h3 & bit) ? c3 : -1;
             chosen = (h2 & bit) ? c2 : chosen;
             chosen = (h1 & bit) ? c1 : chosen;
             chosen = (h0 & bit) ? c0 : chosen;
-            chosen = (meta & GS_META_SELF) ? p : chosen;
             bool found = chosen != -1;
-            if (!found) {
+            if (__builtin_expect(!found, 0)) {
+                const int meta = __builtin_amdgcn_readlane(mypm.y, l);
                 if (meta & (GS_META_TERM | GS_META_EXHAUSTED)) continue;  // nothing within reach: unbalanced
                 // every listed charge is gone: scan again on the current charges
                 if (lane < jb) {
@@ -378,11 +386,15 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
             // the masks follow the two removed charges (pixels are >= 0; -2 never matches, and
             // cpix[k] = -1 marks a hit that no commit can remove)
             const int pix = ((unsigned)chosen >> 30) == 3u ? (chosen & 0x3fffffff) : -2;
-            live &= ~__ballot(mypm.x == pix);
-            h0 &= ~(__ballot(cpix[0] == p) | __ballot(cpix[0] == pix));
-            h1 &= ~(__ballot(cpix[1] == p) | __ballot(cpix[1] == pix));
-            h2 &= ~(__ballot(cpix[2] == p) | __ballot(cpix[2] == pix));
-            h3 &= ~(__ballot(cpix[3] == p) | __ballot(cpix[3] == pix));
+            const unsigned long long el = __ballot(mypm.x == pix);
+            const unsigned long long f0 = __ballot(cpix[0] == pix), f1 = __ballot(cpix[1] == pix);
+            const unsigned long long f2 = __ballot(cpix[2] == pix), f3 = __ballot(cpix[3] == pix);
+            __builtin_amdgcn_sched_barrier(0);
+            live &= ~el;
+            h0 &= ~(e0 | f0);
+            h1 &= ~(e1 | f1);
+            h2 &= ~(e2 | f2);
+            h3 &= ~(e3 | f3);
         }
         // the batch's pairs, in commit order, and its charges cleared for the later batches
         if (lane < jb) {
