@@ -132,6 +132,7 @@ constexpr int GS_K = 4;
 constexpr int GS_META_SELF = 8;        // residue on the border: balanced by a one-pixel cut
 constexpr int GS_META_TERM = 16;       // the list ends with a border point
 constexpr int GS_META_EXHAUSTED = 32;  // the scan reached the largest box without GS_K hits or a border
+constexpr int GS_META_NEG = 64;        // the residue's charge is -1
 
 // Residues per block of 1024 pixels.
 __global__ __launch_bounds__(1024) void k_gs_count(const int8_t* __restrict__ res, long long n, int nblk,
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(1024) void k_gs_cand(const int8_t* __restrict__ res
         if (!stop) meta |= GS_META_EXHAUSTED;
     }
     const long long o = map * n + rank;
-    rec_pm[o] = make_int2((int)p, meta | cnt);
+    rec_pm[o] = make_int2((int)p, meta | cnt | (q < 0 ? GS_META_NEG : 0));
     rec_c[o] = make_int4(cand[0], cand[1], cand[2], cand[3]);
 }
 
@@ -321,15 +322,25 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
                     if (((unsigned)pb >> 30) == 3u) pr.clear(pb & 0x3fffffff);
                 }
                 if constexpr (!LDSW) __builtin_amdgcn_s_waitcnt(0);
-                const int q = r[p];
+                const int q = (meta & GS_META_NEG) ? -1 : 1;
                 const int i = p % W, j = p / W;
                 // The box rings 1, 2, ... (radius below R) in scan order, 64 positions per round: each
                 // lane takes the next position (ring lr, index lq on that ring), several rings per round
-                // while they are short; the first hit in scan order is the lowest lane with a hit.
+                // while they are short; the first hit in scan order is the lowest lane with a hit. Rings
+                // that no border clips have 8 r positions, so a first round 4 or more pixels from every
+                // border takes rings 1-3 and the first 16 positions of ring 4 with a fixed lane layout.
+                const bool inner = R > 4 && i >= 4 && i <= W - 5 && j >= 4 && j <= H - 5;
                 int rc = 1, off0 = 0;  // next position to scan: ring rc, index off0
                 while (!found && rc < R) {
                     int lr = R, lq = 0;  // lanes past the last ring scan nothing
                     int start = 0, rr = rc, off = off0;
+                    if (rc == 1 && inner) {
+                        lr = lane < 8 ? 1 : (lane < 24 ? 2 : (lane < 48 ? 3 : 4));
+                        lq = lane - (lane < 8 ? 0 : (lane < 24 ? 8 : (lane < 48 ? 24 : 48)));
+                        start = 64;
+                        rr = 4;
+                        off = 16;
+                    }
                     while (start < 64 && rr < R) {
                         const int P = 2 * (min(j + rr, H - 1) - max(j - rr, 0) + 1) +
                                       2 * (min(i + rr, W - 1) - max(i - rr, 0) - 1);
