@@ -237,9 +237,9 @@ __device__ __forceinline__ int sel4(const int4& v, int k) { return k == 0 ? v.x 
 // those flags are kept as wave-wide lane masks (one for the residues, one per list position), so a
 // residue's choice is a few scalar bit tests, and after each commit the masks lose the lanes whose
 // residue or listed hit that commit removed (vector compares of the two removed pixels against every
-// lane's pixels). The batch's commits collect in two registers (lane j: the j-th residue and its
-// choice) and are written, and their charges cleared, once at the end of the batch, so the loop
-// itself issues no memory operation. A residue whose listed charges are all gone clears the batch's
+// lane's pixels). Each committing lane keeps its choice in a register and a mask records which lanes
+// committed; the pairs are written (in commit order, which is lane order) and their charges cleared
+// once at the end of the batch, so the loop itself issues no memory operation. A residue whose listed charges are all gone clears the batch's
 // charges so far (clearing twice is harmless) and scans again on the current charges (the signs of
 // the charges it meets come from the residue map).
 template <bool LDSW>
@@ -291,7 +291,10 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
         const bool rpres = pr.at(mypm.x);
         unsigned long long live = __ballot(in && rpres);
         unsigned long long h0 = __ballot(hv[0]), h1 = __ballot(hv[1]), h2 = __ballot(hv[2]), h3 = __ballot(hv[3]);
-        int pa = 0, pb = 0, jb = 0;  // this batch's commits: lane j holds the j-th residue and its choice
+        // this batch's commits: the lanes whose residue committed (commits go in lane order) and, in each
+        // such lane, its choice
+        unsigned long long done = 0;
+        int mych = 0;
         while (live) {
             const int l = __ffsll((long long)live) - 1;
             const unsigned long long bit = 1ull << l;
@@ -317,9 +320,9 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
                 const int meta = __builtin_amdgcn_readlane(mypm.y, l);
                 if (meta & (GS_META_TERM | GS_META_EXHAUSTED)) continue;  // nothing within reach: unbalanced
                 // every listed charge is gone: scan again on the current charges
-                if (lane < jb) {
-                    pr.clear(pa);
-                    if (((unsigned)pb >> 30) == 3u) pr.clear(pb & 0x3fffffff);
+                if ((done >> lane) & 1ull) {
+                    pr.clear(mypm.x);
+                    if (((unsigned)mych >> 30) == 3u) pr.clear(mych & 0x3fffffff);
                 }
                 if constexpr (!LDSW) __builtin_amdgcn_s_waitcnt(0);
                 const int q = (meta & GS_META_NEG) ? -1 : 1;
@@ -391,9 +394,8 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
                 }
                 if (!found) continue;  // nothing within reach: the residue stays unbalanced
             }
-            pa = lane == jb ? p : pa;
-            pb = lane == jb ? chosen : pb;
-            ++jb;
+            mych = lane == l ? chosen : mych;
+            done |= bit;
             // the masks follow the two removed charges (pixels are >= 0; -2 never matches, and
             // cpix[k] = -1 marks a hit that no commit can remove)
             const int pix = ((unsigned)chosen >> 30) == 3u ? (chosen & 0x3fffffff) : -2;
@@ -408,13 +410,15 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
             h3 &= ~(e3 | f3);
         }
         // the batch's pairs, in commit order, and its charges cleared for the later batches
-        if (lane < jb) {
-            out[emitted + lane] = make_int2(pa, pb);
-            pr.clear(pa);
-            if (((unsigned)pb >> 30) == 3u) pr.clear(pb & 0x3fffffff);
+        if ((done >> lane) & 1ull) {
+            const int slot = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(done >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((unsigned)done, 0u));
+            out[emitted + slot] = make_int2(mypm.x, mych);
+            pr.clear(mypm.x);
+            if (((unsigned)mych >> 30) == 3u) pr.clear(mych & 0x3fffffff);
         }
         if constexpr (!LDSW) __builtin_amdgcn_s_waitcnt(0);
-        emitted += jb;
+        emitted += __popcll(done);
     }
     if (lane == 0) npairs[blockIdx.x] = emitted;
 }
