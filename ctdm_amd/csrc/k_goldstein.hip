@@ -577,7 +577,7 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lev(const uint8_t* __restrict__
                                                      int* __restrict__ qb, int* __restrict__ maxdepth)
 {
     extern __shared__ unsigned sm[];
-    __shared__ int cnt[2];
+    __shared__ int cnt[3];  // entries of levels d, d + 1, d + 2 (mod 3)
     const long long n = (long long)H * W;
     const int nb = (int)((n + 31) / 32);
     unsigned* blk = sm;  // cut or visited
@@ -609,14 +609,17 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lev(const uint8_t* __restrict__
             cnt[0] = 1;
         }
         cnt[1] = 0;
+        cnt[2] = 0;
     }
     __syncthreads();
     const int dx4[4] = {1, -1, 0, 0}, dy4[4] = {0, 0, 1, -1};
     int d = 0;
-    for (;; ++d) {
-        const int cur = d & 1, nxt = cur ^ 1;
-        const int nc = cnt[cur];
+    for (;; ++d) {  // one barrier per level
+        const int cur = d & 1, nxt = cur ^ 1;                           // queues
+        const int c0 = d % 3, c1 = (d + 1) % 3, c2 = (d + 2) % 3;       // counters
+        const int nc = cnt[c0];
         if (nc == 0) break;
+        if (threadIdx.x == 0) cnt[c2] = 0;  // level d + 2 appends there only after this pass's barrier
         for (int e = threadIdx.x; e < 4 * nc; e += blockDim.x) {  // one (pixel, direction) per thread
             const int p = qget(cur, e >> 2), t = e & 3;
             const int nx = p % W + dx4[t], ny = p / W + dy4[t];
@@ -626,10 +629,8 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lev(const uint8_t* __restrict__
             if (blk[q >> 5] & bit) continue;
             if (atomicOr(blk + (q >> 5), bit) & bit) continue;  // cut, or claimed by another thread
             lev[q] = d + 1;
-            qput(nxt, reserve_slot(&cnt[nxt]), q);
+            qput(nxt, reserve_slot(&cnt[c1]), q);
         }
-        __syncthreads();
-        if (threadIdx.x == 0) cnt[cur] = 0;  // the level after next appends there, after the next barrier
         __syncthreads();
     }
     if (threadIdx.x == 0 && d > 1) atomicMax(maxdepth, d - 1);  // longest parent chain of the map
