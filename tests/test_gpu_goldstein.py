@@ -88,6 +88,14 @@ def test_goldstein_wide_and_tall_maps_level_path(ctx, orc):
         print(f"{shape}: GPU {tg * 1e3:.1f} ms, oracle {tc:.2f} s, residue density {(rr != 0).mean():.3f}")
 
 
+def test_goldstein_fill_queue_spill(ctx, orc):
+    """A 1264 x 1020 map: its cut bits (161 160 B) leave about 2 KiB of 160 KiB of LDS for the flood fill's
+    two level queues (327 entries each), so most levels keep the rest of their frontier in global memory."""
+    w = ctdm_amd.synth_wrapped(10, 1, 1264, 1020)
+    tg, tc, rr = _check(ctx, orc, w)
+    print(f"1 x 1264 x 1020: GPU {tg * 1e3:.1f} ms, oracle {tc:.2f} s, residue density {(rr != 0).mean():.3f}")
+
+
 def test_goldstein_no_residues(ctx, orc):
     """A smooth ramp has no residues: no cuts, every pixel reached, unwrapped equal to the ramp."""
     H, W = 48, 70
