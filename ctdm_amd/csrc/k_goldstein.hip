@@ -605,7 +605,7 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lev(const uint8_t* __restrict__
             const int s0 = sx + W * sy;
             blk[s0 >> 5] |= 1u << (s0 & 31);
             lev[s0] = 0;
-            qput(0, 0, s0);
+            qput(0, 0, (int)((unsigned)sx | ((unsigned)sy << 16)));
             cnt[0] = 1;
         }
         cnt[1] = 0;
@@ -621,15 +621,16 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lev(const uint8_t* __restrict__
         if (nc == 0) break;
         if (threadIdx.x == 0) cnt[c2] = 0;  // level d + 2 appends there only after this pass's barrier
         for (int e = threadIdx.x; e < 4 * nc; e += blockDim.x) {  // one (pixel, direction) per thread
-            const int p = qget(cur, e >> 2), t = e & 3;
-            const int nx = p % W + dx4[t], ny = p / W + dy4[t];
+            const unsigned v = (unsigned)qget(cur, e >> 2);  // x | y << 16
+            const int t = e & 3;
+            const int nx = (int)(v & 0xffffu) + dx4[t], ny = (int)(v >> 16) + dy4[t];
             if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
             const int q = nx + W * ny;
             const unsigned bit = 1u << (q & 31);
             if (blk[q >> 5] & bit) continue;
             if (atomicOr(blk + (q >> 5), bit) & bit) continue;  // cut, or claimed by another thread
             lev[q] = d + 1;
-            qput(nxt, reserve_slot(&cnt[c1]), q);
+            qput(nxt, reserve_slot(&cnt[c1]), (int)((unsigned)nx | ((unsigned)ny << 16)));
         }
         __syncthreads();
     }
@@ -797,7 +798,8 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
                        pairs, npairs, H, W, ct);
     CTDM_CHECK_LAUNCH("k_gs_draw");
     const size_t bits = sizeof(unsigned) * (size_t)nb;  // cut-or-visited bits
-    if (bits + 64 + 2 * sizeof(int) * 256 <= lds_max) {  // the queues get what LDS has left (>= 256 each)
+    // the queues get what LDS has left (>= 256 each); their entries pack x and y in 16 bits each
+    if (bits + 64 + 2 * sizeof(int) * 256 <= lds_max && W < 65536 && H < 65536) {
         // two-stage fill: levels by the search and parent directions from them, 2 pi counts by pointer jumping
         const int Q = (int)std::min<long long>(n, (long long)((lds_max - 64 - bits) / (2 * sizeof(int))));
         uint8_t* par = (uint8_t*)c.buf("gs.par", (size_t)tot);
