@@ -311,10 +311,24 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
             const unsigned long long e0 = __ballot(cpix[0] == p), e1 = __ballot(cpix[1] == p);
             const unsigned long long e2 = __ballot(cpix[2] == p), e3 = __ballot(cpix[3] == p);
             __builtin_amdgcn_sched_barrier(0);
-            int chosen = (h3 & bit) ? c3 : -1;
-            chosen = (h2 & bit) ? c2 : chosen;
-            chosen = (h1 & bit) ? c1 : chosen;
-            chosen = (h0 & bit) ? c0 : chosen;
+            // The selects as scalar-ALU assembly: s_and_b64 sets SCC to (mask & bit) != 0, which
+            // s_cselect_b32 reads directly (compiled code inserts an s_cmp before each select). Scalar
+            // registers only, no memory access; SCC is declared clobbered.
+            int chosen;
+            unsigned long long tsel;
+            asm volatile(
+                "s_and_b64 %[t], %[b], %[h3]\n\t"
+                "s_cselect_b32 %[ch], %[c3], -1\n\t"
+                "s_and_b64 %[t], %[b], %[h2]\n\t"
+                "s_cselect_b32 %[ch], %[c2], %[ch]\n\t"
+                "s_and_b64 %[t], %[b], %[h1]\n\t"
+                "s_cselect_b32 %[ch], %[c1], %[ch]\n\t"
+                "s_and_b64 %[t], %[b], %[h0]\n\t"
+                "s_cselect_b32 %[ch], %[c0], %[ch]"
+                : [ch] "=&s"(chosen), [t] "=&s"(tsel)
+                : [b] "s"(bit), [h0] "s"(h0), [h1] "s"(h1), [h2] "s"(h2), [h3] "s"(h3), [c0] "s"(c0), [c1] "s"(c1),
+                  [c2] "s"(c2), [c3] "s"(c3)
+                : "scc");
             bool found = chosen != -1;
             if (__builtin_expect(!found, 0)) {
                 const int meta = __builtin_amdgcn_readlane(mypm.y, l);
