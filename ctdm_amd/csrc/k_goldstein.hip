@@ -6,10 +6,10 @@
 //                     double promotions, summed in float, thresholded at +-6.
 //   2. branch cuts, deterministic raster order (the reference runs its search for all residues
 //      concurrently, which is racy; the raster order makes it reproducible). Three phases:
-//      k_gs_cand lists every residue's first hits on the initial charges in parallel, k_gs_commit (one
-//      wavefront per map, presence at 1 bit per pixel, in LDS when it fits, else in global memory)
-//      walks the residues in order and takes the first listed hit still present, and k_gs_draw draws
-//      the chosen cuts in parallel.
+//      k_gs_rank + k_gs_cand list every residue's first hits on the initial charges in parallel (one
+//      thread per residue), k_gs_commit (one wavefront per map, presence at 1 bit per pixel, in LDS when
+//      it fits, else in global memory) walks the residues in order and takes the first listed hit still
+//      present, and k_gs_draw draws the chosen cuts in parallel.
 //   3. flood fill: level-synchronous breadth-first search over non-cut pixels from a fixed start; each
 //                     new pixel takes k from its first neighbour of the previous level in the order
 //                     right, left, below, above. Maps whose cut bits fit in LDS (up to 1.3 M pixels):
@@ -166,19 +166,15 @@ __global__ void k_gs_scan(int* __restrict__ blkcnt, int nblk, int batch, int* __
     nres[map] = acc;
 }
 
-// Phase 1: one thread per pixel; residues write their record at their raster rank. A listed hit is
-// encoded as kind << 30 | target pixel (kind 1: column border, cut along the row; 2: row border, cut
-// along the column; 3: opposite charge), kind 0 with the residue's own pixel for a border residue.
-__global__ __launch_bounds__(1024) void k_gs_cand(const int8_t* __restrict__ res, int H, int W, int nblk,
-                                                  const int* __restrict__ blkoff, int2* __restrict__ rec_pm,
-                                                  int4* __restrict__ rec_c)
+// Phase 1a: one thread per pixel; each residue writes its pixel at its raster rank (rec_pm.x).
+__global__ __launch_bounds__(1024) void k_gs_rank(const int8_t* __restrict__ res, int H, int W, int nblk,
+                                                  const int* __restrict__ blkoff, int2* __restrict__ rec_pm)
 {
     __shared__ int wc[16];
     const long long n = (long long)H * W;
     const long long map = blockIdx.x / nblk, blk = blockIdx.x % nblk;
     const long long p = blk * 1024 + threadIdx.x;
-    const int8_t* r = res + map * n;
-    const int q = p < n ? r[p] : 0;
+    const int q = p < n ? res[map * n + p] : 0;
     const unsigned long long m = __ballot(q != 0);
     const int lane = threadIdx.x & 63;
     if (lane == 0) wc[threadIdx.x >> 6] = __popcll(m);
@@ -186,6 +182,24 @@ __global__ __launch_bounds__(1024) void k_gs_cand(const int8_t* __restrict__ res
     if (q == 0) return;
     int rank = blkoff[blockIdx.x] + __popcll(m & ((1ull << lane) - 1ull));
     for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) rank += wc[w];
+    rec_pm[map * n + rank] = make_int2((int)p, 0);
+}
+
+// Phase 1b: one thread per residue (raster rank), so that the scans fill whole wavefronts; the record
+// gets the residue's list. A listed hit is encoded as kind << 30 | target pixel (kind 1: column border,
+// cut along the row; 2: row border, cut along the column; 3: opposite charge), kind 0 with the
+// residue's own pixel for a border residue.
+__global__ __launch_bounds__(1024) void k_gs_cand(const int8_t* __restrict__ res, int H, int W, int nblk,
+                                                  const int* __restrict__ nres, int2* __restrict__ rec_pm,
+                                                  int4* __restrict__ rec_c)
+{
+    const long long n = (long long)H * W;
+    const long long map = blockIdx.x / nblk, rank = (blockIdx.x % nblk) * 1024 + threadIdx.x;
+    if (rank >= nres[map]) return;
+    const long long o = map * n + rank;
+    const int8_t* r = res + map * n;
+    const long long p = rec_pm[o].x;
+    const int q = r[p];
     const int i = (int)(p % W), j = (int)(p / W);
     int cand[GS_K] = {0, 0, 0, 0};
     int cnt = 0, meta = 0;
@@ -225,7 +239,6 @@ __global__ __launch_bounds__(1024) void k_gs_cand(const int8_t* __restrict__ res
         }
         if (!stop) meta |= GS_META_EXHAUSTED;
     }
-    const long long o = map * n + rank;
     rec_pm[o] = make_int2((int)p, meta | cnt | (q < 0 ? GS_META_NEG : 0));
     rec_c[o] = make_int4(cand[0], cand[1], cand[2], cand[3]);
 }
@@ -797,7 +810,9 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
     int2* pairs = (int2*)c.buf("gs.pairs", sizeof(int2) * (size_t)tot);
     hipLaunchKernelGGL(k_gs_count, dim3((unsigned)(nblk * batch)), dim3(1024), 0, c.stream, r, n, nblk, blk);
     hipLaunchKernelGGL(k_gs_scan, dim3((batch + 63) / 64), dim3(64), 0, c.stream, blk, nblk, batch, nres);
-    hipLaunchKernelGGL(k_gs_cand, dim3((unsigned)(nblk * batch)), dim3(1024), 0, c.stream, r, H, W, nblk, blk,
+    hipLaunchKernelGGL(k_gs_rank, dim3((unsigned)(nblk * batch)), dim3(1024), 0, c.stream, r, H, W, nblk, blk,
+                       rec_pm);
+    hipLaunchKernelGGL(k_gs_cand, dim3((unsigned)(nblk * batch)), dim3(1024), 0, c.stream, r, H, W, nblk, nres,
                        rec_pm, rec_c);
     if (commit_lds <= lds_max) {
         hipLaunchKernelGGL(k_gs_commit<true>, dim3(batch), dim3(64), commit_lds, c.stream, r, H, W, nres, rec_pm,
