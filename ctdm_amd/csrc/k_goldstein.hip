@@ -717,24 +717,35 @@ __global__ void k_gs_ptr_init(const float* __restrict__ phi, const uint8_t* __re
     pr[g] = v;
 }
 
-// One pointer-jumping round, in place. Pairs are read and written as single 64-bit accesses, so every
-// read sees a consistent pair: an update keeps k(p) = sum(p) + k(ptr(p)) whichever round's pair of the
-// parent it reads, and a newer one only shortens the chain. Round r is needed only while 2^r is below
-// the longest chain (maxdepth, from the search); later rounds return at once.
-__global__ __launch_bounds__(256) void k_gs_ptr_jump(int2* __restrict__ pr, long long tot, long long n, int round,
+// Pointer jumping, in place: each pixel follows its pointer GS_HOPS times, reading the pairs as they are,
+// and keeps the last pointer with the summed steps. Pairs are read and written as single 64-bit accesses,
+// so every read sees a consistent pair: an update keeps k(p) = sum(p) + k(ptr(p)) whichever version of a
+// pair it reads, and a newer version only shortens the chain. A launch multiplies every pointer's reach by
+// at least GS_HOPS + 1 (the pairs read within it may be older than the launch's own updates), so launch
+// `step` is needed only while (GS_HOPS + 1)^step is below the longest chain (maxdepth, from the search);
+// later launches return at once.
+constexpr int GS_HOPS = 3;
+__global__ __launch_bounds__(256) void k_gs_ptr_jump(int2* __restrict__ pr, long long tot, long long n, int step,
                                                      const int* __restrict__ maxdepth)
 {
-    if ((1LL << round) >= (long long)*maxdepth) return;
+    long long reach = 1;
+    for (int i = 0; i < step; ++i) reach *= GS_HOPS + 1;
+    if (reach >= (long long)*maxdepth) return;
     unsigned long long* a = reinterpret_cast<unsigned long long*>(pr);
     for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < tot;
          g += (long long)gridDim.x * blockDim.x) {
         const unsigned long long mine = __hip_atomic_load(a + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const int P = (int)(unsigned)mine;
+        int P = (int)(unsigned)mine;
         if (P < 0) continue;
         const long long base = g - g % n;
-        const unsigned long long up = __hip_atomic_load(a + base + P, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const int sum = (int)(mine >> 32) + (int)(up >> 32);
-        const unsigned long long nv = (up & 0xffffffffull) | ((unsigned long long)(unsigned)sum << 32);
+        int sum = (int)(mine >> 32);
+#pragma unroll
+        for (int h = 0; h < GS_HOPS; ++h) {  // a reached pixel's pointer is a reached pixel (never negative)
+            const unsigned long long up = __hip_atomic_load(a + base + P, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            sum += (int)(up >> 32);
+            P = (int)(unsigned)up;
+        }
+        const unsigned long long nv = (unsigned long long)(unsigned)P | ((unsigned long long)(unsigned)sum << 32);
         if (nv != mine) __hip_atomic_store(a + g, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -843,11 +854,12 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
         hipLaunchKernelGGL(k_gs_par_lev, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, lev, batch, H, W, par);
         hipLaunchKernelGGL(k_gs_ptr_init, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, par, batch, H, W,
                            pr);
-        // enough rounds for any chain in a map (2^rounds >= n); rounds past the longest chain return at once
-        int rounds = 0;
-        while ((1LL << rounds) < n) ++rounds;
+        // enough launches for any chain in a map ((GS_HOPS + 1)^launches >= n); launches past the longest
+        // chain return at once
+        int launches = 0;
+        for (long long reach = 1; reach < n; reach *= GS_HOPS + 1) ++launches;
         const unsigned jg = (unsigned)std::min<long long>(grid1d(tot, 256), (long long)std::max(c.cus, 1) * 8);
-        for (int r = 0; r < rounds; ++r)
+        for (int r = 0; r < launches; ++r)
             hipLaunchKernelGGL(k_gs_ptr_jump, dim3(jg), dim3(256), 0, c.stream, pr, tot, n, r, (const int*)maxd);
         CTDM_CHECK_LAUNCH("k_gs_ptr_jump");
         hipLaunchKernelGGL(k_gs_finish<true>, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, ct, nullptr,
