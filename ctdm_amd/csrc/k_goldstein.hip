@@ -31,9 +31,12 @@ namespace ctdm {
 static constexpr float kTWO_PI_F = 6.28318530717958647692f;
 static constexpr float kINV_2PI_F = 0.15915494309189533577f;
 
+// fmodf(a, b) for b > 0, then shifted into [0, b). For |a| < 2b, fmodf is a itself or a - sign(a) b,
+// and |a| - b is exact there (b <= |a| <= 2b); other a (wrapped phases far outside [-pi, pi]) take fmodf.
 __device__ __forceinline__ float gs_mod(float a, float b)
 {
-    float r = fmodf(a, b);
+    const float aa = fabsf(a);
+    float r = aa < b ? a : (aa < __fmul_rn(2.f, b) ? copysignf(__fsub_rn(aa, b), a) : fmodf(a, b));
     if (r < 0.f) r = __fadd_rn(r, b);
     return r;
 }
@@ -90,6 +93,19 @@ __device__ __forceinline__ unsigned nonzero_bits(const uint8_t* __restrict__ m, 
 #pragma unroll
         for (int k = 0; k < 32; ++k) v |= (unsigned)(p0 + k < n && q[k] != 0) << k;
     }
+    return v;
+}
+
+// Word w of the bit array of a residue map's negative charges (bit b: pixel 32 w + b; pixels past n: 0).
+__device__ __forceinline__ unsigned negative_bits(const int8_t* __restrict__ m, long long n, int w)
+{
+    const long long p0 = (long long)w * 32;
+    int8_t q[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) q[k] = m[min(p0 + k, n - 1)];
+    unsigned v = 0;
+#pragma unroll
+    for (int k = 0; k < 32; ++k) v |= (unsigned)(p0 + k < n && q[k] < 0) << k;
     return v;
 }
 
@@ -259,7 +275,7 @@ template <bool LDSW>
 __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res, int H, int W,
                                                   const int* __restrict__ nres, const int2* __restrict__ rec_pm,
                                                   const int4* __restrict__ rec_c, unsigned* __restrict__ gwords,
-                                                  int2* __restrict__ pairs, int* __restrict__ npairs)
+                                                  int sign_lds, int2* __restrict__ pairs, int* __restrict__ npairs)
 {
     extern __shared__ unsigned lwords[];
     const int lane = threadIdx.x;
@@ -268,6 +284,10 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
     const Presence<LDSW> pr{LDSW ? lwords : gwords + (long long)blockIdx.x * nw};
     const int8_t* r = res + blockIdx.x * n;
     for (int w = lane; w < nw; w += 64) pr.w[w] = nonzero_bits(reinterpret_cast<const uint8_t*>(r), n, w);
+    // sign bits (1: charge -1) after the presence bits when both fit: rescans then read no residue map
+    const unsigned* sg = lwords + nw;
+    if (sign_lds)
+        for (int w = lane; w < nw; w += 64) lwords[nw + w] = negative_bits(r, n, w);
     if constexpr (!LDSW) __threadfence();
     __syncthreads();
     const int total = nres[blockIdx.x];
@@ -399,7 +419,9 @@ __global__ __launch_bounds__(64) void k_gs_commit(const int8_t* __restrict__ res
                         const int mm = left ? m1 : (mid ? m1 + 1 + (q2 >> 1) : m2);
                         const int nn = left ? n1 + lq : (mid ? ((q2 & 1) ? n2 : n1) : n1 + (q2 - nmid));
                         const int pp = mm + W * nn;
-                        const bool present = pr.at(pp) && r[pp] == -q;
+                        const bool opposite =
+                            sign_lds ? ((int)((sg[pp >> 5] >> (pp & 31)) & 1u) == (q > 0)) : (r[pp] == -q);
+                        const bool present = pr.at(pp) && opposite;
                         if (mm <= 0 || mm >= W - 1) {
                             c = (int)((1u << 30) | (unsigned)(mm + W * j));
                             hit_here = true;
@@ -825,13 +847,14 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
                        rec_pm);
     hipLaunchKernelGGL(k_gs_cand, dim3((unsigned)(nblk * batch)), dim3(1024), 0, c.stream, r, H, W, nblk, nres,
                        rec_pm, rec_c);
-    if (commit_lds <= lds_max) {
-        hipLaunchKernelGGL(k_gs_commit<true>, dim3(batch), dim3(64), commit_lds, c.stream, r, H, W, nres, rec_pm,
-                           rec_c, nullptr, pairs, npairs);
+    if (commit_lds <= lds_max) {  // presence bits in LDS, and the sign bits beside them when they fit
+        const int sign_lds = 2 * commit_lds <= lds_max;
+        hipLaunchKernelGGL(k_gs_commit<true>, dim3(batch), dim3(64), (sign_lds ? 2 : 1) * commit_lds, c.stream, r,
+                           H, W, nres, rec_pm, rec_c, nullptr, sign_lds, pairs, npairs);
     } else {  // a map's presence bits exceed LDS: they stay in global memory
         unsigned* gw = (unsigned*)c.buf("gs.words", sizeof(unsigned) * (size_t)nw * batch);
         hipLaunchKernelGGL(k_gs_commit<false>, dim3(batch), dim3(64), 0, c.stream, r, H, W, nres, rec_pm, rec_c,
-                           gw, pairs, npairs);
+                           gw, 0, pairs, npairs);
     }
     hipLaunchKernelGGL(k_gs_fill_u8, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, ct, tot);
     hipLaunchKernelGGL(k_gs_draw, dim3((unsigned)((n + 255) / 256), (unsigned)batch), dim3(256), 0, c.stream,
