@@ -115,6 +115,14 @@ def test_goldstein_maximum_residue_density(ctx, orc):
     assert (rr != 0).mean() > 0.2
 
 
+def test_goldstein_input_beyond_one_period(ctx, orc):
+    """Inputs that are not wrapped (uniform noise of +-15 rad): neighbour differences exceed two periods,
+    so the residue kernel's wrap takes its general remainder path, and must still match bit for bit."""
+    rng = np.random.default_rng(13)
+    w = rng.uniform(-15.0, 15.0, size=(2, 96, 120)).astype(np.float32)
+    _check(ctx, orc, w)
+
+
 def test_goldstein_thin_and_narrow_maps(ctx, orc):
     """Maps at the border limits: 3 rows or 3 columns, and a 9-wide tall map whose box radius
     (min(901, W/2) = 4) never reaches a border from the middle column, so an isolated vortex there
