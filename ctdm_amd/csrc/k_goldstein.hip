@@ -51,14 +51,25 @@ __device__ __forceinline__ float gs_wrap_step(float a, float b)
 
 __device__ __forceinline__ int gs_step2pi(float d) { return (int)rintf(__fmul_rn(d, kINV_2PI_F)); }
 
+// Per-pixel kernels run on a grid of (column tiles, rows, maps): column, row and map come from the
+// grid, with no division.
+__device__ __forceinline__ void pixel_of_grid(int W, int& x, int& y, long long& map)
+{
+    x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    y = (int)blockIdx.y;
+    map = blockIdx.z;
+}
+
 // ---------------------------------------------------------------- 1. residues
 __global__ void k_gs_residues(const float* __restrict__ phi, int batch, int H, int W, int8_t* __restrict__ res)
 {
     const long long n = (long long)H * W;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n * batch) return;
-    const long long b = g / n;
-    const int p = (int)(g % n), i = p % W, j = p / W;
+    int i, j;
+    long long b;
+    pixel_of_grid(W, i, j, b);
+    if (i >= W) return;
+    const int p = i + W * j;
+    const long long g = b * n + p;
     const float* m = phi + b * n;
     int8_t q = 0;
     if (i < W - 1 && j < H - 1) {
@@ -692,10 +703,13 @@ __global__ __launch_bounds__(1024) void k_gs_bfs_lev(const uint8_t* __restrict__
 __global__ void k_gs_par_lev(const int* __restrict__ level, int batch, int H, int W, uint8_t* __restrict__ par)
 {
     const long long n = (long long)H * W;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n * batch) return;
-    const int* lev = level + (g / n) * n;
-    const int p = (int)(g % n), x = p % W, y = p / W;
+    int x, y;
+    long long b;
+    pixel_of_grid(W, x, y, b);
+    if (x >= W) return;
+    const int p = x + W * y;
+    const long long g = b * n + p;
+    const int* lev = level + b * n;
     const int d = lev[p];
     uint8_t code = 0;
     if (d == 0) {
@@ -724,10 +738,12 @@ __global__ void k_gs_ptr_init(const float* __restrict__ phi, const uint8_t* __re
                               int W, int2* __restrict__ pr)
 {
     const long long n = (long long)H * W;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n * batch) return;
-    const long long b = g / n;
-    const int p = (int)(g % n);
+    int x, y;
+    long long b;
+    pixel_of_grid(W, x, y, b);
+    if (x >= W) return;
+    const int p = x + W * y;
+    const long long g = b * n + p;
     const int code = par[g];
     int2 v = make_int2(-1, 0);
     if (code == 5) {
@@ -747,19 +763,19 @@ __global__ void k_gs_ptr_init(const float* __restrict__ phi, const uint8_t* __re
 // `step` is needed only while (GS_HOPS + 1)^step is below the longest chain (maxdepth, from the search);
 // later launches return at once.
 constexpr int GS_HOPS = 3;
-__global__ __launch_bounds__(256) void k_gs_ptr_jump(int2* __restrict__ pr, long long tot, long long n, int step,
+__global__ __launch_bounds__(256) void k_gs_ptr_jump(int2* __restrict__ pr, long long n, int step,
                                                      const int* __restrict__ maxdepth)
 {
     long long reach = 1;
     for (int i = 0; i < step; ++i) reach *= GS_HOPS + 1;
     if (reach >= (long long)*maxdepth) return;
     unsigned long long* a = reinterpret_cast<unsigned long long*>(pr);
-    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < tot;
-         g += (long long)gridDim.x * blockDim.x) {
+    const long long base = (long long)blockIdx.y * n;  // the grid's y is the map
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const long long g = base + i;
         const unsigned long long mine = __hip_atomic_load(a + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         int P = (int)(unsigned)mine;
         if (P < 0) continue;
-        const long long base = g - g % n;
         int sum = (int)(mine >> 32);
 #pragma unroll
         for (int h = 0; h < GS_HOPS; ++h) {  // a reached pixel's pointer is a reached pixel (never negative)
@@ -780,10 +796,12 @@ __global__ void k_gs_finish(const float* __restrict__ phi, const uint8_t* __rest
                             float* __restrict__ unwrapped)
 {
     const long long n = (long long)H * W;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n * batch) return;
-    const long long b = g / n;
-    const int p = (int)(g % n), x = p % W, y = p / W;
+    int x, y;
+    long long b;
+    pixel_of_grid(W, x, y, b);
+    if (x >= W) return;
+    const int p = x + W * y;
+    const long long g = b * n + p;
     const uint8_t* cut = cuts + b * n;
     const float* ph = phi + b * n;
     auto kat = [&](int q) -> int {
@@ -823,13 +841,15 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
     CTDM_REQUIRE(wrapped && unwrapped, "goldstein: null pointer");
     CTDM_REQUIRE(batch >= 0 && H >= 3 && W >= 3, "goldstein: maps must be at least 3 x 3");
     CTDM_REQUIRE((long long)H * W < (1LL << 30), "goldstein: map too large");
+    CTDM_REQUIRE(H <= 65535 && batch <= 65535, "goldstein: at most 65535 rows and 65535 maps per call");
     if (batch == 0) return;
     const long long n = (long long)H * W, tot = n * batch;
     int8_t* r = res ? res : (int8_t*)c.buf("gs.res", tot);
     uint8_t* ct = cuts ? cuts : (uint8_t*)c.buf("gs.cut", tot);
     int* qa = (int*)c.buf("gs.qa", sizeof(int) * tot);
     int* qb = (int*)c.buf("gs.qb", sizeof(int) * tot);
-    hipLaunchKernelGGL(k_gs_residues, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, batch, H, W, r);
+    const dim3 pgrid((unsigned)((W + 255) / 256), (unsigned)H, (unsigned)batch);  // per-pixel kernels
+    hipLaunchKernelGGL(k_gs_residues, pgrid, dim3(256), 0, c.stream, wrapped, batch, H, W, r);
     const long long nw = (n + 31) / 32, nb = (n + 31) / 32;
     const size_t lds_max = (size_t)(c.max_lds > 0 ? c.max_lds : 64 * 1024);
     // branch cuts in three phases: candidates in parallel, raster-order commit, drawing
@@ -874,26 +894,27 @@ void goldstein(Ctx& c, const float* wrapped, int batch, int H, int W, int8_t* re
         hipLaunchKernelGGL(k_gs_bfs_lev, dim3(batch), dim3(1024), bits + 2 * sizeof(int) * (size_t)Q, c.stream, ct,
                            H, W, Q, lev, qa, qb, maxd);
         CTDM_CHECK_LAUNCH("k_gs_bfs_lev");
-        hipLaunchKernelGGL(k_gs_par_lev, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, lev, batch, H, W, par);
-        hipLaunchKernelGGL(k_gs_ptr_init, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, par, batch, H, W,
-                           pr);
+        hipLaunchKernelGGL(k_gs_par_lev, pgrid, dim3(256), 0, c.stream, lev, batch, H, W, par);
+        hipLaunchKernelGGL(k_gs_ptr_init, pgrid, dim3(256), 0, c.stream, wrapped, par, batch, H, W, pr);
         // enough launches for any chain in a map ((GS_HOPS + 1)^launches >= n); launches past the longest
         // chain return at once
         int launches = 0;
         for (long long reach = 1; reach < n; reach *= GS_HOPS + 1) ++launches;
-        const unsigned jg = (unsigned)std::min<long long>(grid1d(tot, 256), (long long)std::max(c.cus, 1) * 8);
+        // about 8 workgroups per CU in all, split over the maps (the grid's y)
+        const long long per_map = std::max<long long>(1, (long long)std::max(c.cus, 1) * 8 / batch);
+        const dim3 jg((unsigned)std::min<long long>(grid1d(n, 256), per_map), (unsigned)batch);
         for (int r = 0; r < launches; ++r)
-            hipLaunchKernelGGL(k_gs_ptr_jump, dim3(jg), dim3(256), 0, c.stream, pr, tot, n, r, (const int*)maxd);
+            hipLaunchKernelGGL(k_gs_ptr_jump, jg, dim3(256), 0, c.stream, pr, n, r, (const int*)maxd);
         CTDM_CHECK_LAUNCH("k_gs_ptr_jump");
-        hipLaunchKernelGGL(k_gs_finish<true>, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, ct, nullptr,
-                           pr, batch, H, W, k2pi, unwrapped);
+        hipLaunchKernelGGL(k_gs_finish<true>, pgrid, dim3(256), 0, c.stream, wrapped, ct, nullptr, pr, batch, H, W,
+                           k2pi, unwrapped);
     } else {
         int* kk = (int*)c.buf("gs.k", sizeof(int) * tot);
         int* lev = (int*)c.buf("gs.level", sizeof(int) * tot);
         hipLaunchKernelGGL(k_gs_bfs, dim3(batch), dim3(1024), 0, c.stream, wrapped, ct, H, W, lev, kk, qa, qb);
         CTDM_CHECK_LAUNCH("k_gs_bfs");
-        hipLaunchKernelGGL(k_gs_finish<false>, dim3(grid1d(tot, 256)), dim3(256), 0, c.stream, wrapped, ct, kk,
-                           nullptr, batch, H, W, k2pi, unwrapped);
+        hipLaunchKernelGGL(k_gs_finish<false>, pgrid, dim3(256), 0, c.stream, wrapped, ct, kk, nullptr, batch, H, W,
+                           k2pi, unwrapped);
     }
     CTDM_CHECK_LAUNCH("k_gs_finish");
 }

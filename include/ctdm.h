@@ -206,7 +206,8 @@ ctdm_status ctdm_unwrap_dct_adjoint(ctdm_ctx *ctx, const float *wrapped, const f
  *   cuts      uint8 1 on branch-cut pixels
  *   k2pi      int32 integer 2*pi count; INT32_MIN where the flood fill never reached the pixel
  *   unwrapped float wrapped + 2*pi*k2pi, 0 where unreached (as the reference leaves it)
- * Any output pointer except `unwrapped` may be NULL. */
+ * Any output pointer except `unwrapped` may be NULL. Maps of at least 3 x 3 and below 2^30 pixels, at most
+ * 65535 rows and 65535 maps per call; other sizes return CTDM_EINVAL. */
 ctdm_status ctdm_goldstein(ctdm_ctx *ctx, const float *wrapped, int batch, int H, int W,
                            int8_t *residues, uint8_t *cuts, int32_t *k2pi, float *unwrapped);
 
