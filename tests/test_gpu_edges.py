@@ -49,6 +49,8 @@ def test_unsupported_sizes_and_bad_arguments_return_status(ctx):
         "volume side without a kernel": ("ctdm_fft3d", _p(x), 100, -1),
         "fft direction": ("ctdm_fft3d", _p(x), 64, 2),
         "tiny Goldstein map": ("ctdm_goldstein", _p(x), 1, 2, 2, None, None, None, _p(y)),
+        "Goldstein map with 65536 rows": ("ctdm_goldstein", _p(x), 1, 65536, 3, None, None, None, _p(y)),
+        "65536 Goldstein maps in one call": ("ctdm_goldstein", _p(x), 65536, 3, 3, None, None, None, _p(y)),
         "aliased median": ("ctdm_median3x3", _p(x), _p(x), 1, 8, 8),
         "rows band outside": ("ctdm_retrieve_rows", _p(x), _p(x), 1, 256, 256, 250, _p(y), _p(y)),
         "time_pass kind": ("ctdm_time_pass", 3, 64, 0, 1, C.byref(C.c_double()), C.byref(C.c_double())),
